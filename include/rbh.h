@@ -1,0 +1,193 @@
+/*
+ * rbh.h -- C ABI of librbh, the MI355X (gfx950) implementation of the
+ * prox_rigid_bodies time step of RAINBOW (diku-dk/RAINBOW).
+ *
+ * The reference is pure Python; its plug-in point for this path is the
+ * engine's stepper object (rainbow/simulators/prox_rigid_bodies/api.py:42-53,
+ * called by api.simulate at api.py:676-693).  These entry points are what a
+ * ctypes stepper bound to that plug-in point calls; each comment names the
+ * reference code whose work the call takes over.
+ *
+ * Conventions: every function returns 0 on success or a negative RBH_E_*
+ * code; the message of the last failure is rbh_last_error(ctx).  Arrays are
+ * caller-owned host memory, C order, FP64 unless stated.  Body indices are
+ * local to their scene (insertion order of engine.bodies).  A context owns
+ * one HIP stream on one device and is not thread-safe.
+ */
+#ifndef RBH_H
+#define RBH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBH_OK 0
+#define RBH_E_ARG -1        /* invalid argument (ValueError / RuntimeError in the reference API) */
+#define RBH_E_HIP -2        /* HIP runtime failure */
+#define RBH_E_MEMORY -3     /* device memory or HBM budget exhausted */
+#define RBH_E_CAPACITY -4   /* internal capacity exceeded and could not grow */
+#define RBH_E_COLORS -5     /* colouring needed more than rbh_max_colors() colours */
+#define RBH_E_STATE -6      /* call out of order (e.g. contacts of a scene never stepped) */
+
+/* proximal_solver schemes (proximal_contact/prox_solvers.py:6-37) */
+#define RBH_SCHEME_GAUSS_SEIDEL 0          /* serial, canonical contact order (contacts.py:66-107) */
+#define RBH_SCHEME_PARALLEL_GAUSS_SEIDEL 1 /* random-colour order, colours swept in parallel */
+#define RBH_SCHEME_PARALLEL_JACOBI 2       /* jacobi_solver.py:41-57 */
+#define RBH_SCHEME_PARALLEL_JACOBI_HYBRID 3/* jacobi_solver.py:60-79 */
+
+/* friction models */
+#define RBH_FRICTION_ISOTROPIC 0           /* prox_operators.py:14-34 with mu[0] (contacts.py:255) */
+#define RBH_FRICTION_ANISOTROPIC 1         /* proximal_contact/proximal_operators.py:42-139 */
+
+/* rbh_step flags */
+#define RBH_STEP_COUNT_OVERLAPS 1          /* count overlapping triangle pairs (profile key number_of_overlaps) */
+#define RBH_STEP_TIMING 2                  /* record per-stage GPU times for rbh_get_timings */
+#define RBH_STEP_GRAPH 4                   /* capture the launch sequence in a hipGraph and replay it */
+
+typedef struct rbh_ctx rbh_ctx;
+
+/* Numerical parameters of one scene; field for field the reference's
+ * types.Parameters (types.py:356-418) plus the knobs of this build. */
+typedef struct rbh_params {
+  double time_step;                       /* Parameters.time_step */
+  double envelope;                        /* Parameters.envelope */
+  double absolute_tolerance;              /* Parameters.absolute_tolerance */
+  double relative_tolerance;              /* Parameters.relative_tolerance */
+  double nu_reduce;                       /* Parameters.nu_reduce */
+  double nu_increase;                     /* Parameters.nu_increase */
+  double too_small_merit_change;          /* Parameters.too_small_merit_change */
+  double contact_optimization_tolerance;  /* Parameters.contact_optimization_tolerance */
+  double gap_reduction;                   /* Parameters.gap_reduction */
+  double min_gap_value;                   /* Parameters.min_gap_value */
+  double max_gap_value;                   /* Parameters.max_gap_value */
+  double ellipsoid_tolerance;             /* Parameters.ellipsoid_tolerance */
+  double ellipsoid_expansion;             /* Parameters.ellipsoid_expansion */
+  int max_iterations;                     /* Parameters.max_iterations */
+  int contact_optimization_max_iterations;/* Parameters.contact_optimization_max_iterations */
+  int ellipsoid_max_iterations;           /* Parameters.ellipsoid_max_iterations */
+  int use_bounce;                         /* Parameters.use_bounce */
+  int use_pre_stabilization;              /* Parameters.use_pre_stabilization */
+  int proximal_solver;                    /* RBH_SCHEME_* (soft-body Parameters.proximal_solver) */
+  int friction_model;                     /* RBH_FRICTION_* */
+  int pad;
+  unsigned long long coloring_seed;       /* seed of the random colouring */
+} rbh_params;
+
+/* One scene (one reference Engine): bodies in engine.bodies order. */
+typedef struct rbh_scene_desc {
+  int n_bodies;
+  const int* shape;            /* shape id (rbh_shape_upload) per body */
+  const int* fixed;            /* RigidBody.is_fixed */
+  const int* material;         /* index into the interaction table */
+  const double* mass;          /* RigidBody.mass */
+  const double* inertia;       /* 3 per body: RigidBody.inertia (principal, body frame) */
+  const double* gravity;       /* 3 per body: sum over the body's Gravity forces of g * up (types.py:204-221) */
+  const double* damping;       /* 2 per body: sums of Damping alpha and beta (types.py:239-256) */
+  int n_materials;
+  const double* interactions;  /* n_materials^2 x 4: mu[0], mu[1], mu[2], epsilon (types.py:20-53) */
+  rbh_params params;
+} rbh_scene_desc;
+
+/* Per-scene record of the last step (the profile dict keys of steppers.py:95-106,
+ * collision_detection.py:80-83 and gauss_seidel_solver.py:74-101). */
+typedef struct rbh_step_log {
+  int step_index;              /* steps committed so far */
+  int contact_points;          /* len(engine.contact_points) */
+  int iterations;              /* solver 'iterations' */
+  int exitcode;                /* solver 'exitcode' (0 max iterations, 1 absolute, 2 relative) */
+  int rejects;                 /* rejected (rolled back) sweeps */
+  int sweeps;                  /* sweeps executed, rejected ones included */
+  int colors;                  /* colours used by the sweep order */
+  int solved;                  /* 1 if the contact problem was solved this step */
+  unsigned long long number_of_overlaps; /* overlapping triangle pairs (RBH_STEP_COUNT_OVERLAPS) */
+  unsigned long long fw_cell_evals;      /* trilinear cell evaluations in Frank-Wolfe */
+  double natural_residual;     /* ||x - prox(x - r0 (A x + b))||_inf / max(1, ||x||_inf), FP64 */
+  double max_penetration;      /* Contacts.get_largest_penetration */
+  double final_merit;          /* last accepted merit ||x - sol||_inf */
+} rbh_step_log;
+
+typedef struct rbh_device_info {
+  char name[256];
+  int compute_units;
+  int lds_per_block;
+  unsigned long long hbm_bytes;
+  int precision_bytes;
+  int max_colors;
+} rbh_device_info;
+
+/* Library identity; no reference counterpart. */
+int rbh_version(void);
+
+/* Create a context on `device` computing in FP32 (precision_bytes = 4) or
+ * FP64 (8).  hbm_budget_bytes = 0: no budget.  Replaces the engine-level
+ * state the reference keeps in Python objects (types.py:421-445). */
+int rbh_ctx_create(int device, int precision_bytes, unsigned long long hbm_budget_bytes, rbh_ctx** out);
+int rbh_ctx_destroy(rbh_ctx* ctx);
+const char* rbh_last_error(rbh_ctx* ctx);
+int rbh_device_query(rbh_ctx* ctx, rbh_device_info* out);
+/* Device bytes held by the context (arenas + step workspace). */
+int rbh_memory_usage(rbh_ctx* ctx, unsigned long long* bytes);
+
+/* Signed distance of the res I x J x K node grid spanning [min_coord, max_coord]
+ * to the closed mesh (V, T): replaces igl.signed_distance inside
+ * GRID.create_signed_distance (grid3.py:320-339).  values: I*J*K, index
+ * i + I*(j + J*k). */
+int rbh_sdf_build(rbh_ctx* ctx, const double* V, int nv, const int* T, int nt, const double* min_coord,
+                  const double* max_coord, int I, int J, int K, double* values);
+
+/* Upload one shape (types.py:82-108): body-frame mesh and its SDF grid. */
+int rbh_shape_upload(rbh_ctx* ctx, const double* V, int nv, const int* T, int nt, const double* sdf_values, int I,
+                     int J, int K, const double* min_coord, const double* max_coord, int* shape_id);
+
+/* Scenes: device-resident bodies of one engine.  rbh_scene_create takes the
+ * static data StateStorage reads each step (state_storage.py:108-185). */
+int rbh_scene_create(rbh_ctx* ctx, const rbh_scene_desc* desc, int* scene_id);
+int rbh_scene_destroy(rbh_ctx* ctx, int scene_id);
+int rbh_scene_set_params(rbh_ctx* ctx, int scene_id, const rbh_params* params);
+/* Body state: r (3n), q (4n, s x y z), v (3n), w (3n); replaces
+ * StateStorage.copy_from_engine / copy_to_engine (state_storage.py:43-68). */
+int rbh_scene_set_state(rbh_ctx* ctx, int scene_id, const double* r, const double* q, const double* v,
+                        const double* w);
+int rbh_scene_get_state(rbh_ctx* ctx, int scene_id, double* r, double* q, double* v, double* w);
+/* Number of committed steps of the scene; it keys the colouring's random
+ * draws, so a scene restored from a snapshot replays the same colourings.
+ * No reference counterpart (the reference draws from numpy's global RNG). */
+int rbh_scene_set_step_index(rbh_ctx* ctx, int scene_id, int step_index);
+/* Device bytes a scene of this size would take (admission control). */
+int rbh_scene_footprint(rbh_ctx* ctx, const rbh_scene_desc* desc, unsigned long long* bytes);
+
+/* Advance the listed scenes together by n_steps steps of SemiImplicitStepper.step
+ * (steppers.py:47-106): collision detection (collision_detection.py:293-318),
+ * contact problem (contacts.py:31-55), prox solve (gauss_seidel_solver.py:14-101)
+ * and integration (state_storage.py:239-302).  dt > 0 is used for every scene,
+ * dt <= 0 means each scene's time_step. */
+int rbh_step(rbh_ctx* ctx, const int* scene_ids, int n_scenes, int n_steps, double dt, int flags);
+int rbh_sync(rbh_ctx* ctx);
+
+/* Log of the scene's last step; residuals (cap entries) receives the per-sweep
+ * merit array of gauss_seidel_solver.py:31-33,76 (inf past the last sweep). */
+int rbh_get_log(rbh_ctx* ctx, int scene_id, rbh_step_log* log, double* residuals, int cap);
+
+/* Contacts of the scene's last step in the reference's order
+ * (collision_detection.py:204-290); ia = ContactPoint.bodyA (SDF owner),
+ * ib = ContactPoint.bodyB (triangle owner), tri = triangle of ib, p/n (3 each),
+ * gap, impulse (4 each: normal, s, t, torsion; the solved x of contacts.py:57-64),
+ * colour.  Any output pointer may be NULL.  *count receives the total. */
+int rbh_get_contacts(rbh_ctx* ctx, int scene_id, int cap, int* count, int* ia, int* ib, int* tri, double* p,
+                     double* n, double* gap, double* impulse, int* color);
+
+/* GPU milliseconds per stage of the last rbh_step with RBH_STEP_TIMING:
+ * [0] body prep + refit (update_bvh), [1] broad phase + candidates (narrow_phase),
+ * [2] Frank-Wolfe + reduction (contact_determination), [3] assembly,
+ * [4] colouring + order, [5] solve (solver_time), [6] integration,
+ * [7] total (stepper_time).  Returns the number of entries written. */
+int rbh_get_timings(rbh_ctx* ctx, double* ms, int cap);
+
+/* Batch-level counters of the last step: [0] body pairs, [1] candidate slots,
+ * [2] candidate triangles, [3] contacts. */
+int rbh_get_counters(rbh_ctx* ctx, long long* out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

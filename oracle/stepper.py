@@ -1,0 +1,101 @@
+"""One semi-implicit time step, restated from prox_rigid_bodies/steppers.py:47-106.
+
+copy_from_engine (state_storage.py:43-53) -> delta_u = dt W f_ext ->
+collision detection -> contact problem + prox solve -> delta_u += W J^T sol ->
+velocity_update (state_storage.py:239-248) -> position_update (:251-302:
+r += v dt; q <- unit(q + 0.5 dt (0, w) q), free bodies only) -> energies
+(:188-237) and the largest penetration (contacts.py:223-236) for the log.
+
+Post-stabilisation (steppers.py:15-36, off by default) and hinges are outside
+this build's scope; ``step`` raises NotImplementedError when they are enabled.
+"""
+import numpy as np
+
+from . import coloring, collision, problem
+from . import quat as Q
+
+
+def sweep_groups(scene, contacts):
+    prm = scene.params
+    scheme = prm["proximal_solver"]
+    if scheme == "gauss_seidel":
+        return coloring.serial_groups(contacts.K), None
+    C = coloring.color(contacts.ia, contacts.ib, scene.fixed, int(prm["coloring_seed"]),
+                       int(scene.step_index))
+    return coloring.groups_from_colors(C), C
+
+
+def energies(scene):
+    """state_storage.py:188-237 (gravity direction from all engine gravity forces)."""
+    kin, pot = 0.0, 0.0
+    G = np.zeros(3)
+    for g, up in scene.gravity:
+        G = G + up * g
+    for k in range(scene.N):
+        if scene.fixed[k]:
+            continue
+        R = Q.to_matrix(scene.q[k])
+        I_wcs = Q.update_inertia_tensor(R, scene.inertia[k])
+        if scene.gravity:
+            up = G / np.linalg.norm(G)
+            pot += scene.mass[k] * np.linalg.norm(G) * up.dot(scene.r[k])
+        w = scene.w[k]
+        v = scene.v[k]
+        kin += 0.5 * (scene.mass[k] * v.dot(v) + np.dot(w, np.dot(I_wcs, w)))
+    return kin, pot
+
+
+def integrate(scene, delta_u, dt):
+    u = np.concatenate([scene.v, scene.w], axis=1) + delta_u
+    scene.v = u[:, :3].copy()
+    scene.w = u[:, 3:].copy()
+    for k in range(scene.N):
+        if scene.fixed[k]:
+            continue
+        scene.r[k] = scene.r[k] + scene.v[k] * dt
+        q = scene.q[k] + Q.prod(np.array([0.0, *scene.w[k]]), scene.q[k]) * dt * 0.5
+        scene.q[k] = Q.unit(q)
+
+
+def step(scene, dt):
+    """Advance ``scene`` in place by dt.  Returns a dict with the contacts, the
+    solved problem, the colouring and the per-step log."""
+    prm = scene.params
+    if prm.get("use_post_stabilization"):
+        raise NotImplementedError("post-stabilisation is outside this build's scope")
+    Winv, f_ext, du_ext = problem.body_mass_blocks(scene)
+    delta_u = dt * du_ext
+    stats = {}
+    contacts = collision.detect(scene, stats)
+    out = dict(contacts=contacts, problem=None, colors=None, log={})
+    log = out["log"]
+    log["number_of_overlaps"] = stats["number_of_overlaps"]
+    if contacts.K > 0:
+        P = problem.assemble(scene, contacts, Winv, du_ext, dt)
+        groups, C = sweep_groups(scene, contacts)
+        res = problem.solve(P, groups, prm)
+        delta_u = delta_u + problem.body_w(P, P.sol)
+        out.update(problem=P, colors=C, groups=groups)
+        log.update(residuals=res["residuals"], iterations=res["iterations"],
+                   exitcode=res["exitcode"], rejects=res["rejects"])
+        log["natural_residual"] = problem.natural_residual(P, prm)
+    integrate(scene, delta_u, dt)
+    scene.step_index += 1
+    kin, pot = energies(scene)
+    log.update(dt=dt, contact_points=contacts.K, kinetic_energy=kin, potential_energy=pot,
+               max_penetration=-min(0.0, float(contacts.gap.min()) if contacts.K else 0.0))
+    out["delta_u"] = delta_u
+    return out
+
+
+def simulate(scene, T):
+    """api.py:676-693: substeps dt = min(T_left, time_step)."""
+    if T <= 0:
+        raise ValueError("Time must be positive")
+    logs = []
+    T_left = T
+    while T_left:
+        dt = min(T_left, scene.params["time_step"])
+        logs.append(step(scene, dt)["log"])
+        T_left -= dt
+    return logs

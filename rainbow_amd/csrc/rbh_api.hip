@@ -1,0 +1,1068 @@
+// librbh: host runtime and C ABI (include/rbh.h).
+//
+// Device memory model: one context owns
+//   * a shape arena (body-frame vertices, triangles, SDF values),
+//   * a body arena (SoA state + static data + per-step derived data) in which
+//     every scene occupies a contiguous range,
+//   * a triangle-AABB arena (per body instance),
+//   * scene tables (parameters, per-step state, residual history),
+//   * a step workspace sized for the current batch (pairs, candidates,
+//     contacts, colouring, solver arrays), grown on demand.
+// A step over a batch of scenes is one fixed launch sequence; data-dependent
+// sizes stay on the device, so the sequence can be captured into a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstddef>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rbh.h"
+#include "rbh_kernels.h"
+#include "rbh_sdf.h"
+
+using namespace rbh;
+
+namespace {
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct RangeAlloc {
+  std::vector<std::pair<int, int>> free_;   // (offset, size), sorted by offset
+  int cap = 0;
+  void init(int c) { cap = c; free_.clear(); if (c > 0) free_.push_back({0, c}); }
+  int alloc(int n) {
+    for (size_t i = 0; i < free_.size(); ++i) {
+      if (free_[i].second >= n) {
+        int off = free_[i].first;
+        free_[i].first += n;
+        free_[i].second -= n;
+        if (free_[i].second == 0) free_.erase(free_.begin() + i);
+        return off;
+      }
+    }
+    return -1;
+  }
+  void release(int off, int n) {
+    if (n <= 0) return;
+    free_.push_back({off, n});
+    std::sort(free_.begin(), free_.end());
+    std::vector<std::pair<int, int>> m;
+    for (auto& f : free_) {
+      if (!m.empty() && m.back().first + m.back().second == f.first) m.back().second += f.second;
+      else m.push_back(f);
+    }
+    free_ = m;
+  }
+  void grow(int newcap) {
+    if (newcap <= cap) return;
+    release(cap, newcap - cap);
+    cap = newcap;
+  }
+  int used() const {
+    int f = 0;
+    for (auto& x : free_) f += x.second;
+    return cap - f;
+  }
+};
+
+struct SceneHost {
+  bool live = false;
+  int body_off = 0, nbody = 0, tri_off = 0, ntri = 0, mat_off = 0, nmat = 0, res_off = 0, res_n = 0;
+  rbh_params params{};
+  long long ws_gen = -1;   // workspace generation of the scene's last step
+  int batch_index = -1;    // index inside that batch
+};
+
+enum Stage { ST_BEGIN = 0, ST_PREP, ST_BROAD, ST_FW, ST_ASM, ST_COLOR, ST_SOLVE, ST_INT, ST_N };
+
+}  // namespace
+
+struct rbh_ctx {
+  int device = 0, prec = 4;
+  hipStream_t stream = nullptr;
+  unsigned long long budget = 0;
+  std::string err;
+  hipDeviceProp_t prop{};
+  // shapes
+  std::vector<ShapeInfo> shapes;
+  std::vector<double> sV;   // 4 per vertex
+  std::vector<int> sT;      // 4 per triangle
+  std::vector<double> sdf;
+  bool shapes_dirty = true;
+  DBuf d_shapes, d_sV, d_sT, d_sdf;
+  // body arena
+  RangeAlloc bodies, tris, res;
+  DBuf r, q, v, w, b_shape, b_slot, b_fixed, b_mat, b_tri, b_mass, b_inertia, b_grav, b_damp;
+  DBuf W0, W1, du0, du1, lo, hi, ws0, ws1, ww0, ww1;
+  DBuf tlo, thi;
+  // scenes
+  std::vector<SceneHost> scenes;
+  std::vector<SceneParams> sp_host;
+  std::vector<double> mat_host;
+  DBuf d_sp, d_ss, d_mat, d_res;
+  bool scenes_dirty = true;
+  long long arena_gen = 0;
+  // batch workspace
+  std::vector<int> batch;
+  long long batch_arena_gen = -1;
+  int nb = 0, ns = 0;
+  DBuf bb_body, bs_slot, bs_foff, b2f, f_end, d_nb;
+  DBuf row_cnt, row_off, pairs, pair_sz, slot_off, n_pairs, slot_flag, slot_scan, n_slots;
+  DBuf cand_pair, cand_tri, pair_cstart, n_cand, cand_p, cand_n, cand_keep, keep_scan;
+  DBuf c_ia, c_ib, c_tri, c_pair, c_p, c_n, n_contacts, c_rA, c_rB, c_b, c_r0, c_mu, c_imp, c_color;
+  DBuf bc_cnt, bc_off, bc_fill, bc_list, uf, isl_max, pal, c_deg, c_isl, c_done, col_off;
+  DBuf perm, s_id, s_n, s_rA, s_rB, s_b, s_r0, x0, x1, errbuf, scan_part;
+  int cap_pairs = 0, cap_slots = 0, cap_cand = 0, cap_contacts = 0;
+  long long ws_gen = 0;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long graph_key_gen = -1;
+  double graph_dt = 0;
+  int graph_flags = -1;
+  // timing
+  hipEvent_t ev[ST_N]{};
+  double timings[ST_N]{};
+  long long counters[4]{};
+};
+
+namespace {
+
+#define HIPCHK(ctx, call)                                                                       \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                           \
+      return e_ == hipErrorOutOfMemory ? RBH_E_MEMORY : RBH_E_HIP;                              \
+    }                                                                                           \
+  } while (0)
+
+int fail(rbh_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+size_t total_bytes(rbh_ctx* c);
+
+// grow a device buffer; keep the old contents if `keep`
+int ensure(rbh_ctx* c, DBuf& b, size_t bytes, bool keep = false) {
+  if (bytes <= b.bytes && b.p) return 0;
+  size_t nb = std::max<size_t>(bytes, 256);
+  if (c->budget && total_bytes(c) - b.bytes + nb > c->budget)
+    return fail(c, RBH_E_MEMORY, "HBM budget exceeded (" + std::to_string(c->budget) + " bytes)");
+  void* p = nullptr;
+  HIPCHK(c, hipMalloc(&p, nb));
+  if (keep && b.p && b.bytes) HIPCHK(c, hipMemcpyAsync(p, b.p, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (b.p) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(b.p));
+  }
+  b.p = p;
+  b.bytes = nb;
+  return 0;
+}
+
+size_t total_bytes(rbh_ctx* c) {
+  size_t t = 0;
+  DBuf* all[] = {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
+                 &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
+                 &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi,
+                 &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
+                 &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs,
+                 &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
+                 &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->c_ia, &c->c_ib, &c->c_tri,
+                 &c->c_pair, &c->c_p, &c->c_n, &c->n_contacts, &c->c_rA, &c->c_rB, &c->c_b, &c->c_r0, &c->c_mu,
+                 &c->c_imp, &c->c_color, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->uf, &c->isl_max,
+                 &c->pal, &c->c_deg, &c->c_isl, &c->c_done, &c->col_off, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
+                 &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part};
+  for (DBuf* b : all) t += b->bytes;
+  return t;
+}
+
+void free_all(rbh_ctx* c) {
+  DBuf* all[] = {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
+                 &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
+                 &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi,
+                 &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
+                 &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs,
+                 &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
+                 &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->c_ia, &c->c_ib, &c->c_tri,
+                 &c->c_pair, &c->c_p, &c->c_n, &c->n_contacts, &c->c_rA, &c->c_rB, &c->c_b, &c->c_r0, &c->c_mu,
+                 &c->c_imp, &c->c_color, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->uf, &c->isl_max,
+                 &c->pal, &c->c_deg, &c->c_isl, &c->c_done, &c->col_off, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
+                 &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part};
+  for (DBuf* b : all) {
+    if (b->p) (void)hipFree(b->p);
+    b->p = nullptr;
+    b->bytes = 0;
+  }
+}
+
+void drop_graph(rbh_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_key_gen = -1;
+}
+
+// ------------------------------------------------------------ arenas
+int body_elem_sizes(rbh_ctx* c, std::vector<std::pair<DBuf*, size_t>>& out) {
+  const size_t R = c->prec, V = 4 * R;
+  out = {{&c->r, V},      {&c->q, V},       {&c->v, V},         {&c->w, V},        {&c->b_shape, 4},
+         {&c->b_slot, 4}, {&c->b_fixed, 4}, {&c->b_mat, 4},     {&c->b_tri, 4},    {&c->b_mass, R},
+         {&c->b_inertia, V}, {&c->b_grav, V}, {&c->b_damp, V},  {&c->W0, V},       {&c->W1, V},
+         {&c->du0, V},    {&c->du1, V},     {&c->lo, V},        {&c->hi, V},       {&c->ws0, V},
+         {&c->ws1, V},    {&c->ww0, V},     {&c->ww1, V}};
+  return 0;
+}
+
+int grow_bodies(rbh_ctx* c, int need) {
+  int cap = std::max(need, std::max(1024, c->bodies.cap * 2));
+  std::vector<std::pair<DBuf*, size_t>> el;
+  body_elem_sizes(c, el);
+  for (auto& e : el) {
+    int rc = ensure(c, *e.first, (size_t)cap * e.second, true);
+    if (rc) return rc;
+  }
+  c->bodies.grow(cap);
+  c->arena_gen++;
+  drop_graph(c);
+  return 0;
+}
+
+int grow_tris(rbh_ctx* c, int need) {
+  int cap = std::max(need, std::max(1 << 16, c->tris.cap * 2));
+  size_t V = 4 * (size_t)c->prec;
+  int rc = ensure(c, c->tlo, (size_t)cap * V, true);
+  if (!rc) rc = ensure(c, c->thi, (size_t)cap * V, true);
+  if (rc) return rc;
+  c->tris.grow(cap);
+  c->arena_gen++;
+  drop_graph(c);
+  return 0;
+}
+
+int grow_res(rbh_ctx* c, int need) {
+  int cap = std::max(need, std::max(4096, c->res.cap * 2));
+  int rc = ensure(c, c->d_res, (size_t)cap * 8, true);
+  if (rc) return rc;
+  c->res.grow(cap);
+  c->arena_gen++;
+  drop_graph(c);
+  return 0;
+}
+
+template <typename T> int upload(rbh_ctx* c, DBuf& b, const std::vector<T>& h) {
+  int rc = ensure(c, b, std::max<size_t>(h.size() * sizeof(T), 16));
+  if (rc) return rc;
+  if (!h.empty()) HIPCHK(c, hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+template <typename R> int sync_shapes(rbh_ctx* c) {
+  if (!c->shapes_dirty) return 0;
+  std::vector<V4<R>> v(c->sV.size() / 4);
+  for (size_t i = 0; i < v.size(); ++i) v[i] = V4<R>{R(c->sV[4 * i]), R(c->sV[4 * i + 1]), R(c->sV[4 * i + 2]), R(0)};
+  std::vector<R> g(c->sdf.size());
+  for (size_t i = 0; i < g.size(); ++i) g[i] = R(c->sdf[i]);
+  int rc = upload(c, c->d_shapes, c->shapes);
+  if (!rc) rc = upload(c, c->d_sV, v);
+  if (!rc) rc = upload(c, c->d_sT, c->sT);
+  if (!rc) rc = upload(c, c->d_sdf, g);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->shapes_dirty = false;
+  c->arena_gen++;
+  drop_graph(c);
+  return 0;
+}
+
+SceneParams to_device_params(const rbh_params& p, int nmat, int mat_off, int res_off) {
+  SceneParams s{};
+  s.time_step = p.time_step; s.envelope = p.envelope; s.abs_tol = p.absolute_tolerance;
+  s.rel_tol = p.relative_tolerance; s.nu_reduce = p.nu_reduce; s.nu_increase = p.nu_increase;
+  s.too_small = p.too_small_merit_change; s.fw_tol = p.contact_optimization_tolerance;
+  s.gap_reduction = p.gap_reduction; s.min_gap = p.min_gap_value; s.max_gap = p.max_gap_value;
+  s.ell_tol = p.ellipsoid_tolerance; s.ell_expansion = p.ellipsoid_expansion;
+  s.max_iter = p.max_iterations; s.fw_max_iter = p.contact_optimization_max_iterations;
+  s.ell_max_iter = p.ellipsoid_max_iterations; s.use_bounce = p.use_bounce;
+  s.use_pre_stab = p.use_pre_stabilization; s.scheme = p.proximal_solver; s.friction = p.friction_model;
+  s.nmat = nmat; s.mat_off = mat_off; s.res_off = res_off; s.seed = p.coloring_seed;
+  return s;
+}
+
+int sync_scenes(rbh_ctx* c) {
+  if (!c->scenes_dirty) return 0;
+  size_t n = c->scenes.size();
+  int rc = upload(c, c->d_sp, c->sp_host);
+  if (!rc) rc = upload(c, c->d_mat, c->mat_host);
+  if (!rc) rc = ensure(c, c->d_ss, std::max<size_t>(n, 1) * sizeof(SceneState), true);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->scenes_dirty = false;
+  return 0;
+}
+
+int check_params(rbh_ctx* c, const rbh_params& p) {
+  if (!(p.max_iterations >= 0) || p.contact_optimization_max_iterations < 0 || p.ellipsoid_max_iterations < 0)
+    return fail(c, RBH_E_ARG, "iteration counts must be non-negative");
+  if (p.proximal_solver < 0 || p.proximal_solver > 3)
+    return fail(c, RBH_E_ARG, "Unknown solver scheme: " + std::to_string(p.proximal_solver));
+  if (p.friction_model < 0 || p.friction_model > 1) return fail(c, RBH_E_ARG, "unknown friction model");
+  if (!(p.envelope >= 0.0)) return fail(c, RBH_E_ARG, "envelope must be non-negative");
+  return 0;
+}
+
+// ------------------------------------------------------------ batch workspace
+template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags) {
+  Dev<R> d{};
+#define P(T, b) reinterpret_cast<T>(c->b.p)
+  d.shapes = P(const ShapeInfo*, d_shapes); d.sV = P(const V4<R>*, d_sV); d.sT = P(const int4*, d_sT);
+  d.sdf = P(const R*, d_sdf);
+  d.sp = P(const SceneParams*, d_sp); d.ss = P(SceneState*, d_ss); d.residuals = P(double*, d_res);
+  d.mat = P(const double*, d_mat);
+  d.r = P(V4<R>*, r); d.q = P(V4<R>*, q); d.v = P(V4<R>*, v); d.w = P(V4<R>*, w);
+  d.b_shape = P(const int*, b_shape); d.b_slot = P(const int*, b_slot); d.b_fixed = P(const int*, b_fixed);
+  d.b_mat = P(const int*, b_mat); d.b_tri = P(const int*, b_tri); d.b_mass = P(const R*, b_mass);
+  d.b_inertia = P(const V4<R>*, b_inertia); d.b_grav = P(const V4<R>*, b_grav); d.b_damp = P(const V4<R>*, b_damp);
+  d.b_W0 = P(V4<R>*, W0); d.b_W1 = P(V4<R>*, W1); d.b_du0 = P(V4<R>*, du0); d.b_du1 = P(V4<R>*, du1);
+  d.b_lo = P(V4<R>*, lo); d.b_hi = P(V4<R>*, hi); d.ws0 = P(V4<R>*, ws0); d.ws1 = P(V4<R>*, ws1);
+  d.ww0 = P(V4<R>*, ww0); d.ww1 = P(V4<R>*, ww1); d.tlo = P(V4<R>*, tlo); d.thi = P(V4<R>*, thi);
+  d.nb = c->nb; d.ns = c->ns;
+  d.bb_body = P(const int*, bb_body); d.bs_slot = P(const int*, bs_slot); d.bs_foff = P(const int*, bs_foff);
+  d.b2f = P(const int*, b2f); d.f_end = P(const int*, f_end); d.d_nb = P(const int*, d_nb);
+  d.row_cnt = P(int*, row_cnt); d.row_off = P(int*, row_off);
+  d.pairs = P(int2*, pairs); d.pair_sz = P(int*, pair_sz); d.slot_off = P(int*, slot_off);
+  d.n_pairs = P(int*, n_pairs); d.cap_pairs = c->cap_pairs;
+  d.slot_flag = P(unsigned char*, slot_flag); d.slot_scan = P(int*, slot_scan); d.n_slots = P(int*, n_slots);
+  d.cap_slots = c->cap_slots;
+  d.cand_pair = P(int*, cand_pair); d.cand_tri = P(int*, cand_tri); d.pair_cstart = P(int*, pair_cstart);
+  d.n_cand = P(int*, n_cand); d.cap_cand = c->cap_cand;
+  d.cand_p = P(V4<R>*, cand_p); d.cand_n = P(V4<R>*, cand_n); d.cand_keep = P(unsigned char*, cand_keep);
+  d.keep_scan = P(int*, keep_scan);
+  d.c_ia = P(int*, c_ia); d.c_ib = P(int*, c_ib); d.c_tri = P(int*, c_tri); d.c_pair = P(int*, c_pair);
+  d.c_p = P(V4<R>*, c_p); d.c_n = P(V4<R>*, c_n); d.n_contacts = P(int*, n_contacts);
+  d.cap_contacts = c->cap_contacts;
+  d.c_rA = P(V4<R>*, c_rA); d.c_rB = P(V4<R>*, c_rB); d.c_b = P(V4<R>*, c_b); d.c_r0 = P(V4<R>*, c_r0);
+  d.c_mu = P(V4<R>*, c_mu); d.c_imp = P(V4<R>*, c_imp); d.c_color = P(int*, c_color);
+  d.bc_cnt = P(int*, bc_cnt); d.bc_off = P(int*, bc_off); d.bc_fill = P(int*, bc_fill);
+  d.bc_list = P(int*, bc_list); d.uf = P(int*, uf); d.isl_max = P(int*, isl_max); d.pal = P(unsigned*, pal);
+  d.c_deg = P(int*, c_deg); d.c_isl = P(int*, c_isl); d.c_done = P(unsigned char*, c_done);
+  d.col_off = P(int*, col_off);
+  d.perm = P(int*, perm); d.s_id = P(int2*, s_id); d.s_n = P(V4<R>*, s_n); d.s_rA = P(V4<R>*, s_rA);
+  d.s_rB = P(V4<R>*, s_rB); d.s_b = P(V4<R>*, s_b); d.s_r0 = P(V4<R>*, s_r0); d.x0 = P(V4<R>*, x0);
+  d.x1 = P(V4<R>*, x1);
+  d.err = P(int*, errbuf);
+  d.dt = dt;
+  d.flags = flags;
+#undef P
+  return d;
+}
+
+int scan_parts(int cap) { return (cap + SCAN_TILE - 1) / SCAN_TILE + 2; }
+
+int size_workspace(rbh_ctx* c) {
+  const size_t V = 4 * (size_t)c->prec;
+  const size_t cp = c->cap_pairs, cs = c->cap_slots, cc = c->cap_cand, ck = c->cap_contacts;
+  const size_t nb = std::max(c->nb, 1), ns = std::max(c->ns, 1);
+  int maxcap = std::max({c->cap_pairs, c->cap_slots, c->cap_cand, c->cap_contacts, c->nb});
+  struct E { DBuf* b; size_t bytes; } el[] = {
+      {&c->row_cnt, nb * 4}, {&c->row_off, (nb + 1) * 4},
+      {&c->pairs, cp * 8}, {&c->pair_sz, cp * 4}, {&c->slot_off, (cp + 1) * 4}, {&c->n_pairs, 16},
+      {&c->slot_flag, cs + 16}, {&c->slot_scan, (cs + 1) * 4}, {&c->n_slots, 16},
+      {&c->cand_pair, cc * 4}, {&c->cand_tri, cc * 4}, {&c->pair_cstart, (cp + 1) * 4}, {&c->n_cand, 16},
+      {&c->cand_p, cc * V}, {&c->cand_n, cc * V}, {&c->cand_keep, cc + 16}, {&c->keep_scan, (cc + 1) * 4},
+      {&c->c_ia, ck * 4}, {&c->c_ib, ck * 4}, {&c->c_tri, ck * 4}, {&c->c_pair, ck * 4},
+      {&c->c_p, ck * V}, {&c->c_n, ck * V}, {&c->n_contacts, 16}, {&c->c_rA, ck * V}, {&c->c_rB, ck * V},
+      {&c->c_b, ck * V}, {&c->c_r0, ck * V}, {&c->c_mu, ck * V}, {&c->c_imp, ck * V}, {&c->c_color, ck * 4},
+      {&c->bc_cnt, nb * 4}, {&c->bc_off, (nb + 1) * 4}, {&c->bc_fill, nb * 4}, {&c->bc_list, 2 * ck * 4},
+      {&c->uf, nb * 4}, {&c->isl_max, nb * 4}, {&c->pal, ck * PALW * 4}, {&c->c_deg, ck * 4},
+      {&c->c_isl, ck * 4}, {&c->c_done, ck + 16}, {&c->col_off, ns * (PALMAX + 1) * 4},
+      {&c->perm, ck * 4}, {&c->s_id, ck * 8}, {&c->s_n, ck * V}, {&c->s_rA, ck * V}, {&c->s_rB, ck * V},
+      {&c->s_b, ck * V}, {&c->s_r0, ck * V}, {&c->x0, ck * V}, {&c->x1, ck * V}, {&c->errbuf, 64},
+      {&c->scan_part, (size_t)scan_parts(maxcap) * 4}};
+  for (auto& e : el) {
+    int rc = ensure(c, *e.b, e.bytes);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int build_batch(rbh_ctx* c, const int* ids, int n) {
+  std::vector<int> want(ids, ids + n);
+  if (want == c->batch && c->batch_arena_gen == c->arena_gen) return 0;
+  drop_graph(c);
+  std::vector<int> bb, bsslot, bsfoff, fend;
+  std::vector<int> b2f(std::max(c->bodies.cap, 1), -1);
+  long long pairs_all = 0, tri_sum = 0;
+  for (int i = 0; i < n; ++i) {
+    const SceneHost& s = c->scenes[want[i]];
+    bsslot.push_back(want[i]);
+    bsfoff.push_back((int)bb.size());
+    for (int b = 0; b < s.nbody; ++b) {
+      b2f[s.body_off + b] = (int)bb.size();
+      bb.push_back(s.body_off + b);
+      fend.push_back(s.body_off + s.nbody);
+    }
+    pairs_all += (long long)s.nbody * (s.nbody - 1) / 2;
+    tri_sum += s.ntri;
+  }
+  bsfoff.push_back((int)bb.size());
+  c->nb = (int)bb.size();
+  c->ns = n;
+  std::vector<int> nbv = {c->nb, 0, 0, 0};
+  int rc = upload(c, c->bb_body, bb);
+  if (!rc) rc = upload(c, c->bs_slot, bsslot);
+  if (!rc) rc = upload(c, c->bs_foff, bsfoff);
+  if (!rc) rc = upload(c, c->b2f, b2f);
+  if (!rc) rc = upload(c, c->f_end, fend);
+  if (!rc) rc = upload(c, c->d_nb, nbv);
+  if (rc) return rc;
+  // initial capacities; they grow when a step reports an overflow
+  long long est_pairs = std::min<long long>(pairs_all, 64LL * c->nb + 4096);
+  c->cap_pairs = std::max(c->cap_pairs, (int)std::max<long long>(est_pairs, 1));
+  c->cap_slots = std::max(c->cap_slots, (int)std::min<long long>(8 * tri_sum + 65536, 1LL << 30));
+  c->cap_cand = std::max(c->cap_cand, c->cap_slots / 2);
+  c->cap_contacts = std::max(c->cap_contacts, (int)std::min<long long>(64LL * c->nb + 65536, c->cap_cand));
+  rc = size_workspace(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->batch = want;
+  c->batch_arena_gen = c->arena_gen;
+  return 0;
+}
+
+// ------------------------------------------------------------ launch sequence
+__global__ void k_fill(int* p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+int grid_for(long n, int block, int maxg = 8192) {
+  long g = (n + block - 1) / block;
+  return (int)std::max<long>(1, std::min<long>(g, maxg));
+}
+
+template <typename T> void scan(rbh_ctx* c, const T* in, int* out, const int* n_dev, int cap, int* total) {
+  int nb = std::max(1, (cap + SCAN_TILE - 1) / SCAN_TILE + 1);
+  int* part = reinterpret_cast<int*>(c->scan_part.p);
+  hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, part);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, c->stream, part, n_dev, cap, total);
+  hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, part, out);
+}
+
+template <typename R> void launch_step(rbh_ctx* c, double dt, int flags, bool timing) {
+  Dev<R> d = make_dev<R>(c, dt, flags);
+  hipStream_t s = c->stream;
+  const int nb = c->nb, ns = c->ns;
+  auto mark = [&](int st) { if (timing) (void)hipEventRecord(c->ev[st], s); };
+  mark(ST_BEGIN);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, d.err, 16, 0);
+  hipLaunchKernelGGL(k_scene_reset<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_body_prep<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_refit<R>, dim3(nb), dim3(64), 0, s, d);
+  mark(ST_PREP);
+  // body pairs: count per row, scan, write
+  hipLaunchKernelGGL(k_pair_count<R>, dim3(nb), dim3(64), 0, s, d);
+  scan<int>(c, d.row_cnt, d.row_off, d.d_nb, nb, d.n_pairs);
+  hipLaunchKernelGGL(k_pair_write<R>, dim3(nb), dim3(64), 0, s, d);
+  // candidate slots: (nt_a + nt_b) per pair
+  scan<int>(c, d.pair_sz, d.slot_off, d.n_pairs, c->cap_pairs, d.n_slots);
+  hipLaunchKernelGGL(k_cand_flags<R>, dim3(grid_for(c->cap_pairs, 1, 16384)), dim3(CAND_BLOCK), 0, s, d);
+  scan<unsigned char>(c, d.slot_flag, d.slot_scan, d.n_slots, c->cap_slots, nullptr);
+  hipLaunchKernelGGL(k_cand_write<R>, dim3(grid_for(c->cap_slots, 256)), dim3(256), 0, s, d);
+  mark(ST_BROAD);
+  hipLaunchKernelGGL(k_frank_wolfe<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_dedup<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_keep01<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
+  scan<unsigned char>(c, d.cand_keep, d.keep_scan, d.n_cand, c->cap_cand, nullptr);
+  hipLaunchKernelGGL(k_contact_write<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
+  mark(ST_FW);
+  hipLaunchKernelGGL(k_assemble<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);
+  mark(ST_ASM);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(nb, 256)), dim3(256), 0, s, d.bc_cnt, nb, 0);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(nb, 256)), dim3(256), 0, s, d.bc_fill, nb, 0);
+  hipLaunchKernelGGL(k_csr_count<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);
+  scan<int>(c, d.bc_cnt, d.bc_off, d.d_nb, nb, nullptr);
+  hipLaunchKernelGGL(k_csr_fill<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_color<R>, dim3(ns), dim3(COLOR_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_order<R>, dim3(ns), dim3(COLOR_BLOCK), 0, s, d);
+  mark(ST_COLOR);
+  hipLaunchKernelGGL(k_solve<R>, dim3(ns), dim3(SOLVE_BLOCK), 0, s, d);
+  mark(ST_SOLVE);
+  hipLaunchKernelGGL(k_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_scene_finalize<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
+  mark(ST_INT);
+}
+
+int read_err(rbh_ctx* c, int e[8]) {
+  HIPCHK(c, hipMemcpyAsync(e, c->errbuf.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int first_step_index(rbh_ctx* c, int slot, int* out) {
+  SceneState st;
+  HIPCHK(c, hipMemcpyAsync(&st, reinterpret_cast<SceneState*>(c->d_ss.p) + slot, sizeof(SceneState),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = st.step_index;
+  return 0;
+}
+
+template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flags) {
+  const bool timing = (flags & RBH_STEP_TIMING) != 0;
+  const bool use_graph = (flags & RBH_STEP_GRAPH) != 0;
+  const int kflags = flags & RBH_STEP_COUNT_OVERLAPS;
+  int done = 0;
+  for (int attempt = 0; attempt < 16 && done < n_steps; ++attempt) {
+    int si0 = 0;
+    int rc = first_step_index(c, c->batch[0], &si0);
+    if (rc) return rc;
+    const int todo = n_steps - done;
+    if (use_graph) {
+      if (!c->gexec || c->graph_key_gen != c->arena_gen || c->graph_dt != dt || c->graph_flags != kflags) {
+        drop_graph(c);
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        launch_step<R>(c, dt, kflags, false);
+        HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph));
+        HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->graph_key_gen = c->arena_gen;
+        c->graph_dt = dt;
+        c->graph_flags = kflags;
+      }
+      for (int i = 0; i < todo; ++i) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    } else {
+      for (int i = 0; i < todo; ++i) launch_step<R>(c, dt, kflags, timing && i == todo - 1);
+      HIPCHK(c, hipGetLastError());
+    }
+    int e[8];
+    rc = read_err(c, e);
+    if (rc) return rc;
+    int si1 = 0;
+    rc = first_step_index(c, c->batch[0], &si1);
+    if (rc) return rc;
+    done += si1 - si0;
+    c->ws_gen++;
+    if (e[0] == 0) break;
+    if (e[0] & ERR_COLORS)
+      return fail(c, RBH_E_COLORS, "colouring needs more than " + std::to_string(PALMAX) + " colours");
+    // grow what overflowed and redo the remaining steps
+    auto grow = [](int cur, int need) { return std::max(cur * 2, (int)std::min<long long>(need * 3LL / 2 + 1024, 1LL << 30)); };
+    if (e[0] & ERR_CAP_PAIRS) c->cap_pairs = grow(c->cap_pairs, e[1]);
+    if (e[0] & ERR_CAP_SLOTS) c->cap_slots = grow(c->cap_slots, e[2]);
+    if (e[0] & ERR_CAP_CAND) c->cap_cand = grow(c->cap_cand, e[3]);
+    if (e[0] & ERR_CAP_CONTACTS) c->cap_contacts = grow(c->cap_contacts, e[4]);
+    c->cap_cand = std::max(c->cap_cand, std::min(c->cap_slots, c->cap_cand));
+    drop_graph(c);
+    c->arena_gen++;
+    c->batch_arena_gen = c->arena_gen;
+    rc = size_workspace(c);
+    if (rc) return rc;
+  }
+  if (done < n_steps) return fail(c, RBH_E_CAPACITY, "step workspace could not grow enough");
+  // batch counters
+  int h[4] = {0, 0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(&h[0], c->n_pairs.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&h[1], c->n_slots.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&h[2], c->n_cand.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&h[3], c->n_contacts.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 4; ++i) c->counters[i] = h[i];
+  if (timing && !use_graph) {
+    for (int i = 1; i < ST_N; ++i) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, c->ev[i - 1], c->ev[i]);
+      c->timings[i - 1] = ms;
+    }
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, c->ev[ST_BEGIN], c->ev[ST_INT]);
+    c->timings[ST_N - 1] = tot;
+  }
+  for (size_t i = 0; i < c->batch.size(); ++i) {
+    c->scenes[c->batch[i]].ws_gen = c->ws_gen;
+    c->scenes[c->batch[i]].batch_index = (int)i;
+  }
+  return 0;
+}
+
+bool valid_scene(rbh_ctx* c, int id) { return id >= 0 && id < (int)c->scenes.size() && c->scenes[id].live; }
+
+template <typename R> int set_state(rbh_ctx* c, const SceneHost& s, const double* r, const double* q,
+                                    const double* v, const double* w) {
+  const int n = s.nbody;
+  std::vector<V4<R>> a(n);
+  const double* src[4] = {r, q, v, w};
+  DBuf* dst[4] = {&c->r, &c->q, &c->v, &c->w};
+  for (int k = 0; k < 4; ++k) {
+    if (!src[k]) continue;
+    int dim = k == 1 ? 4 : 3;
+    for (int i = 0; i < n; ++i) {
+      const double* p = src[k] + (size_t)dim * i;
+      a[i] = V4<R>{R(p[0]), R(p[1]), R(p[2]), dim == 4 ? R(p[3]) : R(0)};
+    }
+    HIPCHK(c, hipMemcpyAsync(reinterpret_cast<V4<R>*>(dst[k]->p) + s.body_off, a.data(), n * sizeof(V4<R>),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+template <typename R> int get_state(rbh_ctx* c, const SceneHost& s, double* r, double* q, double* v, double* w) {
+  const int n = s.nbody;
+  std::vector<V4<R>> a(n);
+  double* dstp[4] = {r, q, v, w};
+  DBuf* src[4] = {&c->r, &c->q, &c->v, &c->w};
+  for (int k = 0; k < 4; ++k) {
+    if (!dstp[k]) continue;
+    HIPCHK(c, hipMemcpyAsync(a.data(), reinterpret_cast<V4<R>*>(src[k]->p) + s.body_off, n * sizeof(V4<R>),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int dim = k == 1 ? 4 : 3;
+    for (int i = 0; i < n; ++i) {
+      double* p = dstp[k] + (size_t)dim * i;
+      p[0] = a[i].x; p[1] = a[i].y; p[2] = a[i].z;
+      if (dim == 4) p[3] = a[i].w;
+    }
+  }
+  return 0;
+}
+
+template <typename R> int scene_statics(rbh_ctx* c, const rbh_scene_desc* d, const SceneHost& s, int slot) {
+  const int n = d->n_bodies;
+  std::vector<int> sh(n), sl(n, slot), fx(n), mt(n), tr(n);
+  std::vector<R> ms(n);
+  std::vector<V4<R>> in(n), gr(n), dp(n), z(n, V4<R>{R(0), R(0), R(0), R(0)});
+  std::vector<V4<R>> qid(n, V4<R>{R(1), R(0), R(0), R(0)});
+  int toff = s.tri_off;
+  for (int i = 0; i < n; ++i) {
+    sh[i] = d->shape[i];
+    fx[i] = d->fixed ? (d->fixed[i] != 0) : 0;
+    mt[i] = d->material ? d->material[i] : 0;
+    tr[i] = toff;
+    toff += c->shapes[sh[i]].nt;
+    ms[i] = R(d->mass[i]);
+    in[i] = V4<R>{R(d->inertia[3 * i]), R(d->inertia[3 * i + 1]), R(d->inertia[3 * i + 2]), R(0)};
+    gr[i] = d->gravity ? V4<R>{R(d->gravity[3 * i]), R(d->gravity[3 * i + 1]), R(d->gravity[3 * i + 2]), R(0)} : z[i];
+    dp[i] = d->damping ? V4<R>{R(d->damping[2 * i]), R(d->damping[2 * i + 1]), R(0), R(0)} : z[i];
+  }
+  auto put = [&](DBuf& b, const void* src, size_t el) -> int {
+    HIPCHK(c, hipMemcpyAsync((char*)b.p + el * s.body_off, src, el * n, hipMemcpyHostToDevice, c->stream));
+    return 0;
+  };
+  int rc = 0;
+  rc |= put(c->b_shape, sh.data(), 4); rc |= put(c->b_slot, sl.data(), 4); rc |= put(c->b_fixed, fx.data(), 4);
+  rc |= put(c->b_mat, mt.data(), 4); rc |= put(c->b_tri, tr.data(), 4); rc |= put(c->b_mass, ms.data(), sizeof(R));
+  rc |= put(c->b_inertia, in.data(), sizeof(V4<R>)); rc |= put(c->b_grav, gr.data(), sizeof(V4<R>));
+  rc |= put(c->b_damp, dp.data(), sizeof(V4<R>));
+  rc |= put(c->r, z.data(), sizeof(V4<R>)); rc |= put(c->q, qid.data(), sizeof(V4<R>));
+  rc |= put(c->v, z.data(), sizeof(V4<R>)); rc |= put(c->w, z.data(), sizeof(V4<R>));
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+template <typename R> int get_contacts(rbh_ctx* c, const SceneHost& s, const SceneState& st, int cap, int* count,
+                                       int* ia, int* ib, int* tri, double* p, double* n, double* gap, double* imp,
+                                       int* color) {
+  const int K = st.k_cnt;
+  *count = K;
+  const int m = std::min(K, cap);
+  if (m <= 0) return 0;
+  const int k0 = st.k_off;
+  std::vector<int> ti(m);
+  std::vector<V4<R>> tv(m);
+  auto geti = [&](DBuf& b, int* out, int sub) -> int {
+    if (!out) return 0;
+    HIPCHK(c, hipMemcpyAsync(ti.data(), (int*)b.p + k0, m * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < m; ++i) out[i] = ti[i] - sub;
+    return 0;
+  };
+  int rc = geti(c->c_ia, ia, s.body_off);
+  if (!rc) rc = geti(c->c_ib, ib, s.body_off);
+  if (!rc) rc = geti(c->c_tri, tri, 0);
+  if (!rc) rc = geti(c->c_color, color, 0);
+  if (rc) return rc;
+  auto getv = [&](DBuf& b) -> int {
+    HIPCHK(c, hipMemcpyAsync(tv.data(), (V4<R>*)b.p + k0, m * sizeof(V4<R>), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+  };
+  if (p || gap) {
+    if ((rc = getv(c->c_p))) return rc;
+    for (int i = 0; i < m; ++i) {
+      if (p) { p[3 * i] = tv[i].x; p[3 * i + 1] = tv[i].y; p[3 * i + 2] = tv[i].z; }
+      if (gap) gap[i] = tv[i].w;
+    }
+  }
+  if (n) {
+    if ((rc = getv(c->c_n))) return rc;
+    for (int i = 0; i < m; ++i) { n[3 * i] = tv[i].x; n[3 * i + 1] = tv[i].y; n[3 * i + 2] = tv[i].z; }
+  }
+  if (imp) {
+    if ((rc = getv(c->c_imp))) return rc;
+    for (int i = 0; i < m; ++i) {
+      if (st.solved) { imp[4 * i] = tv[i].x; imp[4 * i + 1] = tv[i].y; imp[4 * i + 2] = tv[i].z; imp[4 * i + 3] = tv[i].w; }
+      else { imp[4 * i] = imp[4 * i + 1] = imp[4 * i + 2] = imp[4 * i + 3] = 0.0; }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+int rbh_version(void) { return 1; }
+
+int rbh_ctx_create(int device, int precision_bytes, unsigned long long hbm_budget_bytes, rbh_ctx** out) {
+  if (!out) return RBH_E_ARG;
+  *out = nullptr;
+  if (precision_bytes != 4 && precision_bytes != 8) return RBH_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return RBH_E_HIP;
+  if (device < 0 || device >= ndev) return RBH_E_ARG;
+  rbh_ctx* c = new rbh_ctx();
+  c->device = device;
+  c->prec = precision_bytes;
+  c->budget = hbm_budget_bytes;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipGetDeviceProperties(&c->prop, device) != hipSuccess) {
+    delete c;
+    return RBH_E_HIP;
+  }
+  for (int i = 0; i < ST_N; ++i) (void)hipEventCreate(&c->ev[i]);
+  c->bodies.init(0);
+  c->tris.init(0);
+  c->res.init(0);
+  *out = c;
+  return 0;
+}
+
+int rbh_ctx_destroy(rbh_ctx* c) {
+  if (!c) return RBH_E_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  free_all(c);
+  for (int i = 0; i < ST_N; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+const char* rbh_last_error(rbh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int rbh_device_query(rbh_ctx* c, rbh_device_info* out) {
+  if (!c || !out) return RBH_E_ARG;
+  std::memset(out, 0, sizeof(*out));
+  std::snprintf(out->name, sizeof(out->name), "%s (%s)", c->prop.name, c->prop.gcnArchName);
+  out->compute_units = c->prop.multiProcessorCount;
+  out->lds_per_block = (int)c->prop.sharedMemPerBlock;
+  out->hbm_bytes = c->prop.totalGlobalMem;
+  out->precision_bytes = c->prec;
+  out->max_colors = PALMAX;
+  return 0;
+}
+
+int rbh_memory_usage(rbh_ctx* c, unsigned long long* bytes) {
+  if (!c || !bytes) return RBH_E_ARG;
+  *bytes = total_bytes(c);
+  return 0;
+}
+
+int rbh_sdf_build(rbh_ctx* c, const double* V, int nv, const int* T, int nt, const double* mn, const double* mx,
+                  int I, int J, int K, double* values) {
+  if (!c || !V || !T || !mn || !mx || !values || nv <= 0 || nt <= 0) return fail(c, RBH_E_ARG, "bad SDF arguments");
+  if (I < 2 || J < 2 || K < 2) return fail(c, RBH_E_ARG, "grid needs at least 2 nodes per axis");
+  for (int i = 0; i < 3; ++i) if (mn[i] > mx[i]) return fail(c, RBH_E_ARG, "min_coord > max_coord");
+  for (int i = 0; i < 3 * nt; ++i) if (T[i] < 0 || T[i] >= nv) return fail(c, RBH_E_ARG, "triangle index out of range");
+  (void)hipSetDevice(c->device);
+  std::vector<double> tri((size_t)nt * 9);
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < 3; ++k)
+      for (int a = 0; a < 3; ++a) tri[(size_t)t * 9 + 3 * k + a] = V[3 * T[3 * t + k] + a];
+  const long n = (long)I * J * K;
+  double *dtri = nullptr, *dout = nullptr;
+  HIPCHK(c, hipMalloc(&dtri, tri.size() * 8));
+  if (hipMalloc(&dout, n * 8) != hipSuccess) { (void)hipFree(dtri); return fail(c, RBH_E_MEMORY, "SDF buffer"); }
+  HIPCHK(c, hipMemcpyAsync(dtri, tri.data(), tri.size() * 8, hipMemcpyHostToDevice, c->stream));
+  double h[3] = {(mx[0] - mn[0]) / (I - 1), (mx[1] - mn[1]) / (J - 1), (mx[2] - mn[2]) / (K - 1)};
+  hipLaunchKernelGGL(k_sdf, dim3((unsigned)((n + SDF_BLOCK - 1) / SDF_BLOCK)), dim3(SDF_BLOCK), 0, c->stream, dtri, nt,
+                     I, J, K, mn[0], mn[1], mn[2], h[0], h[1], h[2], dout);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(values, dout, n * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dtri);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(c, RBH_E_HIP, std::string("SDF kernel: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int rbh_shape_upload(rbh_ctx* c, const double* V, int nv, const int* T, int nt, const double* sdf, int I, int J,
+                     int K, const double* mn, const double* mx, int* shape_id) {
+  if (!c || !V || !T || !sdf || !mn || !mx || !shape_id || nv <= 0 || nt <= 0) return fail(c, RBH_E_ARG, "bad shape");
+  if (I < 2 || J < 2 || K < 2) return fail(c, RBH_E_ARG, "grid needs at least 2 nodes per axis");
+  for (int i = 0; i < 3 * nt; ++i) if (T[i] < 0 || T[i] >= nv) return fail(c, RBH_E_ARG, "triangle index out of range");
+  ShapeInfo s{};
+  s.v_off = (int)(c->sV.size() / 4);
+  s.nv = nv;
+  s.t_off = (int)(c->sT.size() / 4);
+  s.nt = nt;
+  s.g_off = (int)c->sdf.size();
+  s.I = I; s.J = J; s.K = K;
+  const int res[3] = {I - 1, J - 1, K - 1};
+  for (int a = 0; a < 3; ++a) {
+    s.mn[a] = mn[a];
+    s.mx[a] = mx[a];
+    s.h[a] = (mx[a] - mn[a]) / (double)res[a];   // grid3.py:42-44
+  }
+  for (int i = 0; i < nv; ++i) {
+    c->sV.push_back(V[3 * i]); c->sV.push_back(V[3 * i + 1]); c->sV.push_back(V[3 * i + 2]); c->sV.push_back(0.0);
+  }
+  for (int t = 0; t < nt; ++t) {
+    c->sT.push_back(T[3 * t]); c->sT.push_back(T[3 * t + 1]); c->sT.push_back(T[3 * t + 2]); c->sT.push_back(0);
+  }
+  c->sdf.insert(c->sdf.end(), sdf, sdf + (size_t)I * J * K);
+  c->shapes.push_back(s);
+  c->shapes_dirty = true;
+  *shape_id = (int)c->shapes.size() - 1;
+  return 0;
+}
+
+int rbh_scene_footprint(rbh_ctx* c, const rbh_scene_desc* d, unsigned long long* bytes) {
+  if (!c || !d || !bytes) return RBH_E_ARG;
+  long long nt = 0;
+  for (int i = 0; i < d->n_bodies; ++i) {
+    if (d->shape[i] < 0 || d->shape[i] >= (int)c->shapes.size()) return fail(c, RBH_E_ARG, "unknown shape id");
+    nt += c->shapes[d->shape[i]].nt;
+  }
+  const long long V = 4LL * c->prec;
+  // arena (23 body arrays) + triangle AABBs + workspace estimate per body / triangle
+  long long b = (long long)d->n_bodies * (19 * V + 5 * 4 + c->prec) + nt * 2 * V;
+  long long ws_contacts = 64LL * d->n_bodies;
+  long long ws = ws_contacts * (16 * V + PALW * 4 + 64) + 8 * nt * 5 + 4 * nt * (2 * V + 13) + (long long)d->n_bodies * 64;
+  *bytes = (unsigned long long)(b + ws + 8LL * d->params.max_iterations);
+  return 0;
+}
+
+int rbh_scene_create(rbh_ctx* c, const rbh_scene_desc* d, int* scene_id) {
+  if (!c || !d || !scene_id) return fail(c, RBH_E_ARG, "null argument");
+  if (d->n_bodies <= 0) return fail(c, RBH_E_ARG, "a scene needs at least one body");
+  if (!d->shape || !d->mass || !d->inertia) return fail(c, RBH_E_ARG, "shape, mass and inertia are required");
+  int rc = check_params(c, d->params);
+  if (rc) return rc;
+  const int nmat = std::max(1, d->n_materials);
+  long long nt = 0;
+  for (int i = 0; i < d->n_bodies; ++i) {
+    if (d->shape[i] < 0 || d->shape[i] >= (int)c->shapes.size()) return fail(c, RBH_E_ARG, "unknown shape id");
+    int mt = d->material ? d->material[i] : 0;
+    if (mt < 0 || mt >= nmat) return fail(c, RBH_E_ARG, "material index out of range");
+    bool fixed = d->fixed && d->fixed[i];
+    if (!fixed && d->mass[i] == 0.0) return fail(c, RBH_E_ARG, "Body " + std::to_string(i) + " has no mass.");
+    nt += c->shapes[d->shape[i]].nt;
+  }
+  (void)hipSetDevice(c->device);
+  if (c->budget) {
+    unsigned long long fp = 0;
+    rbh_scene_footprint(c, d, &fp);
+    if (total_bytes(c) + fp > c->budget) return fail(c, RBH_E_MEMORY, "scene does not fit the HBM budget");
+  }
+  SceneHost s;
+  s.nbody = d->n_bodies;
+  s.ntri = (int)nt;
+  s.body_off = c->bodies.alloc(s.nbody);
+  if (s.body_off < 0) {
+    if ((rc = grow_bodies(c, c->bodies.used() + s.nbody))) return rc;
+    s.body_off = c->bodies.alloc(s.nbody);
+  }
+  s.tri_off = c->tris.alloc(s.ntri);
+  if (s.tri_off < 0) {
+    if ((rc = grow_tris(c, c->tris.used() + s.ntri))) return rc;
+    s.tri_off = c->tris.alloc(s.ntri);
+  }
+  s.res_n = std::max(1, d->params.max_iterations);
+  s.res_off = c->res.alloc(s.res_n);
+  if (s.res_off < 0) {
+    if ((rc = grow_res(c, c->res.used() + s.res_n))) return rc;
+    s.res_off = c->res.alloc(s.res_n);
+  }
+  if (s.body_off < 0 || s.tri_off < 0 || s.res_off < 0) return fail(c, RBH_E_MEMORY, "arena allocation failed");
+  s.nmat = nmat;
+  s.mat_off = (int)(c->mat_host.size() / 4);
+  for (int i = 0; i < nmat * nmat; ++i)
+    for (int k = 0; k < 4; ++k) c->mat_host.push_back(d->interactions ? d->interactions[4 * i + k] : (k < 3 ? 1.0 : 0.0));
+  s.params = d->params;
+  s.live = true;
+  int slot = -1;
+  for (size_t i = 0; i < c->scenes.size(); ++i) if (!c->scenes[i].live) { slot = (int)i; break; }
+  if (slot < 0) {
+    slot = (int)c->scenes.size();
+    c->scenes.push_back(s);
+    c->sp_host.push_back(SceneParams{});
+  } else {
+    c->scenes[slot] = s;
+  }
+  c->sp_host[slot] = to_device_params(s.params, s.nmat, s.mat_off, s.res_off);
+  c->scenes_dirty = true;
+  if ((rc = sync_scenes(c))) return rc;
+  SceneState z{};
+  HIPCHK(c, hipMemcpyAsync(reinterpret_cast<SceneState*>(c->d_ss.p) + slot, &z, sizeof(z), hipMemcpyHostToDevice,
+                           c->stream));
+  rc = c->prec == 4 ? scene_statics<float>(c, d, s, slot) : scene_statics<double>(c, d, s, slot);
+  if (rc) return rc;
+  c->arena_gen++;
+  drop_graph(c);
+  *scene_id = slot;
+  return 0;
+}
+
+int rbh_scene_destroy(rbh_ctx* c, int id) {
+  if (!c || !valid_scene(c, id)) return fail(c, RBH_E_ARG, "unknown scene");
+  SceneHost& s = c->scenes[id];
+  c->bodies.release(s.body_off, s.nbody);
+  c->tris.release(s.tri_off, s.ntri);
+  c->res.release(s.res_off, s.res_n);
+  s.live = false;
+  c->arena_gen++;
+  drop_graph(c);
+  c->batch.clear();
+  return 0;
+}
+
+int rbh_scene_set_params(rbh_ctx* c, int id, const rbh_params* p) {
+  if (!c || !p || !valid_scene(c, id)) return fail(c, RBH_E_ARG, "unknown scene");
+  int rc = check_params(c, *p);
+  if (rc) return rc;
+  SceneHost& s = c->scenes[id];
+  if (std::max(1, p->max_iterations) > s.res_n) {
+    c->res.release(s.res_off, s.res_n);
+    s.res_n = std::max(1, p->max_iterations);
+    s.res_off = c->res.alloc(s.res_n);
+    if (s.res_off < 0) {
+      if ((rc = grow_res(c, c->res.used() + s.res_n))) return rc;
+      s.res_off = c->res.alloc(s.res_n);
+    }
+  }
+  s.params = *p;
+  c->sp_host[id] = to_device_params(s.params, s.nmat, s.mat_off, s.res_off);
+  c->scenes_dirty = true;
+  drop_graph(c);
+  return sync_scenes(c);
+}
+
+int rbh_scene_set_state(rbh_ctx* c, int id, const double* r, const double* q, const double* v, const double* w) {
+  if (!c || !valid_scene(c, id)) return fail(c, RBH_E_ARG, "unknown scene");
+  (void)hipSetDevice(c->device);
+  return c->prec == 4 ? set_state<float>(c, c->scenes[id], r, q, v, w) : set_state<double>(c, c->scenes[id], r, q, v, w);
+}
+
+int rbh_scene_get_state(rbh_ctx* c, int id, double* r, double* q, double* v, double* w) {
+  if (!c || !valid_scene(c, id)) return fail(c, RBH_E_ARG, "unknown scene");
+  (void)hipSetDevice(c->device);
+  return c->prec == 4 ? get_state<float>(c, c->scenes[id], r, q, v, w) : get_state<double>(c, c->scenes[id], r, q, v, w);
+}
+
+int rbh_scene_set_step_index(rbh_ctx* c, int id, int step_index) {
+  if (!c || !valid_scene(c, id)) return fail(c, RBH_E_ARG, "unknown scene");
+  if (step_index < 0) return fail(c, RBH_E_ARG, "step index must be non-negative");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipMemcpyAsync(reinterpret_cast<char*>(c->d_ss.p) + sizeof(SceneState) * id + offsetof(SceneState, step_index),
+                           &step_index, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int rbh_step(rbh_ctx* c, const int* ids, int n, int n_steps, double dt, int flags) {
+  if (!c || !ids || n <= 0 || n_steps < 0) return fail(c, RBH_E_ARG, "bad step arguments");
+  if (n_steps == 0) return 0;
+  for (int i = 0; i < n; ++i) {
+    if (!valid_scene(c, ids[i])) return fail(c, RBH_E_ARG, "unknown scene " + std::to_string(ids[i]));
+    for (int j = 0; j < i; ++j) if (ids[j] == ids[i]) return fail(c, RBH_E_ARG, "scene listed twice");
+  }
+  if (!(dt > 0.0) && dt != 0.0 && !(dt < 0.0)) return fail(c, RBH_E_ARG, "dt is NaN");
+  (void)hipSetDevice(c->device);
+  int rc = c->prec == 4 ? sync_shapes<float>(c) : sync_shapes<double>(c);
+  if (!rc) rc = sync_scenes(c);
+  if (!rc) rc = build_batch(c, ids, n);
+  if (rc) return rc;
+  return c->prec == 4 ? run_steps<float>(c, n_steps, dt, flags) : run_steps<double>(c, n_steps, dt, flags);
+}
+
+int rbh_sync(rbh_ctx* c) {
+  if (!c) return RBH_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int rbh_get_log(rbh_ctx* c, int id, rbh_step_log* log, double* residuals, int cap) {
+  if (!c || !valid_scene(c, id)) return fail(c, RBH_E_ARG, "unknown scene");
+  SceneState st;
+  HIPCHK(c, hipMemcpyAsync(&st, reinterpret_cast<SceneState*>(c->d_ss.p) + id, sizeof(st), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const SceneHost& s = c->scenes[id];
+  if (log) {
+    std::memset(log, 0, sizeof(*log));
+    log->step_index = st.step_index;
+    log->contact_points = st.k_cnt;
+    log->iterations = st.iterations;
+    log->exitcode = st.exitcode;
+    log->rejects = st.rejects;
+    log->sweeps = st.sweeps;
+    log->colors = st.ncolor;
+    log->solved = st.solved;
+    log->number_of_overlaps = st.n_overlaps;
+    log->fw_cell_evals = st.fw_evals;
+    log->natural_residual = st.natural_residual;
+    log->max_penetration = st.max_pen;
+    log->final_merit = st.final_merit;
+  }
+  if (residuals && cap > 0) {
+    int m = std::min(cap, s.params.max_iterations);
+    if (m > 0)
+      HIPCHK(c, hipMemcpyAsync(residuals, reinterpret_cast<double*>(c->d_res.p) + s.res_off, m * 8,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = m; i < cap; ++i) residuals[i] = INFINITY;
+  }
+  return 0;
+}
+
+int rbh_get_contacts(rbh_ctx* c, int id, int cap, int* count, int* ia, int* ib, int* tri, double* p, double* n,
+                     double* gap, double* impulse, int* color) {
+  if (!c || !count || !valid_scene(c, id)) return fail(c, RBH_E_ARG, "unknown scene");
+  const SceneHost& s = c->scenes[id];
+  if (s.ws_gen != c->ws_gen) return fail(c, RBH_E_STATE, "contacts of this scene were overwritten by a later step");
+  SceneState st;
+  HIPCHK(c, hipMemcpyAsync(&st, reinterpret_cast<SceneState*>(c->d_ss.p) + id, sizeof(st), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return c->prec == 4 ? get_contacts<float>(c, s, st, cap, count, ia, ib, tri, p, n, gap, impulse, color)
+                      : get_contacts<double>(c, s, st, cap, count, ia, ib, tri, p, n, gap, impulse, color);
+}
+
+int rbh_get_timings(rbh_ctx* c, double* ms, int cap) {
+  if (!c || !ms) return RBH_E_ARG;
+  int m = std::min(cap, (int)ST_N);
+  for (int i = 0; i < m; ++i) ms[i] = c->timings[i];
+  return m;
+}
+
+int rbh_get_counters(rbh_ctx* c, long long* out, int cap) {
+  if (!c || !out) return RBH_E_ARG;
+  int m = std::min(cap, 4);
+  for (int i = 0; i < m; ++i) out[i] = c->counters[i];
+  return m;
+}
+
+}  // extern "C"

@@ -1,0 +1,1204 @@
+// Kernels of one batched time step.  A "world" is a list of independent
+// scenes whose bodies live in one SoA arena; every kernel works over the whole
+// batch at once, so one scene stepped alone is simply a batch of one.
+//
+// Pipeline of one step (all sizes that depend on data stay on the device):
+//   body_prep  -> refit -> pair_count/scan/pair_write -> cand_flags ->
+//   scan -> cand_write -> frank_wolfe -> dedup -> scan -> contact_write ->
+//   assemble -> body->contact CSR -> colour (per scene) -> order (per scene) ->
+//   solve (per scene) -> integrate -> finalize.
+#pragma once
+#include "rbh_device.h"
+#include "rbh_scan.h"
+
+namespace rbh {
+
+// -------------------------------------------------------------- shared types
+struct ShapeInfo {
+  int v_off, nv, t_off, nt, g_off, I, J, K;
+  double mn[3], mx[3], h[3];
+};
+
+enum Scheme { SCHEME_GS = 0, SCHEME_PGS = 1, SCHEME_JACOBI = 2, SCHEME_JACOBI_HYBRID = 3 };
+
+struct SceneParams {
+  double time_step, envelope, abs_tol, rel_tol, nu_reduce, nu_increase, too_small;
+  double fw_tol, gap_reduction, min_gap, max_gap, ell_tol, ell_expansion;
+  int max_iter, fw_max_iter, ell_max_iter, use_bounce, use_pre_stab, scheme, friction;
+  int nmat, mat_off, res_off, pad;
+  unsigned long long seed;
+};
+
+struct SceneState {
+  int step_index;      // committed steps
+  int k_off, k_cnt;    // contacts of this step in the batch contact arrays
+  int ncolor;
+  int iterations, exitcode, rejects, sweeps;
+  int n_pairs, n_cand, solved, pad;
+  unsigned long long n_overlaps;
+  unsigned long long fw_evals;   // distinct trilinear cell evaluations in Frank-Wolfe
+  double natural_residual, max_pen, final_merit, pad2;
+};
+
+constexpr int PALW = 64;                 // palette words per contact
+constexpr int PALMAX = PALW * 32;        // largest colour + 1
+constexpr int MAX_ROUNDS = 200;
+
+// error flags (dev.err[0] bits); capacity overflows record the needed size
+enum ErrBits { ERR_CAP_PAIRS = 1, ERR_CAP_SLOTS = 2, ERR_CAP_CAND = 4, ERR_CAP_CONTACTS = 8, ERR_COLORS = 16,
+               ERR_SOLVER = 32 };
+
+template <typename R> struct Dev {
+  // shapes
+  const ShapeInfo* shapes; const V4<R>* sV; const int4* sT; const R* sdf;
+  // scenes (by slot)
+  const SceneParams* sp; SceneState* ss; double* residuals; const double* mat;   // mat: (mu0,mu1,mu2,eps) per pair
+  // body arena
+  V4<R>* r; V4<R>* q; V4<R>* v; V4<R>* w;
+  const int* b_shape; const int* b_slot; const int* b_fixed; const int* b_mat; const int* b_tri;
+  const R* b_mass; const V4<R>* b_inertia; const V4<R>* b_grav; const V4<R>* b_damp;
+  V4<R>* b_W0; V4<R>* b_W1;            // (invI xx,yy,zz,xy), (invI xz,yz, 1/m, 0)
+  V4<R>* b_du0; V4<R>* b_du1;          // W f_ext (linear, angular)
+  V4<R>* b_lo; V4<R>* b_hi;            // world AABB grown by the envelope
+  V4<R>* ws0; V4<R>* ws1;              // accepted W J^T sol
+  V4<R>* ww0; V4<R>* ww1;              // working copy during a sweep
+  V4<R>* tlo; V4<R>* thi;              // per body-instance triangle AABBs
+  // batch
+  int nb, ns;
+  const int* bb_body;                  // flat batch body -> arena body
+  const int* bs_slot;                  // batch scene -> slot
+  const int* bs_foff;                  // batch scene -> first flat body
+  const int* b2f;                      // arena body -> flat batch body
+  const int* f_end;                    // flat batch body -> arena end of its scene
+  const int* d_nb;                     // device copy of nb (scan length)
+  int* row_cnt; int* row_off;          // [nb], [nb+1]
+  int2* pairs; int* pair_sz; int* slot_off; int* n_pairs; int cap_pairs;
+  unsigned char* slot_flag; int* slot_scan; int* n_slots; int cap_slots;
+  int* cand_pair; int* cand_tri; int* pair_cstart; int* n_cand; int cap_cand;
+  V4<R>* cand_p; V4<R>* cand_n; unsigned char* cand_keep; int* keep_scan;
+  // contacts (canonical order)
+  int* c_ia; int* c_ib; int* c_tri; int* c_pair; V4<R>* c_p; V4<R>* c_n; int* n_contacts; int cap_contacts;
+  V4<R>* c_rA; V4<R>* c_rB; V4<R>* c_b; V4<R>* c_r0; V4<R>* c_mu;
+  V4<R>* c_imp;                        // solved impulses (canonical order)
+  int* c_color;
+  // body -> contact lists
+  int* bc_cnt; int* bc_off; int* bc_fill; int* bc_list;
+  int* uf;                             // union-find parent per flat body
+  int* isl_max;                        // per island root (flat body)
+  unsigned* pal;                       // PALW words per contact
+  int* c_deg; int* c_isl; unsigned char* c_done;
+  int* col_off;                        // (PALMAX + 1) per batch scene
+  // solver arrays (colour order)
+  int* perm; int2* s_id; V4<R>* s_n; V4<R>* s_rA; V4<R>* s_rB; V4<R>* s_b; V4<R>* s_r0; V4<R>* x0; V4<R>* x1;
+  int* err;                            // [0] flags, [1..] needed capacities
+  double dt;                           // > 0: use for every scene, else the scene's time_step
+  int flags;
+};
+
+enum StepFlags { STEP_COUNT_OVERLAPS = 1 };
+
+template <typename R> __device__ __forceinline__ R scene_dt(const Dev<R>& d, int slot) {
+  return R(d.dt > 0.0 ? d.dt : d.sp[slot].time_step);
+}
+
+__device__ __forceinline__ bool aborted(const int* err) { return __atomic_load_n(err, __ATOMIC_RELAXED) != 0; }
+
+template <typename R> __device__ __forceinline__ GridView<R> grid_of(const Dev<R>& d, int shape) {
+  const ShapeInfo& s = d.shapes[shape];
+  GridView<R> g;
+  g.val = d.sdf + s.g_off;
+  g.I = s.I; g.J = s.J; g.K = s.K;
+  for (int i = 0; i < 3; ++i) { g.mn[i] = R(s.mn[i]); g.mx[i] = R(s.mx[i]); g.h[i] = R(s.h[i]); }
+  return g;
+}
+
+template <typename R> __device__ __forceinline__ bool box_overlap(V4<R> alo, V4<R> ahi, V4<R> blo, V4<R> bhi) {
+  // kdop.py:21-38: separated iff upper < lower on some axis
+  return !(ahi.x < blo.x || bhi.x < alo.x || ahi.y < blo.y || bhi.y < alo.y || ahi.z < blo.z || bhi.z < alo.z);
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ========================================================== per-body prep
+// state_storage.py:108-185 -- W blocks and delta_u_ext = W f_ext
+template <typename R> __global__ void k_body_prep(Dev<R> d) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= d.nb) return;
+  int b = d.bb_body[f];
+  V4<R> z = {R(0), R(0), R(0), R(0)};
+  d.ws0[b] = z; d.ws1[b] = z; d.ww0[b] = z; d.ww1[b] = z;
+  if (d.b_fixed[b]) {
+    d.b_W0[b] = z; d.b_W1[b] = z; d.b_du0[b] = z; d.b_du1[b] = z;
+    return;
+  }
+  R M[9];
+  qmatrix(quat(d.q[b]), M);
+  V4<R> I = d.b_inertia[b];
+  R Iw[6], Wi[6];
+  rot_diag_rt(M, I.x, I.y, I.z, Iw);
+  rot_diag_rt(M, R(1) / I.x, R(1) / I.y, R(1) / I.z, Wi);
+  V3<R> v = xyz(d.v[b]), w = xyz(d.w[b]);
+  R m = d.b_mass[b];
+  V4<R> g = d.b_grav[b], dm = d.b_damp[b];
+  V3<R> F = V3<R>{-m * g.x, -m * g.y, -m * g.z};
+  F = F - dm.x * v;
+  V3<R> T = -cross(w, sym_mul(Iw, w));
+  T = T - dm.y * w;
+  R im = R(1) / m;
+  V3<R> dl = im * F;
+  V3<R> da = sym_mul(Wi, T);
+  d.b_W0[b] = V4<R>{Wi[0], Wi[1], Wi[2], Wi[3]};
+  d.b_W1[b] = V4<R>{Wi[4], Wi[5], im, R(0)};
+  d.b_du0[b] = V4<R>{dl.x, dl.y, dl.z, R(0)};
+  d.b_du1[b] = V4<R>{da.x, da.y, da.z, R(0)};
+}
+
+// ========================================================== BVH leaf refit
+// collision_detection.py:10-44 + kdop.py:53-75: world vertices, per-triangle
+// AABB grown by the envelope, body AABB = union.  One wave per body.
+template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
+  int f = blockIdx.x;
+  if (f >= d.nb || aborted(d.err)) return;
+  int b = d.bb_body[f];
+  const ShapeInfo& sh = d.shapes[d.b_shape[b]];
+  const R env = R(d.sp[d.b_slot[b]].envelope);
+  Quat<R> q = quat(d.q[b]);
+  V3<R> r = xyz(d.r[b]);
+  const R inf = R(INFINITY);
+  R lx = inf, ly = inf, lz = inf, hx = -inf, hy = -inf, hz = -inf;
+  const int tb = d.b_tri[b];
+  for (int t = threadIdx.x; t < sh.nt; t += 64) {
+    int4 tri = d.sT[sh.t_off + t];
+    V3<R> p0 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.x])) + r;
+    V3<R> p1 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.y])) + r;
+    V3<R> p2 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.z])) + r;
+    V4<R> lo = {fmin(fmin(p0.x, p1.x), p2.x) - env, fmin(fmin(p0.y, p1.y), p2.y) - env,
+                fmin(fmin(p0.z, p1.z), p2.z) - env, R(0)};
+    V4<R> hi = {fmax(fmax(p0.x, p1.x), p2.x) + env, fmax(fmax(p0.y, p1.y), p2.y) + env,
+                fmax(fmax(p0.z, p1.z), p2.z) + env, R(0)};
+    d.tlo[tb + t] = lo;
+    d.thi[tb + t] = hi;
+    lx = fmin(lx, lo.x); ly = fmin(ly, lo.y); lz = fmin(lz, lo.z);
+    hx = fmax(hx, hi.x); hy = fmax(hy, hi.y); hz = fmax(hz, hi.z);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lx = fmin(lx, __shfl_xor(lx, o, 64)); ly = fmin(ly, __shfl_xor(ly, o, 64)); lz = fmin(lz, __shfl_xor(lz, o, 64));
+    hx = fmax(hx, __shfl_xor(hx, o, 64)); hy = fmax(hy, __shfl_xor(hy, o, 64)); hz = fmax(hz, __shfl_xor(hz, o, 64));
+  }
+  if (threadIdx.x == 0) {
+    d.b_lo[b] = V4<R>{lx, ly, lz, R(0)};
+    d.b_hi[b] = V4<R>{hx, hy, hz, R(0)};
+  }
+}
+
+// ========================================================== body pairs
+// itertools.combinations over the scene's bodies (collision_detection.py:70),
+// kept when the root volumes overlap (kdop_bvh.py:340).  One wave per row.
+template <typename R> __global__ void __launch_bounds__(64) k_pair_count(Dev<R> d) {
+  int f = blockIdx.x;
+  if (f >= d.nb) return;
+  if (aborted(d.err)) { if (threadIdx.x == 0) d.row_cnt[f] = 0; return; }
+  int b = d.bb_body[f];
+  int end = d.f_end[f];
+  V4<R> alo = d.b_lo[b], ahi = d.b_hi[b];
+  int cnt = 0;
+  for (int j = b + 1 + threadIdx.x; j < end; j += 64) cnt += box_overlap(alo, ahi, d.b_lo[j], d.b_hi[j]) ? 1 : 0;
+  cnt = wave_sum(cnt);
+  if (threadIdx.x == 0) d.row_cnt[f] = cnt;
+}
+
+template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> d) {
+  int f = blockIdx.x;
+  if (f >= d.nb || aborted(d.err)) return;
+  int b = d.bb_body[f];
+  int end = d.f_end[f];
+  int base = d.row_off[f];
+  if (threadIdx.x == 0 && f == d.nb - 1) {
+    int np = d.row_off[d.nb];
+    if (np > d.cap_pairs) { atomicOr(d.err, ERR_CAP_PAIRS); atomicMax(d.err + 1, np); }
+  }
+  if (d.row_off[d.nb] > d.cap_pairs) return;
+  V4<R> alo = d.b_lo[b], ahi = d.b_hi[b];
+  const int ntA = d.shapes[d.b_shape[b]].nt;
+  for (int j0 = b + 1; j0 < end; j0 += 64) {
+    int j = j0 + threadIdx.x;
+    bool hit = j < end && box_overlap(alo, ahi, d.b_lo[j], d.b_hi[j]);
+    unsigned long long m = __ballot(hit);
+    if (hit) {
+      int pos = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));
+      d.pairs[pos] = make_int2(b, j);
+      d.pair_sz[pos] = ntA + d.shapes[d.b_shape[j]].nt;
+    }
+    base += __popcll(m);
+  }
+}
+
+// ========================================================== candidate triangles
+// Candidate set of X w.r.t. Y: triangles of X whose AABB overlaps the AABB of
+// some triangle of Y (the BVH traversal result, collision_detection.py:71,
+// kdop_bvh.py:328-345, reduced by np.unique at :234-246).  One workgroup per
+// body pair; Y's triangles overlapping X's root volume are staged in LDS.
+constexpr int CAND_BLOCK = 256;
+
+template <typename R> struct CandLds {
+  static constexpr int CAP = (sizeof(R) == 4) ? 2000 : 1000;   // 2 x CAP x sizeof(V4<R>) <= 64000 B
+};
+
+template <typename R>
+__device__ void cand_side(const Dev<R>& d, int X, int Y, int slot_base, bool count_all, V4<R>* lo_s, V4<R>* hi_s,
+                          int* sh_scan, unsigned long long& overlaps) {
+  const int cap = CandLds<R>::CAP;
+  const ShapeInfo& sX = d.shapes[d.b_shape[X]];
+  const ShapeInfo& sY = d.shapes[d.b_shape[Y]];
+  const int tX = d.b_tri[X], tY = d.b_tri[Y];
+  const V4<R> rXlo = d.b_lo[X], rXhi = d.b_hi[X], rYlo = d.b_lo[Y], rYhi = d.b_hi[Y];
+  int ycur = 0;
+  bool first = true;
+  while (true) {
+    // stage the next run of Y triangles that overlap root(X)
+    int nfill = 0;
+    while (ycur < sY.nt) {
+      int t = ycur + threadIdx.x;
+      V4<R> lo, hi;
+      bool in = false;
+      if (t < sY.nt) {
+        lo = d.tlo[tY + t];
+        hi = d.thi[tY + t];
+        in = box_overlap(lo, hi, rXlo, rXhi);
+      }
+      int tot;
+      int pos = block_exclusive_scan(in ? 1 : 0, sh_scan, tot);
+      if (nfill + tot > cap) break;   // uniform: tot is block-wide
+      if (in) { lo_s[nfill + pos] = lo; hi_s[nfill + pos] = hi; }
+      nfill += tot;
+      ycur += CAND_BLOCK;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < sX.nt; t += CAND_BLOCK) {
+      unsigned char flag = first ? 0 : d.slot_flag[slot_base + t];
+      if (flag && !count_all) continue;
+      V4<R> lo = d.tlo[tX + t], hi = d.thi[tX + t];
+      if (!box_overlap(lo, hi, rYlo, rYhi)) { if (first) d.slot_flag[slot_base + t] = 0; continue; }
+      unsigned cnt = 0;
+      for (int e = 0; e < nfill; ++e) {
+        if (box_overlap(lo, hi, lo_s[e], hi_s[e])) {
+          flag = 1;
+          if (!count_all) break;
+          ++cnt;
+        }
+      }
+      overlaps += cnt;
+      d.slot_flag[slot_base + t] = flag;
+    }
+    __syncthreads();
+    first = false;
+    if (ycur >= sY.nt) break;
+  }
+}
+
+template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags(Dev<R> d) {
+  __shared__ V4<R> lo_s[CandLds<R>::CAP];
+  __shared__ V4<R> hi_s[CandLds<R>::CAP];
+  __shared__ int sh_scan[CAND_BLOCK / 64];
+  const int np = *d.n_pairs;
+  if (np > d.cap_pairs) return;
+  if (aborted(d.err)) return;
+  const int nslots = d.slot_off[np];
+  if (nslots > d.cap_slots) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) { atomicOr(d.err, ERR_CAP_SLOTS); atomicMax(d.err + 2, nslots); }
+    return;
+  }
+  const bool count_all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
+  for (int p = blockIdx.x; p < np; p += gridDim.x) {
+    int2 ab = d.pairs[p];
+    int base = d.slot_off[p];
+    unsigned long long ov = 0;
+    cand_side(d, ab.x, ab.y, base, count_all, lo_s, hi_s, sh_scan, ov);
+    int ntA = d.shapes[d.b_shape[ab.x]].nt;
+    unsigned long long ov2 = 0;
+    cand_side(d, ab.y, ab.x, base + ntA, false, lo_s, hi_s, sh_scan, ov2);
+    if (count_all && ov) atomicAdd(&d.ss[d.b_slot[ab.x]].n_overlaps, ov);
+  }
+}
+
+// slot s -> (pair, direction, triangle) by binary search over slot_off
+template <typename R> __global__ void k_cand_write(Dev<R> d) {
+  const int np = *d.n_pairs;
+  if (np > d.cap_pairs || aborted(d.err)) return;
+  const int ns = d.slot_off[np];
+  const int nc = d.slot_scan[ns];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *d.n_cand = nc;
+    if (nc > d.cap_cand) { atomicOr(d.err, ERR_CAP_CAND); atomicMax(d.err + 3, nc); }
+    d.pair_cstart[np] = nc;
+  }
+  if (nc > d.cap_cand) return;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (d.slot_off[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    int p = lo;
+    int local = s - d.slot_off[p];
+    if (local == 0) d.pair_cstart[p] = d.slot_scan[s];
+    if (d.slot_flag[s]) {
+      int c = d.slot_scan[s];
+      int2 ab = d.pairs[p];
+      int ntA = d.shapes[d.b_shape[ab.x]].nt;
+      int dir = local >= ntA ? 1 : 0;
+      int tri = dir ? local - ntA : local;
+      d.cand_pair[c] = p;
+      d.cand_tri[c] = tri | (dir << 31);
+    }
+  }
+}
+
+// ========================================================== Frank-Wolfe
+// collision_detection.py:120-188 per candidate triangle (one lane each).
+template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nc = *d.n_cand;
+  unsigned long long evals = 0;
+  int cur = -1;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    int p = d.cand_pair[c];
+    int tw = d.cand_tri[c];
+    int dir = (tw >> 31) & 1;
+    int t = tw & 0x7fffffff;
+    int2 ab = d.pairs[p];
+    int own = dir ? ab.y : ab.x;     // triangle owner (ContactPoint.bodyB)
+    int sdf = dir ? ab.x : ab.y;     // SDF owner (ContactPoint.bodyA)
+    const int slot = d.b_slot[own];
+    if (slot != cur) {
+      if (cur >= 0 && evals) atomicAdd(&d.ss[cur].fw_evals, evals);
+      cur = slot;
+      evals = 0;
+    }
+    const SceneParams& sp = d.sp[slot];
+    const ShapeInfo& sA = d.shapes[d.b_shape[own]];
+    GridView<R> g = grid_of(d, d.b_shape[sdf]);
+    Quat<R> qA = quat(d.q[own]), qB = quat(d.q[sdf]);
+    Quat<R> qBc = conj(qB);
+    V3<R> rA = xyz(d.r[own]), rB = xyz(d.r[sdf]);
+    int4 tri = d.sT[sA.t_off + t];
+    V3<R> P[3];
+    int vid[3] = {tri.x, tri.y, tri.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      V3<R> vw = qrotate_array(qA, xyz(d.sV[sA.v_off + vid[k]])) + rA;
+      P[k] = qrotate_array(qBc, vw - rB);
+    }
+    // start at the corner with the smallest gradient norm (first on ties)
+    R gn0 = norm(grid_gradient(g, P[0]));
+    R gn1 = norm(grid_gradient(g, P[1]));
+    R gn2 = norm(grid_gradient(g, P[2]));
+    int i0 = 0;
+    R best = gn0;
+    if (gn1 < best) { best = gn1; i0 = 1; }
+    if (gn2 < best) { i0 = 2; }
+    V3<R> x = P[i0];
+    const R tol = R(sp.fw_tol);
+    int nfw = 0;
+    for (int i = 0; i < sp.fw_max_iter; ++i) {
+      V3<R> gr = grid_gradient(g, x);
+      ++nfw;
+      R o0 = dot(P[0], gr), o1 = dot(P[1], gr), o2 = dot(P[2], gr);
+      int vtx = 0;
+      R ob = o0;
+      if (o1 < ob) { ob = o1; vtx = 1; }
+      if (o2 < ob) { ob = o2; vtx = 2; }
+      R alpha = R(2) / R(i + 2);
+      x = x + alpha * (P[vtx] - x);
+      if (ob > tol) break;
+    }
+    bool ok = false;
+    V4<R> pw4 = {R(0), R(0), R(0), R(0)}, nw4 = {R(0), R(0), R(0), R(0)};
+    evals += 3 + nfw;
+    if (grid_inside(g, x, R(0.5))) {
+      ++evals;
+      R phi = grid_value(g, x);
+      if (phi <= R(sp.envelope)) {
+        V3<R> nb = grid_gradient(g, x);
+        V3<R> nw = qrotate(qB, nb);
+        R l = norm(nw);
+        if (l > R(0)) {
+          V3<R> pw = qrotate(qB, x) + rB;
+          pw4 = V4<R>{pw.x, pw.y, pw.z, phi};
+          nw4 = V4<R>{nw.x / l, nw.y / l, nw.z / l, R(0)};
+          ok = true;
+        }
+      }
+    }
+    d.cand_p[c] = pw4;
+    d.cand_n[c] = nw4;
+    d.cand_keep[c] = ok ? 1 : 0;
+  }
+  // merge the lanes' totals per scene inside the wave, one atomic per scene
+  unsigned long long live = __ballot(cur >= 0);
+  while (live) {
+    int leader = __ffsll((long long)live) - 1;
+    int s0 = __shfl(cur, leader, 64);
+    bool mine = cur == s0 && cur >= 0;
+    unsigned long long v = mine ? evals : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((int)(threadIdx.x & 63) == leader && v) atomicAdd(&d.ss[s0].fw_evals, v);
+    live &= ~__ballot(mine);
+  }
+}
+
+// collision_detection.py:255-290: keep the first of contacts with equal
+// position within a body pair (candidates of one pair are contiguous).
+template <typename R> __global__ void k_dedup(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nc = *d.n_cand;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    if (!d.cand_keep[c]) continue;
+    int p = d.cand_pair[c];
+    int c0 = d.pair_cstart[p];
+    V4<R> pc = d.cand_p[c];
+    bool dup = false;
+    for (int e = c0; e < c; ++e) {
+      if (!d.cand_keep[e]) continue;
+      V4<R> pe = d.cand_p[e];
+      if (pe.x == pc.x && pe.y == pc.y && pe.z == pc.z) { dup = true; break; }
+    }
+    // a later lane reads this flag only for e < its own c and only after the
+    // decision above, so clearing it here cannot change another lane's test
+    // (duplicates of an earlier contact equal that earlier contact as well)
+    if (dup) d.cand_keep[c] = 2;
+  }
+}
+
+template <typename R> __global__ void k_contact_write(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nc = *d.n_cand;
+  const int nk = d.keep_scan[nc];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *d.n_contacts = nk;
+    if (nk > d.cap_contacts) { atomicOr(d.err, ERR_CAP_CONTACTS); atomicMax(d.err + 4, nk); }
+  }
+  if (nk > d.cap_contacts) return;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    if (d.cand_keep[c] != 1) continue;
+    int k = d.keep_scan[c];
+    int p = d.cand_pair[c];
+    int dir = (d.cand_tri[c] >> 31) & 1;
+    int2 ab = d.pairs[p];
+    int own = dir ? ab.y : ab.x;
+    int sdf = dir ? ab.x : ab.y;
+    d.c_ia[k] = sdf;                   // ContactPoint(bodyB, bodyA, ...) at collision_detection.py:187
+    d.c_ib[k] = own;
+    d.c_tri[k] = d.cand_tri[c] & 0x7fffffff;
+    d.c_pair[k] = p;
+    d.c_p[k] = d.cand_p[c];
+    d.c_n[k] = d.cand_n[c];
+    int slot = d.b_slot[sdf];
+    atomicMin(&d.ss[slot].k_off, k);
+    atomicAdd(&d.ss[slot].k_cnt, 1);
+    R pen = -d.cand_p[c].w;
+    if (pen > R(0)) atomic_max_nonneg(&d.ss[slot].max_pen, (double)pen);
+  }
+}
+
+// after dedup: duplicates (flag 2) become 0 so the scan counts kept contacts
+template <typename R> __global__ void k_keep01(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nc = *d.n_cand;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+    if (d.cand_keep[c] == 2) d.cand_keep[c] = 0;
+}
+
+// ========================================================== assembly
+// contacts.py:31-55, 110-287 -- frame, arms, b, r = 1/diag(J W J^T), mu
+template <typename R>
+__device__ __forceinline__ void load_W(const Dev<R>& d, int b, R& im, R W[6]) {
+  V4<R> a = d.b_W0[b], c = d.b_W1[b];
+  W[0] = a.x; W[1] = a.y; W[2] = a.z; W[3] = a.w; W[4] = c.x; W[5] = c.y; im = c.z;
+}
+
+template <typename R> __global__ void k_assemble(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nk = *d.n_contacts;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
+    int A = d.c_ia[k], B = d.c_ib[k];
+    int slot = d.b_slot[A];
+    const SceneParams& sp = d.sp[slot];
+    const R dt = scene_dt(d, slot);
+    V4<R> p4 = d.c_p[k];
+    V3<R> p = xyz(p4);
+    V3<R> rA = p - xyz(d.r[A]);
+    V3<R> rB = p - xyz(d.r[B]);
+    V3<R> s, t, n;
+    orthonormal_frame(xyz(d.c_n[k]), s, t, n);
+    // rows: (n, s, t, torsion); J_A = -[lin | rA x dir], J_B = +[lin | rB x dir]
+    V3<R> dirs[3] = {n, s, t};
+    V3<R> vA = xyz(d.v[A]), wA = xyz(d.w[A]), vB = xyz(d.v[B]), wB = xyz(d.w[B]);
+    V3<R> dlA = dt * xyz(d.b_du0[A]), daA = dt * xyz(d.b_du1[A]);
+    V3<R> dlB = dt * xyz(d.b_du0[B]), daB = dt * xyz(d.b_du1[B]);
+    R imA, imB, WA[6], WB[6];
+    load_W(d, A, imA, WA);
+    load_W(d, B, imB, WB);
+    R vrow[4], brow[4], diag[4];
+    for (int i = 0; i < 4; ++i) {
+      V3<R> lin, angA, angB;
+      if (i < 3) {
+        lin = dirs[i];
+        angA = cross(rA, dirs[i]);
+        angB = cross(rB, dirs[i]);
+      } else {
+        lin = V3<R>{R(0), R(0), R(0)};
+        angA = n;
+        angB = n;
+      }
+      // J u summed in the csr column order: A linear, A angular, B linear, B angular
+      R vv = -lin.x * vA.x;
+      vv += -lin.y * vA.y; vv += -lin.z * vA.z;
+      vv += -angA.x * wA.x; vv += -angA.y * wA.y; vv += -angA.z * wA.z;
+      vv += lin.x * vB.x; vv += lin.y * vB.y; vv += lin.z * vB.z;
+      vv += angB.x * wB.x; vv += angB.y * wB.y; vv += angB.z * wB.z;
+      R bj = -lin.x * dlA.x;
+      bj += -lin.y * dlA.y; bj += -lin.z * dlA.z;
+      bj += -angA.x * daA.x; bj += -angA.y * daA.y; bj += -angA.z * daA.z;
+      bj += lin.x * dlB.x; bj += lin.y * dlB.y; bj += lin.z * dlB.z;
+      bj += angB.x * daB.x; bj += angB.y * daB.y; bj += angB.z * daB.z;
+      vrow[i] = vv;
+      brow[i] = bj;
+      R dg = imA * dot(lin, lin) + dot(angA, sym_mul(WA, angA)) + imB * dot(lin, lin) + dot(angB, sym_mul(WB, angB));
+      diag[i] = dg == R(0) ? R(1) : dg;
+    }
+    const double* mt = d.mat + 4 * (sp.mat_off + d.b_mat[A] * sp.nmat + d.b_mat[B]);
+    R e0 = sp.use_bounce ? R(mt[3]) : R(0);
+    R g0 = R(0);
+    if (sp.use_pre_stab) {
+      R rate = R(sp.gap_reduction) / dt;
+      R upper = -R(sp.max_gap) / dt;
+      R lower = -R(sp.min_gap);
+      if (p4.w < lower && vrow[0] <= R(0)) g0 = fmax(upper, rate * p4.w);
+    }
+    V4<R> bb;
+    bb.x = ((R(1) + e0) * vrow[0] + brow[0]) + g0;
+    bb.y = (R(1) * vrow[1] + brow[1]);
+    bb.z = (R(1) * vrow[2] + brow[2]);
+    bb.w = (R(1) * vrow[3] + brow[3]);
+    d.c_rA[k] = V4<R>{rA.x, rA.y, rA.z, R(0)};
+    d.c_rB[k] = V4<R>{rB.x, rB.y, rB.z, R(0)};
+    d.c_b[k] = bb;
+    d.c_r0[k] = V4<R>{R(1) / diag[0], R(1) / diag[1], R(1) / diag[2], R(1) / diag[3]};
+    d.c_mu[k] = V4<R>{R(mt[0]), R(mt[1]), R(mt[2]), R(0)};
+  }
+}
+
+// ========================================================== body -> contacts
+template <typename R> __global__ void k_csr_count(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nk = *d.n_contacts;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
+    int A = d.c_ia[k], B = d.c_ib[k];
+    if (!d.b_fixed[A]) atomicAdd(&d.bc_cnt[d.b2f[A]], 1);
+    if (!d.b_fixed[B]) atomicAdd(&d.bc_cnt[d.b2f[B]], 1);
+  }
+}
+
+template <typename R> __global__ void k_csr_fill(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nk = *d.n_contacts;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
+    int A = d.c_ia[k], B = d.c_ib[k];
+    if (!d.b_fixed[A]) { int f = d.b2f[A]; d.bc_list[d.bc_off[f] + atomicAdd(&d.bc_fill[f], 1)] = k; }
+    if (!d.b_fixed[B]) { int f = d.b2f[B]; d.bc_list[d.bc_off[f] + atomicAdd(&d.bc_fill[f], 1)] = k; }
+  }
+}
+
+// ========================================================== colouring
+// Vivace random colouring per island (proximal_contact/blocking.py:62-111 in
+// the deterministic form described in oracle/coloring.py).  One workgroup per
+// scene; contact k of the scene has local index k - k_off.
+constexpr int COLOR_BLOCK = 256;
+
+__device__ __forceinline__ int uf_find(int* uf, int a) {
+  while (true) {
+    int p = __atomic_load_n(&uf[a], __ATOMIC_RELAXED);
+    if (p == a) return a;
+    int gp = __atomic_load_n(&uf[p], __ATOMIC_RELAXED);
+    if (gp != p) atomicCAS(&uf[a], p, gp);   // path halving; parents only decrease
+    a = p;
+  }
+}
+
+__device__ __forceinline__ void uf_union(int* uf, int a, int b) {
+  while (true) {
+    a = uf_find(uf, a);
+    b = uf_find(uf, b);
+    if (a == b) return;
+    if (a > b) { int t = a; a = b; b = t; }
+    if (atomicCAS(&uf[b], b, a) == b) return;
+  }
+}
+
+// Visit the conflict-graph neighbours of contact k (each once).
+template <typename R, typename F>
+__device__ __forceinline__ void for_neighbours(const Dev<R>& d, int k, F&& fn) {
+  int A = d.c_ia[k], B = d.c_ib[k];
+  bool fa = !d.b_fixed[A], fb = !d.b_fixed[B];
+  if (fa) {
+    int f = d.b2f[A];
+    int o = d.bc_off[f], n = d.bc_cnt[f];
+    for (int i = 0; i < n; ++i) { int j = d.bc_list[o + i]; if (j != k) fn(j); }
+  }
+  if (fb) {
+    int f = d.b2f[B];
+    int o = d.bc_off[f], n = d.bc_cnt[f];
+    for (int i = 0; i < n; ++i) {
+      int j = d.bc_list[o + i];
+      if (j == k) continue;
+      if (fa) {   // contacts on both bodies were visited through A
+        int ja = d.c_ia[j], jb = d.c_ib[j];
+        if (ja == A || jb == A) continue;
+      }
+      fn(j);
+    }
+  }
+}
+
+__device__ __forceinline__ int pal_count(const unsigned* p) {
+  int c = 0;
+  for (int w = 0; w < PALW; ++w) c += __popc(p[w]);
+  return c;
+}
+
+__device__ __forceinline__ int pal_select(const unsigned* p, int idx) {
+  for (int w = 0; w < PALW; ++w) {
+    int c = __popc(p[w]);
+    if (idx < c) {
+      unsigned m = p[w];
+      for (int i = 0; i < idx; ++i) m &= m - 1;   // drop the lowest set bits
+      return w * 32 + __ffs(m) - 1;
+    }
+    idx -= c;
+  }
+  return -1;
+}
+
+template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color(Dev<R> d) {
+  __shared__ int sh_scan[COLOR_BLOCK / 64];
+  __shared__ int s_flag;
+  __shared__ int s_nE;
+  const int bs = blockIdx.x;
+  if (aborted(d.err)) return;
+  const int slot = d.bs_slot[bs];
+  const SceneParams& sp = d.sp[slot];
+  SceneState& st = d.ss[slot];
+  const int K = st.k_cnt, k0 = st.k_off;
+  if (K == 0) { if (threadIdx.x == 0) st.ncolor = 0; return; }
+  if (sp.scheme == SCHEME_GS) {   // serial order: contact i is colour i
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) d.c_color[k0 + i] = i;
+    if (threadIdx.x == 0) st.ncolor = K;
+    return;
+  }
+  const int f0 = d.bs_foff[bs], f1 = d.bs_foff[bs + 1];
+  // islands
+  for (int f = f0 + threadIdx.x; f < f1; f += COLOR_BLOCK) d.uf[f] = f;
+  __syncthreads();
+  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    int k = k0 + i;
+    int A = d.c_ia[k], B = d.c_ib[k];
+    if (!d.b_fixed[A] && !d.b_fixed[B]) uf_union(d.uf, d.b2f[A], d.b2f[B]);
+  }
+  __syncthreads();
+  // degrees, palettes, island labels
+  if (threadIdx.x == 0) s_flag = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    int k = k0 + i;
+    int A = d.c_ia[k], B = d.c_ib[k];
+    int isl = -1;
+    if (!d.b_fixed[A]) isl = uf_find(d.uf, d.b2f[A]);
+    else if (!d.b_fixed[B]) isl = uf_find(d.uf, d.b2f[B]);
+    d.c_isl[k] = isl;
+    int deg = 0;
+    for_neighbours(d, k, [&](int) { ++deg; });
+    d.c_deg[k] = deg;
+    if (deg > PALMAX) atomicOr(&s_flag, 1);
+    unsigned* p = d.pal + (size_t)k * PALW;
+    for (int w = 0; w < PALW; ++w) {
+      int lo = w * 32;
+      unsigned m = 0;
+      if (deg >= lo + 32) m = 0xffffffffu;
+      else if (deg > lo) m = (1u << (deg - lo)) - 1u;
+      p[w] = m;
+    }
+    d.c_color[k] = 0;
+    d.c_done[k] = 0;
+  }
+  __syncthreads();
+  if (s_flag) { if (threadIdx.x == 0) { atomicOr(d.err, ERR_COLORS); } return; }
+  const unsigned long long seed = sp.seed;
+  const unsigned long long stepk = (unsigned long long)st.step_index;
+  int rnd = 0;
+  for (; rnd < MAX_ROUNDS; ++rnd) {
+    // tentative draws
+    int left = 0;
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      int k = k0 + i;
+      if (d.c_done[k]) continue;
+      ++left;
+      const unsigned* p = d.pal + (size_t)k * PALW;
+      int n = pal_count(p);
+      if (n > 0) d.c_color[k] = pal_select(p, (int)color_draw(seed, stepk, (unsigned long long)i, (unsigned long long)rnd, (unsigned)n));
+    }
+    int tot;
+    block_exclusive_scan(left, sh_scan, tot);
+    if (tot == 0) break;
+    __syncthreads();
+    // acceptance: no uncoloured neighbour drew the same colour (c_done = 2 marks this round's winners)
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      int k = k0 + i;
+      if (d.c_done[k]) continue;
+      int ck = d.c_color[k];
+      bool ok = true;
+      for_neighbours(d, k, [&](int j) { if (d.c_done[j] != 1 && d.c_color[j] == ck) ok = false; });
+      if (ok) d.c_done[k] = 2;
+    }
+    __syncthreads();
+    // winners remove their colour from the palettes of neighbours uncoloured at round start
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      int k = k0 + i;
+      if (d.c_done[k] != 2) continue;
+      int ck = d.c_color[k];
+      unsigned bit = ~(1u << (ck & 31));
+      for_neighbours(d, k, [&](int j) { if (d.c_done[j] != 1) atomicAnd(&d.pal[(size_t)j * PALW + (ck >> 5)], bit); });
+    }
+    __syncthreads();
+    // commit winners; per-island maximum colour (kept or tentative)
+    for (int f = f0 + threadIdx.x; f < f1; f += COLOR_BLOCK) d.isl_max[f] = -1;
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      int k = k0 + i;
+      if (d.c_done[k] == 2) d.c_done[k] = 1;
+      if (d.c_isl[k] >= 0) atomicMax(&d.isl_max[d.c_isl[k]], d.c_color[k]);
+    }
+    __syncthreads();
+    // uncoloured nodes with an empty palette get max+1, max+2, ... in index order
+    if (threadIdx.x == 0) { s_nE = 0; s_flag = 0; }
+    __syncthreads();
+    for (int base = 0; base < K; base += COLOR_BLOCK) {
+      int i = base + threadIdx.x;
+      int k = k0 + i;
+      bool e = false;
+      if (i < K && !d.c_done[k]) e = pal_count(d.pal + (size_t)k * PALW) == 0;
+      if (e) atomicOr(&s_flag, 1);
+    }
+    __syncthreads();
+    if (s_flag) {
+      // rare path: rank by scanning earlier contacts of the same island
+      for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+        int k = k0 + i;
+        if (d.c_done[k] || pal_count(d.pal + (size_t)k * PALW) != 0) continue;
+        int isl = d.c_isl[k];
+        int rank = 0;
+        for (int j = 0; j < i; ++j) {
+          int kj = k0 + j;
+          if (!d.c_done[kj] && d.c_isl[kj] == isl && pal_count(d.pal + (size_t)kj * PALW) == 0) ++rank;
+        }
+        d.c_deg[k] = -1 - rank;   // stash the rank until every thread has ranked
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+        int k = k0 + i;
+        if (d.c_done[k] || d.c_deg[k] >= 0) continue;
+        int rank = -1 - d.c_deg[k];
+        int c = d.isl_max[d.c_isl[k]] + 1 + rank;
+        d.c_deg[k] = 0;
+        if (c >= PALMAX) { atomicOr(d.err, ERR_COLORS); continue; }
+        d.pal[(size_t)k * PALW + (c >> 5)] |= 1u << (c & 31);
+      }
+      __syncthreads();
+    }
+  }
+  // after MAX_ROUNDS rounds: fresh colours in index order per island
+  if (rnd >= MAX_ROUNDS) {
+    for (int f = f0 + threadIdx.x; f < f1; f += COLOR_BLOCK) d.isl_max[f] = -1;
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK)
+      if (d.c_isl[k0 + i] >= 0) atomicMax(&d.isl_max[d.c_isl[k0 + i]], d.c_color[k0 + i]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      int k = k0 + i;
+      if (d.c_done[k]) continue;
+      int isl = d.c_isl[k];
+      int rank = 0;
+      for (int j = 0; j < i; ++j)
+        if (!d.c_done[k0 + j] && d.c_isl[k0 + j] == isl) ++rank;
+      d.c_deg[k] = -1 - rank;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      int k = k0 + i;
+      if (d.c_done[k]) continue;
+      int c = d.isl_max[d.c_isl[k]] + 1 + (-1 - d.c_deg[k]);
+      if (c >= PALMAX) atomicOr(d.err, ERR_COLORS);
+      d.c_color[k] = c;
+    }
+    __syncthreads();
+  }
+  // number of colours
+  __shared__ int s_max;
+  if (threadIdx.x == 0) s_max = -1;
+  __syncthreads();
+  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicMax(&s_max, d.c_color[k0 + i]);
+  __syncthreads();
+  if (threadIdx.x == 0) st.ncolor = s_max + 1;
+}
+
+// ========================================================== colour order
+// Counting sort of the scene's contacts by colour; gather the solver arrays.
+template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order(Dev<R> d) {
+  const int bs = blockIdx.x;
+  if (aborted(d.err)) return;
+  const int slot = d.bs_slot[bs];
+  SceneState& st = d.ss[slot];
+  const int K = st.k_cnt, k0 = st.k_off;
+  if (K == 0) return;
+  int* coff = d.col_off + (size_t)bs * (PALMAX + 1);
+  const bool serial = d.sp[slot].scheme == SCHEME_GS;
+  const int nc = serial ? 0 : st.ncolor;
+  if (!serial) {
+    for (int c = threadIdx.x; c <= nc; c += COLOR_BLOCK) coff[c] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicAdd(&coff[d.c_color[k0 + i] + 1], 1);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int c = 0; c < nc; ++c) coff[c + 1] += coff[c];
+    __syncthreads();
+    // stable placement: rank among same-coloured contacts with smaller index,
+    // one colour-chunk pass at a time (chunk = block size)
+    __shared__ int s_cnt[PALMAX];
+    for (int c = threadIdx.x; c < nc; c += COLOR_BLOCK) s_cnt[c] = 0;
+    __syncthreads();
+    __shared__ int s_col[COLOR_BLOCK];
+    for (int base = 0; base < K; base += COLOR_BLOCK) {
+      int i = base + threadIdx.x;
+      int c = i < K ? d.c_color[k0 + i] : -1;
+      s_col[threadIdx.x] = c;
+      __syncthreads();
+      if (i < K) {
+        int rank = 0;
+        for (int j = 0; j < (int)threadIdx.x; ++j) rank += (s_col[j] == c);
+        int pos = coff[c] + s_cnt[c] + rank;
+        d.perm[k0 + pos] = k0 + i;
+      }
+      __syncthreads();
+      if (i < K) {
+        // the last of each colour in this chunk advances the colour counter
+        bool last = true;
+        for (int j = threadIdx.x + 1; j < COLOR_BLOCK && base + j < K; ++j) if (s_col[j] == c) { last = false; break; }
+        if (last) {
+          int n = 0;
+          for (int j = 0; j <= (int)threadIdx.x; ++j) n += (s_col[j] == c);
+          s_cnt[c] += n;
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) d.perm[k0 + i] = k0 + i;
+  }
+  __syncthreads();
+  for (int pos = threadIdx.x; pos < K; pos += COLOR_BLOCK) {
+    int k = d.perm[k0 + pos];
+    V4<R> mu = d.c_mu[k];
+    V4<R> n = d.c_n[k], rA = d.c_rA[k], rB = d.c_rB[k];
+    d.s_id[k0 + pos] = make_int2(d.c_ia[k], d.c_ib[k]);
+    d.s_n[k0 + pos] = V4<R>{n.x, n.y, n.z, mu.x};
+    d.s_rA[k0 + pos] = V4<R>{rA.x, rA.y, rA.z, mu.y};
+    d.s_rB[k0 + pos] = V4<R>{rB.x, rB.y, rB.z, mu.z};
+    d.s_b[k0 + pos] = d.c_b[k];
+    d.s_r0[k0 + pos] = d.c_r0[k];
+    V4<R> z = {R(0), R(0), R(0), R(0)};
+    d.x0[k0 + pos] = z;
+    d.x1[k0 + pos] = z;
+  }
+}
+
+// ========================================================== solve
+// gauss_seidel_solver.py:14-101 with Contacts.sweep (contacts.py:66-107):
+// colours in order, every contact of a colour in parallel.  x is double
+// buffered: the sweep reads the accepted iterate and writes the other buffer,
+// so a rejected sweep is discarded by not swapping.
+constexpr int SOLVE_BLOCK = 256;
+
+template <typename R> struct ContactRow {
+  V3<R> lin[4], angA[4], angB[4];
+};
+
+template <typename R>
+__device__ __forceinline__ void contact_rows(V4<R> n4, V4<R> rA4, V4<R> rB4, ContactRow<R>& J) {
+  V3<R> s, t, n;
+  orthonormal_frame(xyz(n4), s, t, n);
+  V3<R> rA = xyz(rA4), rB = xyz(rB4);
+  J.lin[0] = n; J.lin[1] = s; J.lin[2] = t; J.lin[3] = V3<R>{R(0), R(0), R(0)};
+  J.angA[0] = cross(rA, n); J.angA[1] = cross(rA, s); J.angA[2] = cross(rA, t); J.angA[3] = n;
+  J.angB[0] = cross(rB, n); J.angB[1] = cross(rB, s); J.angB[2] = cross(rB, t); J.angB[3] = n;
+}
+
+// z = x - r (J w + b); x_n = max(0, z_n); friction by the chosen prox
+template <typename R>
+__device__ __forceinline__ V4<R> contact_prox(const Dev<R>& d, const SceneParams& sp, const ContactRow<R>& J, V4<R> xo,
+                                              V4<R> bb, V4<R> rr, V4<R> mu, V3<R> wlA, V3<R> waA, V3<R> wlB, V3<R> waB,
+                                              bool use_old_xn) {
+  R Jw[4];
+  for (int i = 0; i < 4; ++i)
+    Jw[i] = -dot(J.lin[i], wlA) - dot(J.angA[i], waA) + dot(J.lin[i], wlB) + dot(J.angB[i], waB);
+  R z0 = xo.x - rr.x * (Jw[0] + bb.x);
+  R z1 = xo.y - rr.y * (Jw[1] + bb.y);
+  R z2 = xo.z - rr.z * (Jw[2] + bb.z);
+  R z3 = xo.w - rr.w * (Jw[3] + bb.w);
+  R xn = fmax(R(0), z0);
+  R xr = use_old_xn ? xo.x : xn;
+  R os, ot, orr;
+  if (sp.friction == 1) {
+    EllipsoidParams ep{sp.ell_tol, sp.ell_expansion, sp.ell_max_iter};
+    prox_ellipsoid<R>(z1, z2, z3, mu.x, mu.y, mu.z, xr, ep, os, ot, orr);
+  } else {
+    prox_sphere<R>(z1, z2, z3, mu.x, xr, os, ot, orr);
+  }
+  return V4<R>{xn, os, ot, orr};
+}
+
+template <typename R>
+__device__ __forceinline__ void apply_dw(const Dev<R>& d, int body, R sign, const ContactRow<R>& J, const V3<R>* ang,
+                                         V4<R> dx) {
+  R im, W[6];
+  load_W(d, body, im, W);
+  R dl[3] = {R(0), R(0), R(0)};
+  V3<R> a = {R(0), R(0), R(0)};
+  R dd[4] = {dx.x, dx.y, dx.z, dx.w};
+  for (int i = 0; i < 4; ++i) {
+    R s = sign * dd[i];
+    dl[0] += J.lin[i].x * s; dl[1] += J.lin[i].y * s; dl[2] += J.lin[i].z * s;
+    a = a + s * ang[i];
+  }
+  V3<R> da = sym_mul(W, a);
+  V4<R> l = d.ww0[body], g = d.ww1[body];
+  d.ww0[body] = V4<R>{l.x + im * dl[0], l.y + im * dl[1], l.z + im * dl[2], R(0)};
+  d.ww1[body] = V4<R>{g.x + da.x, g.y + da.y, g.z + da.z, R(0)};
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double m = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sh[i]);
+  __syncthreads();
+  return m;
+}
+
+template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve(Dev<R> d) {
+  __shared__ double sh_red[SOLVE_BLOCK / 64];
+  __shared__ int s_dec;   // bit0 accept, bit1 exit
+  const int bs = blockIdx.x;
+  if (aborted(d.err)) return;
+  const int slot = d.bs_slot[bs];
+  const SceneParams& sp = d.sp[slot];
+  SceneState& st = d.ss[slot];
+  const int K = st.k_cnt, k0 = st.k_off;
+  double* res = d.residuals + sp.res_off;
+  for (int i = threadIdx.x; i < sp.max_iter; i += SOLVE_BLOCK) res[i] = INFINITY;
+  if (K == 0) {
+    if (threadIdx.x == 0) { st.solved = 0; st.iterations = 0; st.exitcode = 0; st.sweeps = 0; st.rejects = 0;
+                            st.natural_residual = 0.0; st.final_merit = 0.0; }
+    return;
+  }
+  const bool serial = sp.scheme == SCHEME_GS;
+  const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
+  const int nc = st.ncolor;
+  const int* coff = d.col_off + (size_t)bs * (PALMAX + 1);
+  const int f0 = d.bs_foff[bs], f1 = d.bs_foff[bs + 1];
+  double last_merit = INFINITY, glast = INFINITY, rscale = 1.0;
+  int acc = 0, it = 0, rejects = 0, exitcode = 0, iters = sp.max_iter, sweeps = 0;
+  double gm = 0.0;
+  for (it = 0; it < sp.max_iter; ++it) {
+    V4<R>* xa = acc ? d.x1 : d.x0;
+    V4<R>* xn = acc ? d.x0 : d.x1;
+    const R rs = R(rscale);
+    double mloc = 0.0;
+    if (!jacobi) {
+      for (int c = 0; c < nc; ++c) {
+        const int c_lo = serial ? c : coff[c], c_hi = serial ? c + 1 : coff[c + 1];
+        for (int pos = c_lo + (int)threadIdx.x; pos < c_hi; pos += SOLVE_BLOCK) {
+          int kk = k0 + pos;
+          int2 id = d.s_id[kk];
+          V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
+          ContactRow<R> J;
+          contact_rows(n4, rA4, rB4, J);
+          V4<R> r0 = d.s_r0[kk];
+          V4<R> rr = {r0.x * rs, r0.y * rs, r0.z * rs, r0.w * rs};
+          V4<R> xo = xa[kk];
+          V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
+          V3<R> wlA = xyz(d.ww0[id.x]), waA = xyz(d.ww1[id.x]), wlB = xyz(d.ww0[id.y]), waB = xyz(d.ww1[id.y]);
+          V4<R> x = contact_prox(d, sp, J, xo, d.s_b[kk], rr, mu, wlA, waA, wlB, waB, false);
+          xn[kk] = x;
+          V4<R> dx = {x.x - xo.x, x.y - xo.y, x.z - xo.z, x.w - xo.w};
+          mloc = fmax(mloc, fmax(fmax(fabs((double)x.x - (double)xo.x), fabs((double)x.y - (double)xo.y)),
+                                 fmax(fabs((double)x.z - (double)xo.z), fabs((double)x.w - (double)xo.w))));
+          if (!d.b_fixed[id.x]) apply_dw(d, id.x, R(-1), J, J.angA, dx);
+          if (!d.b_fixed[id.y]) apply_dw(d, id.y, R(1), J, J.angB, dx);
+        }
+        __syncthreads();
+      }
+    } else {
+      const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
+      for (int pos = threadIdx.x; pos < K; pos += SOLVE_BLOCK) {
+        int kk = k0 + pos;
+        int2 id = d.s_id[kk];
+        V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
+        ContactRow<R> J;
+        contact_rows(n4, rA4, rB4, J);
+        V4<R> r0 = d.s_r0[kk];
+        V4<R> rr = {r0.x * rs, r0.y * rs, r0.z * rs, r0.w * rs};
+        V4<R> xo = xa[kk];
+        V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
+        V3<R> wlA = xyz(d.ws0[id.x]), waA = xyz(d.ws1[id.x]), wlB = xyz(d.ws0[id.y]), waB = xyz(d.ws1[id.y]);
+        V4<R> x = contact_prox(d, sp, J, xo, d.s_b[kk], rr, mu, wlA, waA, wlB, waB, !hybrid);
+        xn[kk] = x;
+        mloc = fmax(mloc, fmax(fmax(fabs((double)x.x - (double)xo.x), fabs((double)x.y - (double)xo.y)),
+                               fmax(fabs((double)x.z - (double)xo.z), fabs((double)x.w - (double)xo.w))));
+      }
+      __syncthreads();
+      for (int c = 0; c < nc; ++c) {
+        for (int pos = coff[c] + (int)threadIdx.x; pos < coff[c + 1]; pos += SOLVE_BLOCK) {
+          int kk = k0 + pos;
+          int2 id = d.s_id[kk];
+          ContactRow<R> J;
+          contact_rows(d.s_n[kk], d.s_rA[kk], d.s_rB[kk], J);
+          V4<R> x = xn[kk], xo = xa[kk];
+          V4<R> dx = {x.x - xo.x, x.y - xo.y, x.z - xo.z, x.w - xo.w};
+          if (!d.b_fixed[id.x]) apply_dw(d, id.x, R(-1), J, J.angA, dx);
+          if (!d.b_fixed[id.y]) apply_dw(d, id.y, R(1), J, J.angB, dx);
+        }
+        __syncthreads();
+      }
+    }
+    ++sweeps;
+    double merit = block_max(mloc, sh_red);
+    if (threadIdx.x == 0) {
+      int dec = 0;
+      if (merit > last_merit) {
+        rscale *= sp.nu_reduce;
+        gm = fmax(0.0, last_merit);
+        ++rejects;
+      } else {
+        dec |= 1;
+        if (last_merit - merit < sp.too_small) rscale *= sp.nu_increase;
+        last_merit = merit;
+        gm = fmax(0.0, merit);
+      }
+      res[it] = gm;
+      if (gm < sp.abs_tol) { dec |= 2; exitcode = 1; }
+      else if (fabs(gm - glast) < sp.rel_tol * glast) { dec |= 2; exitcode = 2; }
+      glast = gm;
+      s_dec = dec;
+    }
+    __syncthreads();
+    int dec = s_dec;
+    // every thread tracks the same scalars
+    if (threadIdx.x != 0) {
+      if (dec & 1) { if (last_merit - merit < sp.too_small) rscale *= sp.nu_increase; last_merit = merit; }
+      else { rscale *= sp.nu_reduce; }
+    }
+    if (dec & 1) {
+      acc ^= 1;
+      for (int f = f0 + threadIdx.x; f < f1; f += SOLVE_BLOCK) {
+        int b = d.bb_body[f];
+        d.ws0[b] = d.ww0[b]; d.ws1[b] = d.ww1[b];
+      }
+    } else {
+      for (int f = f0 + threadIdx.x; f < f1; f += SOLVE_BLOCK) {
+        int b = d.bb_body[f];
+        d.ww0[b] = d.ws0[b]; d.ww1[b] = d.ws1[b];
+      }
+    }
+    __syncthreads();
+    if (dec & 2) { iters = it; break; }
+  }
+  // accepted impulses back to canonical order; natural-map residual in FP64
+  V4<R>* xs = acc ? d.x1 : d.x0;
+  double rloc = 0.0, xmax = 0.0;
+  for (int pos = threadIdx.x; pos < K; pos += SOLVE_BLOCK) {
+    int kk = k0 + pos;
+    V4<R> x = xs[kk];
+    d.c_imp[d.perm[kk]] = x;
+    int2 id = d.s_id[kk];
+    V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
+    ContactRow<R> J;
+    contact_rows(n4, rA4, rB4, J);
+    V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
+    V3<R> wlA = xyz(d.ws0[id.x]), waA = xyz(d.ws1[id.x]), wlB = xyz(d.ws0[id.y]), waB = xyz(d.ws1[id.y]);
+    V4<R> y = contact_prox(d, sp, J, x, d.s_b[kk], d.s_r0[kk], mu, wlA, waA, wlB, waB, false);
+    rloc = fmax(rloc, fmax(fmax(fabs((double)x.x - (double)y.x), fabs((double)x.y - (double)y.y)),
+                           fmax(fabs((double)x.z - (double)y.z), fabs((double)x.w - (double)y.w))));
+    xmax = fmax(xmax, fmax(fmax(fabs((double)x.x), fabs((double)x.y)), fmax(fabs((double)x.z), fabs((double)x.w))));
+  }
+  double rmax = block_max(rloc, sh_red);
+  double xm = block_max(xmax, sh_red);
+  if (threadIdx.x == 0) {
+    st.solved = 1;
+    st.iterations = iters;
+    st.exitcode = exitcode;
+    st.rejects = rejects;
+    st.sweeps = sweeps;
+    st.final_merit = last_merit;
+    st.natural_residual = rmax / fmax(1.0, xm);
+  }
+}
+
+// ========================================================== integration
+// state_storage.py:239-302: u += dt W f_ext + W J^T sol; r += v dt;
+// q <- unit(q + (0, w) q dt 0.5); fixed bodies untouched.
+template <typename R> __global__ void k_integrate(Dev<R> d) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= d.nb || aborted(d.err)) return;
+  int b = d.bb_body[f];
+  if (d.b_fixed[b]) return;
+  const R dt = scene_dt(d, d.b_slot[b]);
+  V4<R> du0 = d.b_du0[b], du1 = d.b_du1[b], s0 = d.ws0[b], s1 = d.ws1[b];
+  V4<R> v = d.v[b], w = d.w[b];
+  v.x = v.x + (dt * du0.x + s0.x); v.y = v.y + (dt * du0.y + s0.y); v.z = v.z + (dt * du0.z + s0.z);
+  w.x = w.x + (dt * du1.x + s1.x); w.y = w.y + (dt * du1.y + s1.y); w.z = w.z + (dt * du1.z + s1.z);
+  V4<R> r = d.r[b];
+  r.x = r.x + v.x * dt; r.y = r.y + v.y * dt; r.z = r.z + v.z * dt;
+  Quat<R> q = quat(d.q[b]);
+  Quat<R> dq = qprod(Quat<R>{R(0), w.x, w.y, w.z}, q);
+  Quat<R> qn = {q.s + dq.s * dt * R(0.5), q.x + dq.x * dt * R(0.5), q.y + dq.y * dt * R(0.5), q.z + dq.z * dt * R(0.5)};
+  qn = qunit(qn);
+  d.v[b] = v; d.w[b] = w; d.r[b] = r;
+  d.q[b] = V4<R>{qn.s, qn.x, qn.y, qn.z};
+}
+
+template <typename R> __global__ void k_scene_reset(Dev<R> d) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.ns) return;
+  SceneState& st = d.ss[d.bs_slot[s]];
+  st.k_off = 0x7fffffff; st.k_cnt = 0; st.ncolor = 0; st.n_overlaps = 0; st.fw_evals = 0; st.max_pen = 0.0;
+  st.solved = 0;
+}
+
+template <typename R> __global__ void k_scene_finalize(Dev<R> d) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.ns || aborted(d.err)) return;
+  SceneState& st = d.ss[d.bs_slot[s]];
+  if (st.k_cnt == 0) st.k_off = 0;
+  st.step_index += 1;
+}
+
+}  // namespace rbh

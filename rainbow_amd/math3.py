@@ -1,0 +1,109 @@
+"""Quaternion and vector helpers used by the host-side scene API.
+
+Same conventions as rainbow/math/quaternion.py and vector3.py: q = [s, x, y, z],
+rotation r -> (q (0, r) q*)[1:].
+"""
+from math import cos, sin, sqrt
+
+import numpy as np
+
+
+def identity():
+    return np.array([1.0, 0.0, 0.0, 0.0])
+
+
+def unit(q):
+    q = np.asarray(q, dtype=np.float64)
+    n = np.linalg.norm(q)
+    return q / n if n > 0 else identity()
+
+
+def conjugate(q):
+    return np.array([q[0], -q[1], -q[2], -q[3]], dtype=np.float64)
+
+
+def prod(a, b):
+    s = a[0] * b[0] - np.dot(a[1:], b[1:])
+    v = a[0] * b[1:] + a[1:] * b[0] + np.cross(a[1:], b[1:])
+    return np.array([s, v[0], v[1], v[2]], dtype=np.float64)
+
+
+def rotate(q, r):
+    t = prod(prod(q, np.array([0.0, r[0], r[1], r[2]])), conjugate(q))
+    return t[1:].copy()
+
+
+def rotate_array(q, V):
+    """Rotate each row of V (quaternion sandwich written out per component)."""
+    V = np.asarray(V, dtype=np.float64)
+    s, x, y, z = q
+    vx, vy, vz = V[:, 0], V[:, 1], V[:, 2]
+    i0 = s * 0.0 - x * vx - y * vy - z * vz
+    i1 = s * vx + x * 0.0 + y * vz - z * vy
+    i2 = s * vy - x * vz + y * 0.0 + z * vx
+    i3 = s * vz + x * vy - y * vx + z * 0.0
+    bs, bx, by, bz = s, -x, -y, -z
+    return np.stack([i0 * bx + i1 * bs + i2 * bz - i3 * by,
+                     i0 * by - i1 * bz + i2 * bs + i3 * bx,
+                     i0 * bz + i1 * by - i2 * bx + i3 * bs], axis=1)
+
+
+def Ru(radians, axis):
+    c, s = cos(radians / 2.0), sin(radians / 2.0)
+    n = np.asarray(axis, dtype=np.float64) / np.linalg.norm(axis)
+    return np.array([c, s * n[0], s * n[1], s * n[2]])
+
+
+def Rx(radians):
+    return np.array([cos(radians / 2.0), sin(radians / 2.0), 0.0, 0.0])
+
+
+def Ry(radians):
+    return np.array([cos(radians / 2.0), 0.0, sin(radians / 2.0), 0.0])
+
+
+def Rz(radians):
+    return np.array([cos(radians / 2.0), 0.0, 0.0, sin(radians / 2.0)])
+
+
+def to_matrix(q):
+    s, x, y, z = unit(q)
+    return np.array([
+        [1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - s * z), 2.0 * (s * y + x * z)],
+        [2.0 * (x * y + s * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (z * y - s * x)],
+        [2.0 * (-s * y + x * z), 2.0 * (z * y + s * x), 1.0 - 2.0 * (y * y + x * x)]])
+
+
+def from_matrix(M):
+    """Rotation matrix to quaternion (largest-component branch selection)."""
+    M = np.asarray(M, dtype=np.float64)
+    tr = M[0, 0] + M[1, 1] + M[2, 2]
+    if tr < 0.0:
+        i = int(np.argmax([M[0, 0], M[1, 1], M[2, 2]]))
+        if M[1, 1] > M[0, 0]:
+            i = 1
+        if M[2, 2] > M[i, i]:
+            i = 2
+        if i == 0:
+            r = sqrt((M[0, 0] - (M[1, 1] + M[2, 2])) + 1.0)
+            qx = 0.5 * r
+            r = 0.5 / r
+            return np.array([(M[2, 1] - M[1, 2]) * r, qx, (M[0, 1] + M[1, 0]) * r, (M[2, 0] + M[0, 2]) * r])
+        if i == 1:
+            r = sqrt((M[1, 1] - (M[2, 2] + M[0, 0])) + 1.0)
+            qy = 0.5 * r
+            r = 0.5 / r
+            return np.array([(M[0, 2] - M[2, 0]) * r, (M[0, 1] + M[1, 0]) * r, qy, (M[1, 2] + M[2, 1]) * r])
+        r = sqrt((M[2, 2] - (M[0, 0] + M[1, 1])) + 1.0)
+        qz = 0.5 * r
+        r = 0.5 / r
+        return np.array([(M[1, 0] - M[0, 1]) * r, (M[2, 0] + M[0, 2]) * r, (M[1, 2] + M[2, 1]) * r, qz])
+    r = sqrt(tr + 1.0)
+    qs = 0.5 * r
+    r = 0.5 / r
+    return np.array([qs, (M[2, 1] - M[1, 2]) * r, (M[0, 2] - M[2, 0]) * r, (M[1, 0] - M[0, 1]) * r])
+
+
+def unit3(a):
+    a = np.asarray(a, dtype=np.float64)
+    return a / np.linalg.norm(a)

@@ -43,6 +43,8 @@ extern "C" {
 #define RBH_STEP_COUNT_OVERLAPS 1          /* count overlapping triangle pairs (profile key number_of_overlaps) */
 #define RBH_STEP_TIMING 2                  /* record per-stage GPU times for rbh_get_timings */
 #define RBH_STEP_GRAPH 4                   /* capture the launch sequence in a hipGraph and replay it */
+#define RBH_STEP_FORCE_SINGLE 8            /* solve each scene on one workgroup */
+#define RBH_STEP_FORCE_MULTI 16            /* solve a lone scene on many workgroups (islands in parallel) */
 
 typedef struct rbh_ctx rbh_ctx;
 

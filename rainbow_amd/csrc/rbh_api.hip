@@ -117,9 +117,12 @@ struct rbh_ctx {
   DBuf row_cnt, row_off, pairs, pair_sz, slot_off, n_pairs, slot_flag, slot_scan, n_slots;
   DBuf cand_pair, cand_tri, pair_cstart, n_cand, cand_p, cand_n, cand_keep, keep_scan;
   DBuf c_ia, c_ib, c_tri, c_pair, c_p, c_n, n_contacts, c_rA, c_rB, c_b, c_r0, c_mu, c_imp, c_color;
-  DBuf bc_cnt, bc_off, bc_fill, bc_list, uf, isl_max, pal, c_deg, c_isl, c_done, col_off;
+  DBuf f_bs, uf, isl_cnt, isl_off, isl_fill, isl_max, c_lab, uperm, u_cnt, u_base, n_units, u_start, u_end, u_ncol,
+      u_bs, body_claim, body_loc, bc_cnt, bc_off, bc_fill, bc_list, pal, g_pc, g_done, col_off, merit, bar;
   DBuf perm, s_id, s_n, s_rA, s_rB, s_b, s_r0, x0, x1, errbuf, scan_part;
-  int cap_pairs = 0, cap_slots = 0, cap_cand = 0, cap_contacts = 0;
+  int cap_pairs = 0, cap_slots = 0, cap_cand = 0, cap_contacts = 0, cap_units = 0, unit_T = 2048;
+  int max_iter_batch = 1, coop_max = 0, last_units = 0;
+  bool multi_ok = false;
   long long ws_gen = 0;
   // graph
   hipGraph_t graph = nullptr;
@@ -169,36 +172,30 @@ int ensure(rbh_ctx* c, DBuf& b, size_t bytes, bool keep = false) {
   return 0;
 }
 
+std::vector<DBuf*> all_bufs(rbh_ctx* c) {
+  return {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
+          &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
+          &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi,
+          &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
+          &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs,
+          &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
+          &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->c_ia, &c->c_ib, &c->c_tri,
+          &c->c_pair, &c->c_p, &c->c_n, &c->n_contacts, &c->c_rA, &c->c_rB, &c->c_b, &c->c_r0, &c->c_mu,
+          &c->c_imp, &c->c_color, &c->f_bs, &c->uf, &c->isl_cnt, &c->isl_off, &c->isl_fill, &c->isl_max,
+          &c->c_lab, &c->uperm, &c->u_cnt, &c->u_base, &c->n_units, &c->u_start, &c->u_end, &c->u_ncol,
+          &c->u_bs, &c->body_claim, &c->body_loc, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->pal,
+          &c->g_pc, &c->g_done, &c->col_off, &c->merit, &c->bar, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
+          &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part};
+}
+
 size_t total_bytes(rbh_ctx* c) {
   size_t t = 0;
-  DBuf* all[] = {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
-                 &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
-                 &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi,
-                 &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
-                 &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs,
-                 &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
-                 &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->c_ia, &c->c_ib, &c->c_tri,
-                 &c->c_pair, &c->c_p, &c->c_n, &c->n_contacts, &c->c_rA, &c->c_rB, &c->c_b, &c->c_r0, &c->c_mu,
-                 &c->c_imp, &c->c_color, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->uf, &c->isl_max,
-                 &c->pal, &c->c_deg, &c->c_isl, &c->c_done, &c->col_off, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
-                 &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part};
-  for (DBuf* b : all) t += b->bytes;
+  for (DBuf* b : all_bufs(c)) t += b->bytes;
   return t;
 }
 
 void free_all(rbh_ctx* c) {
-  DBuf* all[] = {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
-                 &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
-                 &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi,
-                 &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
-                 &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs,
-                 &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
-                 &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->c_ia, &c->c_ib, &c->c_tri,
-                 &c->c_pair, &c->c_p, &c->c_n, &c->n_contacts, &c->c_rA, &c->c_rB, &c->c_b, &c->c_r0, &c->c_mu,
-                 &c->c_imp, &c->c_color, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->uf, &c->isl_max,
-                 &c->pal, &c->c_deg, &c->c_isl, &c->c_done, &c->col_off, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
-                 &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part};
-  for (DBuf* b : all) {
+  for (DBuf* b : all_bufs(c)) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -353,10 +350,15 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags) {
   d.cap_contacts = c->cap_contacts;
   d.c_rA = P(V4<R>*, c_rA); d.c_rB = P(V4<R>*, c_rB); d.c_b = P(V4<R>*, c_b); d.c_r0 = P(V4<R>*, c_r0);
   d.c_mu = P(V4<R>*, c_mu); d.c_imp = P(V4<R>*, c_imp); d.c_color = P(int*, c_color);
+  d.f_bs = P(const int*, f_bs); d.uf = P(int*, uf); d.isl_cnt = P(int*, isl_cnt); d.isl_off = P(int*, isl_off);
+  d.isl_fill = P(int*, isl_fill); d.isl_max = P(int*, isl_max); d.c_lab = P(int*, c_lab); d.uperm = P(int*, uperm);
+  d.u_cnt = P(int*, u_cnt); d.u_base = P(int*, u_base); d.n_units = P(int*, n_units); d.cap_units = c->cap_units;
+  d.unit_T = c->unit_T; d.u_start = P(int*, u_start); d.u_end = P(int*, u_end); d.u_ncol = P(int*, u_ncol);
+  d.u_bs = P(int*, u_bs); d.body_claim = P(int*, body_claim); d.body_loc = P(int2*, body_loc);
   d.bc_cnt = P(int*, bc_cnt); d.bc_off = P(int*, bc_off); d.bc_fill = P(int*, bc_fill);
-  d.bc_list = P(int*, bc_list); d.uf = P(int*, uf); d.isl_max = P(int*, isl_max); d.pal = P(unsigned*, pal);
-  d.c_deg = P(int*, c_deg); d.c_isl = P(int*, c_isl); d.c_done = P(unsigned char*, c_done);
-  d.col_off = P(int*, col_off);
+  d.bc_list = P(int*, bc_list); d.pal = P(unsigned*, pal); d.g_pc = P(int*, g_pc);
+  d.g_done = P(unsigned char*, g_done); d.col_off = P(int*, col_off); d.merit = P(double*, merit);
+  d.bar = P(unsigned*, bar);
   d.perm = P(int*, perm); d.s_id = P(int2*, s_id); d.s_n = P(V4<R>*, s_n); d.s_rA = P(V4<R>*, s_rA);
   d.s_rB = P(V4<R>*, s_rB); d.s_b = P(V4<R>*, s_b); d.s_r0 = P(V4<R>*, s_r0); d.x0 = P(V4<R>*, x0);
   d.x1 = P(V4<R>*, x1);
@@ -373,7 +375,9 @@ int size_workspace(rbh_ctx* c) {
   const size_t V = 4 * (size_t)c->prec;
   const size_t cp = c->cap_pairs, cs = c->cap_slots, cc = c->cap_cand, ck = c->cap_contacts;
   const size_t nb = std::max(c->nb, 1), ns = std::max(c->ns, 1);
-  int maxcap = std::max({c->cap_pairs, c->cap_slots, c->cap_cand, c->cap_contacts, c->nb});
+  c->cap_units = (int)std::min<long long>((long long)(c->cap_contacts + c->unit_T - 1) / c->unit_T + c->ns + 1, 1 << 20);
+  const size_t nu = c->cap_units, nl = (size_t)c->nb + c->ns + 2;
+  int maxcap = std::max({c->cap_pairs, c->cap_slots, c->cap_cand, c->cap_contacts, c->nb + c->ns + 2, c->ns});
   struct E { DBuf* b; size_t bytes; } el[] = {
       {&c->row_cnt, nb * 4}, {&c->row_off, (nb + 1) * 4},
       {&c->pairs, cp * 8}, {&c->pair_sz, cp * 4}, {&c->slot_off, (cp + 1) * 4}, {&c->n_pairs, 16},
@@ -383,9 +387,14 @@ int size_workspace(rbh_ctx* c) {
       {&c->c_ia, ck * 4}, {&c->c_ib, ck * 4}, {&c->c_tri, ck * 4}, {&c->c_pair, ck * 4},
       {&c->c_p, ck * V}, {&c->c_n, ck * V}, {&c->n_contacts, 16}, {&c->c_rA, ck * V}, {&c->c_rB, ck * V},
       {&c->c_b, ck * V}, {&c->c_r0, ck * V}, {&c->c_mu, ck * V}, {&c->c_imp, ck * V}, {&c->c_color, ck * 4},
-      {&c->bc_cnt, nb * 4}, {&c->bc_off, (nb + 1) * 4}, {&c->bc_fill, nb * 4}, {&c->bc_list, 2 * ck * 4},
-      {&c->uf, nb * 4}, {&c->isl_max, nb * 4}, {&c->pal, ck * PALW * 4}, {&c->c_deg, ck * 4},
-      {&c->c_isl, ck * 4}, {&c->c_done, ck + 16}, {&c->col_off, ns * (PALMAX + 1) * 4},
+      {&c->f_bs, nb * 4}, {&c->uf, nb * 4}, {&c->isl_cnt, nl * 4}, {&c->isl_off, (nl + 1) * 4},
+      {&c->isl_fill, nl * 4}, {&c->isl_max, nl * 4}, {&c->c_lab, ck * 4}, {&c->uperm, ck * 4},
+      {&c->u_cnt, ns * 4}, {&c->u_base, (ns + 1) * 4}, {&c->n_units, 16}, {&c->u_start, nu * 4},
+      {&c->u_end, nu * 4}, {&c->u_ncol, nu * 4}, {&c->u_bs, nu * 4}, {&c->body_claim, nb * 4},
+      {&c->body_loc, nb * 8}, {&c->bc_cnt, nb * 4}, {&c->bc_off, (nb + 1) * 4}, {&c->bc_fill, nb * 4},
+      {&c->bc_list, 2 * ck * 4}, {&c->pal, ck * PALW * 4}, {&c->g_pc, ck * 4}, {&c->g_done, ck + 16},
+      {&c->col_off, std::max(ns, nu) * (PALMAX + 1) * 4}, {&c->merit, (size_t)std::max(c->max_iter_batch, 1) * 8},
+      {&c->bar, 64},
       {&c->perm, ck * 4}, {&c->s_id, ck * 8}, {&c->s_n, ck * V}, {&c->s_rA, ck * V}, {&c->s_rB, ck * V},
       {&c->s_b, ck * V}, {&c->s_r0, ck * V}, {&c->x0, ck * V}, {&c->x1, ck * V}, {&c->errbuf, 64},
       {&c->scan_part, (size_t)scan_parts(maxcap) * 4}};
@@ -400,9 +409,11 @@ int build_batch(rbh_ctx* c, const int* ids, int n) {
   std::vector<int> want(ids, ids + n);
   if (want == c->batch && c->batch_arena_gen == c->arena_gen) return 0;
   drop_graph(c);
-  std::vector<int> bb, bsslot, bsfoff, fend;
+  std::vector<int> bb, bsslot, bsfoff, fend, fbs;
   std::vector<int> b2f(std::max(c->bodies.cap, 1), -1);
   long long pairs_all = 0, tri_sum = 0;
+  int max_iter = 1;
+  bool serial = false;
   for (int i = 0; i < n; ++i) {
     const SceneHost& s = c->scenes[want[i]];
     bsslot.push_back(want[i]);
@@ -411,21 +422,31 @@ int build_batch(rbh_ctx* c, const int* ids, int n) {
       b2f[s.body_off + b] = (int)bb.size();
       bb.push_back(s.body_off + b);
       fend.push_back(s.body_off + s.nbody);
+      fbs.push_back(i);
     }
     pairs_all += (long long)s.nbody * (s.nbody - 1) / 2;
     tri_sum += s.ntri;
+    max_iter = std::max(max_iter, s.params.max_iterations);
+    serial = serial || s.params.proximal_solver == RBH_SCHEME_GAUSS_SEIDEL;
   }
   bsfoff.push_back((int)bb.size());
   c->nb = (int)bb.size();
   c->ns = n;
-  std::vector<int> nbv = {c->nb, 0, 0, 0};
+  std::vector<int> nbv = {c->nb, c->nb + c->ns + 1, c->ns, 0};
   int rc = upload(c, c->bb_body, bb);
   if (!rc) rc = upload(c, c->bs_slot, bsslot);
   if (!rc) rc = upload(c, c->bs_foff, bsfoff);
   if (!rc) rc = upload(c, c->b2f, b2f);
   if (!rc) rc = upload(c, c->f_end, fend);
+  if (!rc) rc = upload(c, c->f_bs, fbs);
   if (!rc) rc = upload(c, c->d_nb, nbv);
   if (rc) return rc;
+  // one scene: colour small units (islands) and solve them on many
+  // workgroups; batches: one workgroup per scene
+  c->max_iter_batch = max_iter;
+  c->multi_ok = (n == 1 && !serial);
+  c->unit_T = c->multi_ok ? 128 : UNIT_CAP;
+  c->last_units = 0;
   // initial capacities; they grow when a step reports an overflow
   long long est_pairs = std::min<long long>(pairs_all, 64LL * c->nb + 4096);
   c->cap_pairs = std::max(c->cap_pairs, (int)std::max<long long>(est_pairs, 1));
@@ -458,7 +479,7 @@ template <typename T> void scan(rbh_ctx* c, const T* in, int* out, const int* n_
   hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, part, out);
 }
 
-template <typename R> void launch_step(rbh_ctx* c, double dt, int flags, bool timing) {
+template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, bool timing, bool multi, int multi_grid) {
   Dev<R> d = make_dev<R>(c, dt, flags);
   hipStream_t s = c->stream;
   const int nb = c->nb, ns = c->ns;
@@ -487,19 +508,40 @@ template <typename R> void launch_step(rbh_ctx* c, double dt, int flags, bool ti
   mark(ST_FW);
   hipLaunchKernelGGL(k_assemble<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);
   mark(ST_ASM);
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(nb, 256)), dim3(256), 0, s, d.bc_cnt, nb, 0);
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(nb, 256)), dim3(256), 0, s, d.bc_fill, nb, 0);
-  hipLaunchKernelGGL(k_csr_count<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);
+  // islands -> colouring units -> body lists in unit order
+  const int G = grid_for(c->cap_contacts, 256);
+  hipLaunchKernelGGL(k_uf_init<R>, dim3(grid_for(nb + ns + 1, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_uf_union<R>, dim3(G), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_isl_count<R>, dim3(G), dim3(256), 0, s, d);
+  scan<int>(c, d.isl_cnt, d.isl_off, d.d_nb + 1, nb + ns + 1, nullptr);
+  hipLaunchKernelGGL(k_isl_fill<R>, dim3(G), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_unit_count<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
+  scan<int>(c, d.u_cnt, d.u_base, d.d_nb + 2, ns, d.n_units);
+  hipLaunchKernelGGL(k_unit_reset<R>, dim3(grid_for(c->cap_units, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_unit_assign<R>, dim3(grid_for(nb + ns, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_csr_count<R>, dim3(G), dim3(256), 0, s, d);
   scan<int>(c, d.bc_cnt, d.bc_off, d.d_nb, nb, nullptr);
-  hipLaunchKernelGGL(k_csr_fill<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(k_color<R>, dim3(ns), dim3(COLOR_BLOCK), 0, s, d);
-  hipLaunchKernelGGL(k_order<R>, dim3(ns), dim3(COLOR_BLOCK), 0, s, d);
-  mark(ST_COLOR);
-  hipLaunchKernelGGL(k_solve<R>, dim3(ns), dim3(SOLVE_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_csr_fill<R>, dim3(G), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_color_units<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
+  if (multi) {
+    hipLaunchKernelGGL(k_order_units<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
+    mark(ST_COLOR);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(d.merit), 2 * c->max_iter_batch, 0);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(d.bar), 16, 0);
+    void* args[] = {&d};
+    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<void*>(k_solve_multi<R>), dim3(multi_grid),
+                                              dim3(SOLVE_BLOCK), args, 0, s);
+    if (e != hipSuccess) return e;
+  } else {
+    hipLaunchKernelGGL(k_order<R>, dim3(ns), dim3(COLOR_BLOCK), 0, s, d);
+    mark(ST_COLOR);
+    hipLaunchKernelGGL(k_solve<R>, dim3(ns), dim3(SOLVE_BLOCK), 0, s, d);
+  }
   mark(ST_SOLVE);
   hipLaunchKernelGGL(k_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_scene_finalize<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
   mark(ST_INT);
+  return hipGetLastError();
 }
 
 int read_err(rbh_ctx* c, int e[8]) {
@@ -521,6 +563,16 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
   const bool timing = (flags & RBH_STEP_TIMING) != 0;
   const bool use_graph = (flags & RBH_STEP_GRAPH) != 0;
   const int kflags = flags & RBH_STEP_COUNT_OVERLAPS;
+  if (c->coop_max == 0) {
+    int per_cu = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_multi<R>),
+                                                       SOLVE_BLOCK, 0);
+    c->coop_max = std::max(1, per_cu) * c->prop.multiProcessorCount;
+  }
+  bool multi = c->multi_ok && !use_graph && c->last_units != 1;
+  if (flags & RBH_STEP_FORCE_SINGLE) multi = false;
+  if ((flags & RBH_STEP_FORCE_MULTI) && c->multi_ok && !use_graph) multi = true;
+  const int multi_grid = std::max(1, std::min(c->last_units > 0 ? c->last_units : 128, c->coop_max));
   int done = 0;
   for (int attempt = 0; attempt < 16 && done < n_steps; ++attempt) {
     int si0 = 0;
@@ -531,7 +583,11 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
       if (!c->gexec || c->graph_key_gen != c->arena_gen || c->graph_dt != dt || c->graph_flags != kflags) {
         drop_graph(c);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        launch_step<R>(c, dt, kflags, false);
+        hipError_t le = launch_step<R>(c, dt, kflags, false, false, 1);
+        if (le != hipSuccess) {
+          (void)hipStreamEndCapture(c->stream, &c->graph);
+          return fail(c, RBH_E_HIP, std::string("step launch: ") + hipGetErrorString(le));
+        }
         HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph));
         HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
         c->graph_key_gen = c->arena_gen;
@@ -540,8 +596,10 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
       }
       for (int i = 0; i < todo; ++i) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     } else {
-      for (int i = 0; i < todo; ++i) launch_step<R>(c, dt, kflags, timing && i == todo - 1);
-      HIPCHK(c, hipGetLastError());
+      for (int i = 0; i < todo; ++i) {
+        hipError_t le = launch_step<R>(c, dt, kflags, timing && i == todo - 1, multi, multi_grid);
+        if (le != hipSuccess) return fail(c, RBH_E_HIP, std::string("step launch: ") + hipGetErrorString(le));
+      }
     }
     int e[8];
     rc = read_err(c, e);
@@ -560,6 +618,8 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
     if (e[0] & ERR_CAP_SLOTS) c->cap_slots = grow(c->cap_slots, e[2]);
     if (e[0] & ERR_CAP_CAND) c->cap_cand = grow(c->cap_cand, e[3]);
     if (e[0] & ERR_CAP_CONTACTS) c->cap_contacts = grow(c->cap_contacts, e[4]);
+    if (e[0] & ERR_CAP_UNITS) c->cap_contacts = grow(c->cap_contacts, e[5] * c->unit_T);
+    if (e[0] & ERR_SOLVER) return fail(c, RBH_E_HIP, "grid barrier of the solver timed out");
     c->cap_cand = std::max(c->cap_cand, std::min(c->cap_slots, c->cap_cand));
     drop_graph(c);
     c->arena_gen++;
@@ -568,6 +628,10 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
     if (rc) return rc;
   }
   if (done < n_steps) return fail(c, RBH_E_CAPACITY, "step workspace could not grow enough");
+  int nu = 0;
+  HIPCHK(c, hipMemcpyAsync(&nu, c->n_units.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->last_units = std::max(1, nu);
   // batch counters
   int h[4] = {0, 0, 0, 0};
   HIPCHK(c, hipMemcpyAsync(&h[0], c->n_pairs.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -1023,7 +1087,7 @@ int rbh_get_log(rbh_ctx* c, int id, rbh_step_log* log, double* residuals, int ca
     log->solved = st.solved;
     log->number_of_overlaps = st.n_overlaps;
     log->fw_cell_evals = st.fw_evals;
-    log->natural_residual = st.natural_residual;
+    log->natural_residual = st.nat_r / std::max(1.0, st.nat_x);
     log->max_penetration = st.max_pen;
     log->final_merit = st.final_merit;
   }

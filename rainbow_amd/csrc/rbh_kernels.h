@@ -37,7 +37,8 @@ struct SceneState {
   int n_pairs, n_cand, solved, pad;
   unsigned long long n_overlaps;
   unsigned long long fw_evals;   // distinct trilinear cell evaluations in Frank-Wolfe
-  double natural_residual, max_pen, final_merit, pad2;
+  double nat_r, nat_x;     // natural-map residual: max |x - prox(...)| and max |x| (FP64)
+  double max_pen, final_merit;
 };
 
 constexpr int PALW = 64;                 // palette words per contact
@@ -46,7 +47,7 @@ constexpr int MAX_ROUNDS = 200;
 
 // error flags (dev.err[0] bits); capacity overflows record the needed size
 enum ErrBits { ERR_CAP_PAIRS = 1, ERR_CAP_SLOTS = 2, ERR_CAP_CAND = 4, ERR_CAP_CONTACTS = 8, ERR_COLORS = 16,
-               ERR_SOLVER = 32 };
+               ERR_SOLVER = 32, ERR_CAP_UNITS = 64 };
 
 template <typename R> struct Dev {
   // shapes
@@ -81,13 +82,21 @@ template <typename R> struct Dev {
   V4<R>* c_rA; V4<R>* c_rB; V4<R>* c_b; V4<R>* c_r0; V4<R>* c_mu;
   V4<R>* c_imp;                        // solved impulses (canonical order)
   int* c_color;
-  // body -> contact lists
-  int* bc_cnt; int* bc_off; int* bc_fill; int* bc_list;
+  // islands and colouring units (contacts regrouped by island: "unit order")
+  const int* f_bs;                     // flat batch body -> batch scene
   int* uf;                             // union-find parent per flat body
-  int* isl_max;                        // per island root (flat body)
-  unsigned* pal;                       // PALW words per contact
-  int* c_deg; int* c_isl; unsigned char* c_done;
-  int* col_off;                        // (PALMAX + 1) per batch scene
+  int* isl_cnt; int* isl_off; int* isl_fill; int* isl_max;   // per island label [nb + ns (+1)]
+  int* c_lab;                          // island label per contact
+  int* uperm;                          // unit-order position -> contact
+  int* u_cnt; int* u_base; int* n_units; int cap_units; int unit_T;
+  int* u_start; int* u_end; int* u_ncol; int* u_bs;
+  int* body_claim; int2* body_loc;     // per flat body, LDS staging of body lists
+  int* bc_cnt; int* bc_off; int* bc_fill; int* bc_list;   // flat body -> unit-order positions
+  unsigned* pal;                       // PALW words per unit-order position
+  int* g_pc; unsigned char* g_done;    // colouring state of units too large for LDS
+  int* col_off;                        // (PALMAX + 1) per batch scene or per unit
+  double* merit;                       // per-sweep merit of a scene solved by several workgroups
+  unsigned* bar;                       // grid barrier counters
   // solver arrays (colour order)
   int* perm; int2* s_id; V4<R>* s_n; V4<R>* s_rA; V4<R>* s_rB; V4<R>* s_b; V4<R>* s_r0; V4<R>* x0; V4<R>* x1;
   int* err;                            // [0] flags, [1..] needed capacities
@@ -595,572 +604,7 @@ template <typename R> __global__ void k_assemble(Dev<R> d) {
   }
 }
 
-// ========================================================== body -> contacts
-template <typename R> __global__ void k_csr_count(Dev<R> d) {
-  if (aborted(d.err)) return;
-  const int nk = *d.n_contacts;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
-    int A = d.c_ia[k], B = d.c_ib[k];
-    if (!d.b_fixed[A]) atomicAdd(&d.bc_cnt[d.b2f[A]], 1);
-    if (!d.b_fixed[B]) atomicAdd(&d.bc_cnt[d.b2f[B]], 1);
-  }
-}
-
-template <typename R> __global__ void k_csr_fill(Dev<R> d) {
-  if (aborted(d.err)) return;
-  const int nk = *d.n_contacts;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
-    int A = d.c_ia[k], B = d.c_ib[k];
-    if (!d.b_fixed[A]) { int f = d.b2f[A]; d.bc_list[d.bc_off[f] + atomicAdd(&d.bc_fill[f], 1)] = k; }
-    if (!d.b_fixed[B]) { int f = d.b2f[B]; d.bc_list[d.bc_off[f] + atomicAdd(&d.bc_fill[f], 1)] = k; }
-  }
-}
-
-// ========================================================== colouring
-// Vivace random colouring per island (proximal_contact/blocking.py:62-111 in
-// the deterministic form described in oracle/coloring.py).  One workgroup per
-// scene; contact k of the scene has local index k - k_off.
-constexpr int COLOR_BLOCK = 256;
-
-__device__ __forceinline__ int uf_find(int* uf, int a) {
-  while (true) {
-    int p = __atomic_load_n(&uf[a], __ATOMIC_RELAXED);
-    if (p == a) return a;
-    int gp = __atomic_load_n(&uf[p], __ATOMIC_RELAXED);
-    if (gp != p) atomicCAS(&uf[a], p, gp);   // path halving; parents only decrease
-    a = p;
-  }
-}
-
-__device__ __forceinline__ void uf_union(int* uf, int a, int b) {
-  while (true) {
-    a = uf_find(uf, a);
-    b = uf_find(uf, b);
-    if (a == b) return;
-    if (a > b) { int t = a; a = b; b = t; }
-    if (atomicCAS(&uf[b], b, a) == b) return;
-  }
-}
-
-// Visit the conflict-graph neighbours of contact k (each once).
-template <typename R, typename F>
-__device__ __forceinline__ void for_neighbours(const Dev<R>& d, int k, F&& fn) {
-  int A = d.c_ia[k], B = d.c_ib[k];
-  bool fa = !d.b_fixed[A], fb = !d.b_fixed[B];
-  if (fa) {
-    int f = d.b2f[A];
-    int o = d.bc_off[f], n = d.bc_cnt[f];
-    for (int i = 0; i < n; ++i) { int j = d.bc_list[o + i]; if (j != k) fn(j); }
-  }
-  if (fb) {
-    int f = d.b2f[B];
-    int o = d.bc_off[f], n = d.bc_cnt[f];
-    for (int i = 0; i < n; ++i) {
-      int j = d.bc_list[o + i];
-      if (j == k) continue;
-      if (fa) {   // contacts on both bodies were visited through A
-        int ja = d.c_ia[j], jb = d.c_ib[j];
-        if (ja == A || jb == A) continue;
-      }
-      fn(j);
-    }
-  }
-}
-
-__device__ __forceinline__ int pal_count(const unsigned* p) {
-  int c = 0;
-  for (int w = 0; w < PALW; ++w) c += __popc(p[w]);
-  return c;
-}
-
-__device__ __forceinline__ int pal_select(const unsigned* p, int idx) {
-  for (int w = 0; w < PALW; ++w) {
-    int c = __popc(p[w]);
-    if (idx < c) {
-      unsigned m = p[w];
-      for (int i = 0; i < idx; ++i) m &= m - 1;   // drop the lowest set bits
-      return w * 32 + __ffs(m) - 1;
-    }
-    idx -= c;
-  }
-  return -1;
-}
-
-template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color(Dev<R> d) {
-  __shared__ int sh_scan[COLOR_BLOCK / 64];
-  __shared__ int s_flag;
-  __shared__ int s_nE;
-  const int bs = blockIdx.x;
-  if (aborted(d.err)) return;
-  const int slot = d.bs_slot[bs];
-  const SceneParams& sp = d.sp[slot];
-  SceneState& st = d.ss[slot];
-  const int K = st.k_cnt, k0 = st.k_off;
-  if (K == 0) { if (threadIdx.x == 0) st.ncolor = 0; return; }
-  if (sp.scheme == SCHEME_GS) {   // serial order: contact i is colour i
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) d.c_color[k0 + i] = i;
-    if (threadIdx.x == 0) st.ncolor = K;
-    return;
-  }
-  const int f0 = d.bs_foff[bs], f1 = d.bs_foff[bs + 1];
-  // islands
-  for (int f = f0 + threadIdx.x; f < f1; f += COLOR_BLOCK) d.uf[f] = f;
-  __syncthreads();
-  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-    int k = k0 + i;
-    int A = d.c_ia[k], B = d.c_ib[k];
-    if (!d.b_fixed[A] && !d.b_fixed[B]) uf_union(d.uf, d.b2f[A], d.b2f[B]);
-  }
-  __syncthreads();
-  // degrees, palettes, island labels
-  if (threadIdx.x == 0) s_flag = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-    int k = k0 + i;
-    int A = d.c_ia[k], B = d.c_ib[k];
-    int isl = -1;
-    if (!d.b_fixed[A]) isl = uf_find(d.uf, d.b2f[A]);
-    else if (!d.b_fixed[B]) isl = uf_find(d.uf, d.b2f[B]);
-    d.c_isl[k] = isl;
-    int deg = 0;
-    for_neighbours(d, k, [&](int) { ++deg; });
-    d.c_deg[k] = deg;
-    if (deg > PALMAX) atomicOr(&s_flag, 1);
-    unsigned* p = d.pal + (size_t)k * PALW;
-    for (int w = 0; w < PALW; ++w) {
-      int lo = w * 32;
-      unsigned m = 0;
-      if (deg >= lo + 32) m = 0xffffffffu;
-      else if (deg > lo) m = (1u << (deg - lo)) - 1u;
-      p[w] = m;
-    }
-    d.c_color[k] = 0;
-    d.c_done[k] = 0;
-  }
-  __syncthreads();
-  if (s_flag) { if (threadIdx.x == 0) { atomicOr(d.err, ERR_COLORS); } return; }
-  const unsigned long long seed = sp.seed;
-  const unsigned long long stepk = (unsigned long long)st.step_index;
-  int rnd = 0;
-  for (; rnd < MAX_ROUNDS; ++rnd) {
-    // tentative draws
-    int left = 0;
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-      int k = k0 + i;
-      if (d.c_done[k]) continue;
-      ++left;
-      const unsigned* p = d.pal + (size_t)k * PALW;
-      int n = pal_count(p);
-      if (n > 0) d.c_color[k] = pal_select(p, (int)color_draw(seed, stepk, (unsigned long long)i, (unsigned long long)rnd, (unsigned)n));
-    }
-    int tot;
-    block_exclusive_scan(left, sh_scan, tot);
-    if (tot == 0) break;
-    __syncthreads();
-    // acceptance: no uncoloured neighbour drew the same colour (c_done = 2 marks this round's winners)
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-      int k = k0 + i;
-      if (d.c_done[k]) continue;
-      int ck = d.c_color[k];
-      bool ok = true;
-      for_neighbours(d, k, [&](int j) { if (d.c_done[j] != 1 && d.c_color[j] == ck) ok = false; });
-      if (ok) d.c_done[k] = 2;
-    }
-    __syncthreads();
-    // winners remove their colour from the palettes of neighbours uncoloured at round start
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-      int k = k0 + i;
-      if (d.c_done[k] != 2) continue;
-      int ck = d.c_color[k];
-      unsigned bit = ~(1u << (ck & 31));
-      for_neighbours(d, k, [&](int j) { if (d.c_done[j] != 1) atomicAnd(&d.pal[(size_t)j * PALW + (ck >> 5)], bit); });
-    }
-    __syncthreads();
-    // commit winners; per-island maximum colour (kept or tentative)
-    for (int f = f0 + threadIdx.x; f < f1; f += COLOR_BLOCK) d.isl_max[f] = -1;
-    __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-      int k = k0 + i;
-      if (d.c_done[k] == 2) d.c_done[k] = 1;
-      if (d.c_isl[k] >= 0) atomicMax(&d.isl_max[d.c_isl[k]], d.c_color[k]);
-    }
-    __syncthreads();
-    // uncoloured nodes with an empty palette get max+1, max+2, ... in index order
-    if (threadIdx.x == 0) { s_nE = 0; s_flag = 0; }
-    __syncthreads();
-    for (int base = 0; base < K; base += COLOR_BLOCK) {
-      int i = base + threadIdx.x;
-      int k = k0 + i;
-      bool e = false;
-      if (i < K && !d.c_done[k]) e = pal_count(d.pal + (size_t)k * PALW) == 0;
-      if (e) atomicOr(&s_flag, 1);
-    }
-    __syncthreads();
-    if (s_flag) {
-      // rare path: rank by scanning earlier contacts of the same island
-      for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-        int k = k0 + i;
-        if (d.c_done[k] || pal_count(d.pal + (size_t)k * PALW) != 0) continue;
-        int isl = d.c_isl[k];
-        int rank = 0;
-        for (int j = 0; j < i; ++j) {
-          int kj = k0 + j;
-          if (!d.c_done[kj] && d.c_isl[kj] == isl && pal_count(d.pal + (size_t)kj * PALW) == 0) ++rank;
-        }
-        d.c_deg[k] = -1 - rank;   // stash the rank until every thread has ranked
-      }
-      __syncthreads();
-      for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-        int k = k0 + i;
-        if (d.c_done[k] || d.c_deg[k] >= 0) continue;
-        int rank = -1 - d.c_deg[k];
-        int c = d.isl_max[d.c_isl[k]] + 1 + rank;
-        d.c_deg[k] = 0;
-        if (c >= PALMAX) { atomicOr(d.err, ERR_COLORS); continue; }
-        d.pal[(size_t)k * PALW + (c >> 5)] |= 1u << (c & 31);
-      }
-      __syncthreads();
-    }
-  }
-  // after MAX_ROUNDS rounds: fresh colours in index order per island
-  if (rnd >= MAX_ROUNDS) {
-    for (int f = f0 + threadIdx.x; f < f1; f += COLOR_BLOCK) d.isl_max[f] = -1;
-    __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK)
-      if (d.c_isl[k0 + i] >= 0) atomicMax(&d.isl_max[d.c_isl[k0 + i]], d.c_color[k0 + i]);
-    __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-      int k = k0 + i;
-      if (d.c_done[k]) continue;
-      int isl = d.c_isl[k];
-      int rank = 0;
-      for (int j = 0; j < i; ++j)
-        if (!d.c_done[k0 + j] && d.c_isl[k0 + j] == isl) ++rank;
-      d.c_deg[k] = -1 - rank;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-      int k = k0 + i;
-      if (d.c_done[k]) continue;
-      int c = d.isl_max[d.c_isl[k]] + 1 + (-1 - d.c_deg[k]);
-      if (c >= PALMAX) atomicOr(d.err, ERR_COLORS);
-      d.c_color[k] = c;
-    }
-    __syncthreads();
-  }
-  // number of colours
-  __shared__ int s_max;
-  if (threadIdx.x == 0) s_max = -1;
-  __syncthreads();
-  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicMax(&s_max, d.c_color[k0 + i]);
-  __syncthreads();
-  if (threadIdx.x == 0) st.ncolor = s_max + 1;
-}
-
-// ========================================================== colour order
-// Counting sort of the scene's contacts by colour; gather the solver arrays.
-template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order(Dev<R> d) {
-  const int bs = blockIdx.x;
-  if (aborted(d.err)) return;
-  const int slot = d.bs_slot[bs];
-  SceneState& st = d.ss[slot];
-  const int K = st.k_cnt, k0 = st.k_off;
-  if (K == 0) return;
-  int* coff = d.col_off + (size_t)bs * (PALMAX + 1);
-  const bool serial = d.sp[slot].scheme == SCHEME_GS;
-  const int nc = serial ? 0 : st.ncolor;
-  if (!serial) {
-    for (int c = threadIdx.x; c <= nc; c += COLOR_BLOCK) coff[c] = 0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicAdd(&coff[d.c_color[k0 + i] + 1], 1);
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int c = 0; c < nc; ++c) coff[c + 1] += coff[c];
-    __syncthreads();
-    // stable placement: rank among same-coloured contacts with smaller index,
-    // one colour-chunk pass at a time (chunk = block size)
-    __shared__ int s_cnt[PALMAX];
-    for (int c = threadIdx.x; c < nc; c += COLOR_BLOCK) s_cnt[c] = 0;
-    __syncthreads();
-    __shared__ int s_col[COLOR_BLOCK];
-    for (int base = 0; base < K; base += COLOR_BLOCK) {
-      int i = base + threadIdx.x;
-      int c = i < K ? d.c_color[k0 + i] : -1;
-      s_col[threadIdx.x] = c;
-      __syncthreads();
-      if (i < K) {
-        int rank = 0;
-        for (int j = 0; j < (int)threadIdx.x; ++j) rank += (s_col[j] == c);
-        int pos = coff[c] + s_cnt[c] + rank;
-        d.perm[k0 + pos] = k0 + i;
-      }
-      __syncthreads();
-      if (i < K) {
-        // the last of each colour in this chunk advances the colour counter
-        bool last = true;
-        for (int j = threadIdx.x + 1; j < COLOR_BLOCK && base + j < K; ++j) if (s_col[j] == c) { last = false; break; }
-        if (last) {
-          int n = 0;
-          for (int j = 0; j <= (int)threadIdx.x; ++j) n += (s_col[j] == c);
-          s_cnt[c] += n;
-        }
-      }
-      __syncthreads();
-    }
-  } else {
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) d.perm[k0 + i] = k0 + i;
-  }
-  __syncthreads();
-  for (int pos = threadIdx.x; pos < K; pos += COLOR_BLOCK) {
-    int k = d.perm[k0 + pos];
-    V4<R> mu = d.c_mu[k];
-    V4<R> n = d.c_n[k], rA = d.c_rA[k], rB = d.c_rB[k];
-    d.s_id[k0 + pos] = make_int2(d.c_ia[k], d.c_ib[k]);
-    d.s_n[k0 + pos] = V4<R>{n.x, n.y, n.z, mu.x};
-    d.s_rA[k0 + pos] = V4<R>{rA.x, rA.y, rA.z, mu.y};
-    d.s_rB[k0 + pos] = V4<R>{rB.x, rB.y, rB.z, mu.z};
-    d.s_b[k0 + pos] = d.c_b[k];
-    d.s_r0[k0 + pos] = d.c_r0[k];
-    V4<R> z = {R(0), R(0), R(0), R(0)};
-    d.x0[k0 + pos] = z;
-    d.x1[k0 + pos] = z;
-  }
-}
-
-// ========================================================== solve
-// gauss_seidel_solver.py:14-101 with Contacts.sweep (contacts.py:66-107):
-// colours in order, every contact of a colour in parallel.  x is double
-// buffered: the sweep reads the accepted iterate and writes the other buffer,
-// so a rejected sweep is discarded by not swapping.
-constexpr int SOLVE_BLOCK = 256;
-
-template <typename R> struct ContactRow {
-  V3<R> lin[4], angA[4], angB[4];
-};
-
-template <typename R>
-__device__ __forceinline__ void contact_rows(V4<R> n4, V4<R> rA4, V4<R> rB4, ContactRow<R>& J) {
-  V3<R> s, t, n;
-  orthonormal_frame(xyz(n4), s, t, n);
-  V3<R> rA = xyz(rA4), rB = xyz(rB4);
-  J.lin[0] = n; J.lin[1] = s; J.lin[2] = t; J.lin[3] = V3<R>{R(0), R(0), R(0)};
-  J.angA[0] = cross(rA, n); J.angA[1] = cross(rA, s); J.angA[2] = cross(rA, t); J.angA[3] = n;
-  J.angB[0] = cross(rB, n); J.angB[1] = cross(rB, s); J.angB[2] = cross(rB, t); J.angB[3] = n;
-}
-
-// z = x - r (J w + b); x_n = max(0, z_n); friction by the chosen prox
-template <typename R>
-__device__ __forceinline__ V4<R> contact_prox(const Dev<R>& d, const SceneParams& sp, const ContactRow<R>& J, V4<R> xo,
-                                              V4<R> bb, V4<R> rr, V4<R> mu, V3<R> wlA, V3<R> waA, V3<R> wlB, V3<R> waB,
-                                              bool use_old_xn) {
-  R Jw[4];
-  for (int i = 0; i < 4; ++i)
-    Jw[i] = -dot(J.lin[i], wlA) - dot(J.angA[i], waA) + dot(J.lin[i], wlB) + dot(J.angB[i], waB);
-  R z0 = xo.x - rr.x * (Jw[0] + bb.x);
-  R z1 = xo.y - rr.y * (Jw[1] + bb.y);
-  R z2 = xo.z - rr.z * (Jw[2] + bb.z);
-  R z3 = xo.w - rr.w * (Jw[3] + bb.w);
-  R xn = fmax(R(0), z0);
-  R xr = use_old_xn ? xo.x : xn;
-  R os, ot, orr;
-  if (sp.friction == 1) {
-    EllipsoidParams ep{sp.ell_tol, sp.ell_expansion, sp.ell_max_iter};
-    prox_ellipsoid<R>(z1, z2, z3, mu.x, mu.y, mu.z, xr, ep, os, ot, orr);
-  } else {
-    prox_sphere<R>(z1, z2, z3, mu.x, xr, os, ot, orr);
-  }
-  return V4<R>{xn, os, ot, orr};
-}
-
-template <typename R>
-__device__ __forceinline__ void apply_dw(const Dev<R>& d, int body, R sign, const ContactRow<R>& J, const V3<R>* ang,
-                                         V4<R> dx) {
-  R im, W[6];
-  load_W(d, body, im, W);
-  R dl[3] = {R(0), R(0), R(0)};
-  V3<R> a = {R(0), R(0), R(0)};
-  R dd[4] = {dx.x, dx.y, dx.z, dx.w};
-  for (int i = 0; i < 4; ++i) {
-    R s = sign * dd[i];
-    dl[0] += J.lin[i].x * s; dl[1] += J.lin[i].y * s; dl[2] += J.lin[i].z * s;
-    a = a + s * ang[i];
-  }
-  V3<R> da = sym_mul(W, a);
-  V4<R> l = d.ww0[body], g = d.ww1[body];
-  d.ww0[body] = V4<R>{l.x + im * dl[0], l.y + im * dl[1], l.z + im * dl[2], R(0)};
-  d.ww1[body] = V4<R>{g.x + da.x, g.y + da.y, g.z + da.z, R(0)};
-}
-
-__device__ __forceinline__ double block_max(double v, double* sh) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  double m = 0.0;
-  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sh[i]);
-  __syncthreads();
-  return m;
-}
-
-template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve(Dev<R> d) {
-  __shared__ double sh_red[SOLVE_BLOCK / 64];
-  __shared__ int s_dec;   // bit0 accept, bit1 exit
-  const int bs = blockIdx.x;
-  if (aborted(d.err)) return;
-  const int slot = d.bs_slot[bs];
-  const SceneParams& sp = d.sp[slot];
-  SceneState& st = d.ss[slot];
-  const int K = st.k_cnt, k0 = st.k_off;
-  double* res = d.residuals + sp.res_off;
-  for (int i = threadIdx.x; i < sp.max_iter; i += SOLVE_BLOCK) res[i] = INFINITY;
-  if (K == 0) {
-    if (threadIdx.x == 0) { st.solved = 0; st.iterations = 0; st.exitcode = 0; st.sweeps = 0; st.rejects = 0;
-                            st.natural_residual = 0.0; st.final_merit = 0.0; }
-    return;
-  }
-  const bool serial = sp.scheme == SCHEME_GS;
-  const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
-  const int nc = st.ncolor;
-  const int* coff = d.col_off + (size_t)bs * (PALMAX + 1);
-  const int f0 = d.bs_foff[bs], f1 = d.bs_foff[bs + 1];
-  double last_merit = INFINITY, glast = INFINITY, rscale = 1.0;
-  int acc = 0, it = 0, rejects = 0, exitcode = 0, iters = sp.max_iter, sweeps = 0;
-  double gm = 0.0;
-  for (it = 0; it < sp.max_iter; ++it) {
-    V4<R>* xa = acc ? d.x1 : d.x0;
-    V4<R>* xn = acc ? d.x0 : d.x1;
-    const R rs = R(rscale);
-    double mloc = 0.0;
-    if (!jacobi) {
-      for (int c = 0; c < nc; ++c) {
-        const int c_lo = serial ? c : coff[c], c_hi = serial ? c + 1 : coff[c + 1];
-        for (int pos = c_lo + (int)threadIdx.x; pos < c_hi; pos += SOLVE_BLOCK) {
-          int kk = k0 + pos;
-          int2 id = d.s_id[kk];
-          V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
-          ContactRow<R> J;
-          contact_rows(n4, rA4, rB4, J);
-          V4<R> r0 = d.s_r0[kk];
-          V4<R> rr = {r0.x * rs, r0.y * rs, r0.z * rs, r0.w * rs};
-          V4<R> xo = xa[kk];
-          V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
-          V3<R> wlA = xyz(d.ww0[id.x]), waA = xyz(d.ww1[id.x]), wlB = xyz(d.ww0[id.y]), waB = xyz(d.ww1[id.y]);
-          V4<R> x = contact_prox(d, sp, J, xo, d.s_b[kk], rr, mu, wlA, waA, wlB, waB, false);
-          xn[kk] = x;
-          V4<R> dx = {x.x - xo.x, x.y - xo.y, x.z - xo.z, x.w - xo.w};
-          mloc = fmax(mloc, fmax(fmax(fabs((double)x.x - (double)xo.x), fabs((double)x.y - (double)xo.y)),
-                                 fmax(fabs((double)x.z - (double)xo.z), fabs((double)x.w - (double)xo.w))));
-          if (!d.b_fixed[id.x]) apply_dw(d, id.x, R(-1), J, J.angA, dx);
-          if (!d.b_fixed[id.y]) apply_dw(d, id.y, R(1), J, J.angB, dx);
-        }
-        __syncthreads();
-      }
-    } else {
-      const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
-      for (int pos = threadIdx.x; pos < K; pos += SOLVE_BLOCK) {
-        int kk = k0 + pos;
-        int2 id = d.s_id[kk];
-        V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
-        ContactRow<R> J;
-        contact_rows(n4, rA4, rB4, J);
-        V4<R> r0 = d.s_r0[kk];
-        V4<R> rr = {r0.x * rs, r0.y * rs, r0.z * rs, r0.w * rs};
-        V4<R> xo = xa[kk];
-        V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
-        V3<R> wlA = xyz(d.ws0[id.x]), waA = xyz(d.ws1[id.x]), wlB = xyz(d.ws0[id.y]), waB = xyz(d.ws1[id.y]);
-        V4<R> x = contact_prox(d, sp, J, xo, d.s_b[kk], rr, mu, wlA, waA, wlB, waB, !hybrid);
-        xn[kk] = x;
-        mloc = fmax(mloc, fmax(fmax(fabs((double)x.x - (double)xo.x), fabs((double)x.y - (double)xo.y)),
-                               fmax(fabs((double)x.z - (double)xo.z), fabs((double)x.w - (double)xo.w))));
-      }
-      __syncthreads();
-      for (int c = 0; c < nc; ++c) {
-        for (int pos = coff[c] + (int)threadIdx.x; pos < coff[c + 1]; pos += SOLVE_BLOCK) {
-          int kk = k0 + pos;
-          int2 id = d.s_id[kk];
-          ContactRow<R> J;
-          contact_rows(d.s_n[kk], d.s_rA[kk], d.s_rB[kk], J);
-          V4<R> x = xn[kk], xo = xa[kk];
-          V4<R> dx = {x.x - xo.x, x.y - xo.y, x.z - xo.z, x.w - xo.w};
-          if (!d.b_fixed[id.x]) apply_dw(d, id.x, R(-1), J, J.angA, dx);
-          if (!d.b_fixed[id.y]) apply_dw(d, id.y, R(1), J, J.angB, dx);
-        }
-        __syncthreads();
-      }
-    }
-    ++sweeps;
-    double merit = block_max(mloc, sh_red);
-    if (threadIdx.x == 0) {
-      int dec = 0;
-      if (merit > last_merit) {
-        rscale *= sp.nu_reduce;
-        gm = fmax(0.0, last_merit);
-        ++rejects;
-      } else {
-        dec |= 1;
-        if (last_merit - merit < sp.too_small) rscale *= sp.nu_increase;
-        last_merit = merit;
-        gm = fmax(0.0, merit);
-      }
-      res[it] = gm;
-      if (gm < sp.abs_tol) { dec |= 2; exitcode = 1; }
-      else if (fabs(gm - glast) < sp.rel_tol * glast) { dec |= 2; exitcode = 2; }
-      glast = gm;
-      s_dec = dec;
-    }
-    __syncthreads();
-    int dec = s_dec;
-    // every thread tracks the same scalars
-    if (threadIdx.x != 0) {
-      if (dec & 1) { if (last_merit - merit < sp.too_small) rscale *= sp.nu_increase; last_merit = merit; }
-      else { rscale *= sp.nu_reduce; }
-    }
-    if (dec & 1) {
-      acc ^= 1;
-      for (int f = f0 + threadIdx.x; f < f1; f += SOLVE_BLOCK) {
-        int b = d.bb_body[f];
-        d.ws0[b] = d.ww0[b]; d.ws1[b] = d.ww1[b];
-      }
-    } else {
-      for (int f = f0 + threadIdx.x; f < f1; f += SOLVE_BLOCK) {
-        int b = d.bb_body[f];
-        d.ww0[b] = d.ws0[b]; d.ww1[b] = d.ws1[b];
-      }
-    }
-    __syncthreads();
-    if (dec & 2) { iters = it; break; }
-  }
-  // accepted impulses back to canonical order; natural-map residual in FP64
-  V4<R>* xs = acc ? d.x1 : d.x0;
-  double rloc = 0.0, xmax = 0.0;
-  for (int pos = threadIdx.x; pos < K; pos += SOLVE_BLOCK) {
-    int kk = k0 + pos;
-    V4<R> x = xs[kk];
-    d.c_imp[d.perm[kk]] = x;
-    int2 id = d.s_id[kk];
-    V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
-    ContactRow<R> J;
-    contact_rows(n4, rA4, rB4, J);
-    V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
-    V3<R> wlA = xyz(d.ws0[id.x]), waA = xyz(d.ws1[id.x]), wlB = xyz(d.ws0[id.y]), waB = xyz(d.ws1[id.y]);
-    V4<R> y = contact_prox(d, sp, J, x, d.s_b[kk], d.s_r0[kk], mu, wlA, waA, wlB, waB, false);
-    rloc = fmax(rloc, fmax(fmax(fabs((double)x.x - (double)y.x), fabs((double)x.y - (double)y.y)),
-                           fmax(fabs((double)x.z - (double)y.z), fabs((double)x.w - (double)y.w))));
-    xmax = fmax(xmax, fmax(fmax(fabs((double)x.x), fabs((double)x.y)), fmax(fabs((double)x.z), fabs((double)x.w))));
-  }
-  double rmax = block_max(rloc, sh_red);
-  double xm = block_max(xmax, sh_red);
-  if (threadIdx.x == 0) {
-    st.solved = 1;
-    st.iterations = iters;
-    st.exitcode = exitcode;
-    st.rejects = rejects;
-    st.sweeps = sweeps;
-    st.final_merit = last_merit;
-    st.natural_residual = rmax / fmax(1.0, xm);
-  }
-}
+#include "rbh_solve.h"
 
 // ========================================================== integration
 // state_storage.py:239-302: u += dt W f_ext + W J^T sol; r += v dt;
@@ -1190,7 +634,7 @@ template <typename R> __global__ void k_scene_reset(Dev<R> d) {
   if (s >= d.ns) return;
   SceneState& st = d.ss[d.bs_slot[s]];
   st.k_off = 0x7fffffff; st.k_cnt = 0; st.ncolor = 0; st.n_overlaps = 0; st.fw_evals = 0; st.max_pen = 0.0;
-  st.solved = 0;
+  st.solved = 0; st.nat_r = 0.0; st.nat_x = 0.0;
 }
 
 template <typename R> __global__ void k_scene_finalize(Dev<R> d) {

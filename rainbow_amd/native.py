@@ -16,7 +16,7 @@ RBH_E_ARG, RBH_E_HIP, RBH_E_MEMORY, RBH_E_CAPACITY, RBH_E_COLORS, RBH_E_STATE = 
 SCHEMES = {"gauss_seidel": 0, "parallel_gauss_seidel": 1, "parallel_jacobi": 2,
            "parallel_jacobi_hybrid": 3, "parallel_jacboi_hybrid": 3}
 FRICTION = {"isotropic": 0, "anisotropic": 1}
-STEP_COUNT_OVERLAPS, STEP_TIMING, STEP_GRAPH = 1, 2, 4
+STEP_COUNT_OVERLAPS, STEP_TIMING, STEP_GRAPH, STEP_FORCE_SINGLE, STEP_FORCE_MULTI = 1, 2, 4, 8, 16
 TIMING_KEYS = ["update_bvh", "narrow_phase", "contact_determination", "contact_assembly",
                "coloring", "solver_time", "integration", "stepper_time"]
 

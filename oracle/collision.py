@@ -48,12 +48,26 @@ def _overlap_matrix(loA, hiA, loB, hiB):
     return ~np.any(sep, axis=-1)
 
 
-def candidates(aabbA, aabbB, chunk=4096):
-    """Candidate triangles of A and of B and the number of overlapping pairs."""
+def _slack(loA, hiA, loB, hiB):
+    """Overlap slack of boxes: min over axes of min(hiA - loB, hiB - loA);
+    >= 0 exactly when kdop.overlap reports an overlap."""
+    return np.min(np.minimum(hiA[:, None, :] - loB[None, :, :], hiB[None, :, :] - loA[:, None, :]), axis=-1)
+
+
+def candidates(aabbA, aabbB, chunk=4096, slack=None):
+    """Candidate triangles of A and of B and the number of overlapping pairs.
+
+    ``slack`` (optional dict) receives, per triangle of A ("A") and of B
+    ("B"), the largest overlap slack against the other body's triangles (the
+    decision margin of the candidate test; root-volume slack for triangles
+    outside the other root)."""
     loA, hiA = aabbA
     loB, hiB = aabbB
     rootA = (loA.min(axis=0), hiA.max(axis=0))
     rootB = (loB.min(axis=0), hiB.max(axis=0))
+    if slack is not None:
+        slack["A"] = _slack(loA, hiA, rootB[0][None], rootB[1][None])[:, 0]
+        slack["B"] = _slack(loB, hiB, rootA[0][None], rootA[1][None])[:, 0]
     if np.any(rootA[1] < rootB[0]) or np.any(rootB[1] < rootA[0]):
         return np.zeros(0, np.int64), np.zeros(0, np.int64), 0
     # Only triangles overlapping the other body's root volume can overlap a leaf.
@@ -62,11 +76,18 @@ def candidates(aabbA, aabbB, chunk=4096):
     hitA = np.zeros(len(ia), dtype=bool)
     hitB = np.zeros(len(ib), dtype=bool)
     count = 0
+    bestB = np.full(len(ib), -np.inf)
     for s in range(0, len(ia), chunk):
         m = _overlap_matrix(loA[ia[s:s + chunk]], hiA[ia[s:s + chunk]], loB[ib], hiB[ib])
         count += int(m.sum())
         hitA[s:s + chunk] |= m.any(axis=1)
         hitB |= m.any(axis=0)
+        if slack is not None and len(ib):
+            sl = _slack(loA[ia[s:s + chunk]], hiA[ia[s:s + chunk]], loB[ib], hiB[ib])
+            slack["A"][ia[s:s + chunk]] = sl.max(axis=1)
+            bestB = np.maximum(bestB, sl.max(axis=0))
+    if slack is not None and len(ib):
+        slack["B"][ib] = bestB
     return ia[hitA], ib[hitB], count
 
 
@@ -75,7 +96,28 @@ def _argmin_first(vals):
     return np.argmin(vals, axis=-1)
 
 
-def frank_wolfe_contacts(scene, a, b, tris):
+def _gap2(v, scale):
+    """Gap between the smallest and second smallest of 3 values per row, in
+    units of ``scale`` (a fixed magnitude, not the values themselves: where the
+    SDF is flat the values are rounding noise and any gap between them is
+    meaningless)."""
+    srt = np.sort(v, axis=1)
+    return (srt[:, 1] - srt[:, 0]) / np.maximum(scale, 1e-300)
+
+
+def _cell_margin(grid, P):
+    """Distance (in cells) of the lookup points P to the nearest cell face that
+    changes grid3.get_enclosing_cell_idx (the floor decision); the
+    min_coord-scaled gradient jumps across those faces."""
+    f = (P - grid.min_coord) / grid.spacing
+    hi = np.array([grid.I - 2, grid.J - 2, grid.K - 2], dtype=float)
+    d = np.abs(f - np.round(f))
+    # faces outside the clipped index range do not change the cell
+    d = np.where((f < 0.5) | (f > hi + 0.5), np.inf, d)
+    return d.min(axis=1)
+
+
+def frank_wolfe_contacts(scene, a, b, tris, margins=None):
     """Contacts of triangles ``tris`` of body ``a`` against the SDF of body ``b``.
 
     Returns (p_world, n_world, gap, tri_index) for the triangles that produce a
@@ -95,6 +137,12 @@ def frank_wolfe_contacts(scene, a, b, tris):
     C = len(tris)
     gn = np.stack([np.linalg.norm(G.get_gradient_v(grid, P[:, v, :]), axis=1) for v in range(3)], axis=1)
     x = P[np.arange(C), _argmin_first(gn)]
+    margin = None
+    if margins is not None:
+        # gradient norms of an SDF are ~1 where they carry information
+        margin = _gap2(gn, np.maximum(gn.max(axis=1), 1.0))
+        for v in range(3):
+            margin = np.minimum(margin, _cell_margin(grid, P[:, v, :]))
     active = np.ones(C, dtype=bool)
     tol = prm["contact_optimization_tolerance"]
     for i in range(int(prm["contact_optimization_max_iterations"])):
@@ -102,6 +150,8 @@ def frank_wolfe_contacts(scene, a, b, tris):
         if len(idx) == 0:
             break
         g = G.get_gradient_v(grid, x[idx])
+        if margin is not None:
+            margin[idx] = np.minimum(margin[idx], _cell_margin(grid, x[idx]))
         Pi = P[idx]
         obj = np.stack([Pi[:, v, 0] * g[:, 0] + Pi[:, v, 1] * g[:, 1] + Pi[:, v, 2] * g[:, 2]
                         for v in range(3)], axis=1)
@@ -110,11 +160,25 @@ def frank_wolfe_contacts(scene, a, b, tris):
         alpha = 2 / (i + 2)
         x[idx] = x[idx] + alpha * (s_i - x[idx])
         best = obj[np.arange(len(idx)), vtx]
+        if margin is not None:
+            # objectives s . grad: scale |s| (|grad| ~ 1 for an SDF)
+            scale = np.maximum(np.abs(Pi).max(axis=(1, 2)), 1e-6)
+            margin[idx] = np.minimum(margin[idx], np.minimum(_gap2(obj, scale), np.abs(best - tol) / scale))
         active[idx[best > tol]] = False
     inside = G.is_inside_v(grid, x, 0.5)
     phi = np.full(C, np.inf)
     phi[inside] = G.get_value_v(grid, x[inside])
     keep = inside & (phi <= prm["envelope"])
+    if margin is not None:
+        h = float(np.min(grid.spacing))
+        ins = np.minimum(np.min(np.abs(x - (grid.max_coord - 0.5)), axis=1),
+                         np.min(np.abs(x - (grid.min_coord + 0.5)), axis=1)) / h
+        margin = np.minimum(margin, ins)
+        phv = np.abs(G.get_value_v(grid, x) - prm["envelope"]) / h
+        margin = np.minimum(margin, phv)
+        margin = np.minimum(margin, _cell_margin(grid, x))
+        for t, m in zip(tris, margin):
+            margins[(int(b), int(a), int(t))] = min(margins.get((int(b), int(a), int(t)), np.inf), float(m))
     idx = np.nonzero(keep)[0]
     n_b = G.get_gradient_v(grid, x[idx]) if len(idx) else np.zeros((0, 3))
     qb = scene.q[b]
@@ -194,7 +258,7 @@ def reduce_contacts(c):
     return c.take(np.array(keep, dtype=np.int64))
 
 
-def detect(scene, stats=None):
+def detect(scene, stats=None, margins=None):
     """run_collision_detection: returns the reduced Contacts.
 
     ``stats`` (dict) receives ``number_of_overlaps`` (overlapping triangle pairs)
@@ -209,15 +273,35 @@ def detect(scene, stats=None):
     ia, ib, p, n, gap, tri = [], [], [], [], [], []
     n_overlaps = 0
     n_cand = 0
+    # body pairs in combinations order; pairs whose root volumes are disjoint
+    # have no overlapping triangle pair and are skipped (found vectorised)
+    rlo = np.array([lo.min(axis=0) for lo, _ in aabbs]).reshape(-1, 3)
+    rhi = np.array([hi.max(axis=0) for _, hi in aabbs]).reshape(-1, 3)
+    pairs = []
     for a in range(N):
-        for b in range(a + 1, N):
-            ca, cb, cnt = candidates(aabbs[a], aabbs[b])
+        ok = ~np.any((rhi[a] < rlo[a + 1:]) | (rhi[a + 1:] < rlo[a]), axis=1)
+        if margins is not None:
+            # keep near-touching roots too, so candidate margins are recorded
+            sl = np.min(np.minimum(rhi[a] - rlo[a + 1:], rhi[a + 1:] - rlo[a]), axis=1)
+            ok |= np.abs(sl) < 1e-4
+        pairs.extend((a, a + 1 + j) for j in np.nonzero(ok)[0])
+    for a, b in pairs:
+        if True:
+            slack = {} if margins is not None else None
+            ca, cb, cnt = candidates(aabbs[a], aabbs[b], slack=slack)
+            if margins is not None:
+                for (owner, sdf, key) in ((a, b, "A"), (b, a, "B")):
+                    sl = slack[key]
+                    near = np.nonzero(np.abs(sl) < 1e-4)[0]
+                    for t in near:
+                        k = (int(sdf), int(owner), int(t))
+                        margins[k] = min(margins.get(k, np.inf), float(abs(sl[t])))
             if cnt == 0:
                 continue
             n_overlaps += cnt
             n_cand += len(ca) + len(cb)
             for (owner, sdf, tr) in ((a, b, ca), (b, a, cb)):
-                pw, nw, g, t = frank_wolfe_contacts(scene, owner, sdf, tr)
+                pw, nw, g, t = frank_wolfe_contacts(scene, owner, sdf, tr, margins)
                 ia.append(np.full(len(t), sdf))
                 ib.append(np.full(len(t), owner))
                 p.append(pw)
@@ -236,4 +320,16 @@ def detect(scene, stats=None):
         stats["number_of_overlaps"] = n_overlaps
         stats["candidates"] = n_cand
         stats["raw_contacts"] = c.K
+    if margins is not None and c.K:
+        # contact reduction compares positions exactly: a raw contact close
+        # to another contact of the same body pair sits near that decision
+        # boundary (recorded up to a relative 1e-5, the reach of FP32 rounding)
+        for k in range(c.K):
+            same = np.nonzero((np.minimum(c.ia, c.ib) == min(c.ia[k], c.ib[k]))
+                              & (np.maximum(c.ia, c.ib) == max(c.ia[k], c.ib[k])))[0]
+            d = np.abs(c.p[same] - c.p[k]).max(axis=1) / max(1.0, np.abs(c.p[k]).max())
+            d[same == k] = np.inf
+            if len(d) and d.min() < 1e-5:
+                key = (int(c.ia[k]), int(c.ib[k]), int(c.tri[k]))
+                margins[key] = min(margins.get(key, np.inf), float(d.min()))
     return reduce_contacts(c)

@@ -22,10 +22,15 @@ def conjugate(q):
 
 
 def prod(a, b):
-    """quaternion.py:186-193: [a0 b0 - A.B, a0 B + b0 A + A x B]."""
-    s = a[0] * b[0] - np.dot(a[1:], b[1:])
-    v = a[0] * b[1:] + a[1:] * b[0] + np.cross(a[1:], b[1:])
-    return np.array([s, v[0], v[1], v[2]], dtype=np.float64)
+    """quaternion.py:186-193: [a0 b0 - A.B, a0 B + b0 A + A x B].
+
+    The dot product is summed left to right and the cross product written out
+    (np.dot on 3-vectors goes through BLAS, whose rounding order is not fixed)."""
+    s = a[0] * b[0] - (a[1] * b[1] + a[2] * b[2] + a[3] * b[3])
+    x = a[0] * b[1] + a[1] * b[0] + (a[2] * b[3] - a[3] * b[2])
+    y = a[0] * b[2] + a[2] * b[0] + (a[3] * b[1] - a[1] * b[3])
+    z = a[0] * b[3] + a[3] * b[0] + (a[1] * b[2] - a[2] * b[1])
+    return np.array([s, x, y, z], dtype=np.float64)
 
 
 def rotate(q, r):

@@ -57,16 +57,18 @@ def integrate(scene, delta_u, dt):
         scene.q[k] = Q.unit(q)
 
 
-def step(scene, dt):
+def step(scene, dt, margins=None):
     """Advance ``scene`` in place by dt.  Returns a dict with the contacts, the
-    solved problem, the colouring and the per-step log."""
+    solved problem, the colouring and the per-step log.  ``margins`` (dict)
+    receives the decision margins of the candidate triangles, keyed by
+    (SDF owner, triangle owner, triangle)."""
     prm = scene.params
     if prm.get("use_post_stabilization"):
         raise NotImplementedError("post-stabilisation is outside this build's scope")
     Winv, f_ext, du_ext = problem.body_mass_blocks(scene)
     delta_u = dt * du_ext
     stats = {}
-    contacts = collision.detect(scene, stats)
+    contacts = collision.detect(scene, stats, margins)
     out = dict(contacts=contacts, problem=None, colors=None, log={})
     log = out["log"]
     log["number_of_overlaps"] = stats["number_of_overlaps"]

@@ -9,6 +9,9 @@ SRC = os.path.join(HERE, "csrc", "rbh_api.hip")
 OUT = os.path.join(HERE, "librbh.so")
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-shared", "-fPIC",
+               # no fused multiply-add: every product and sum is rounded on its own,
+               # as in the FP64 oracle, so the FP64 build repeats its arithmetic
+               "-ffp-contract=off",
                # keep the compiler from emitting packed FP32 instructions
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

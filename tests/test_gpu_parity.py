@@ -1,0 +1,283 @@
+"""GPU parity: librbh (through its C ABI) against the FP64 oracle.
+
+FP64 build of the kernels: the same contact sets, colourings and accept/reject
+sequences as the oracle; impulses, states and residual histories within 1e-9.
+FP32 product build: contact sets equal except candidates whose FP64 decision
+margin (AABB slack, Frank-Wolfe argmin ties, break test, is_inside inset,
+phi vs envelope) is below EPS_MARGIN; matched contacts within FP32 tolerance;
+final natural-map residual no worse than the oracle's + 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle.scene as OS
+import oracle.stepper as OST
+from rainbow_amd import native, scenes
+import rainbow_amd.api as API
+from rainbow_amd import mesh as MESH
+
+pytestmark = pytest.mark.gpu
+
+EPS_MARGIN = 1e-4
+
+
+@pytest.fixture(scope="module")
+def bank():
+    return scenes.ShapeBank(resolution=32)
+
+
+def mixed_scene(bank, n=8, seed=0, sink=0.25, params=None, spread=1.0):
+    eng = scenes.new_engine(bank, params)
+    scenes.set_friction(eng, 0.4)
+    API.create_gravity_force(eng, "gravity", 9.81, np.array([0.0, 1.0, 0.0]))
+    scenes.add_ground(eng, bank)
+    box = bank.get("box_shape", lambda: MESH.create_box(1.0, 1.0, 1.0))
+    sph = bank.get("sphere_shape", lambda: MESH.create_sphere(0.5, 16, 16))
+    rng = np.random.default_rng(seed)
+    for i in range(n):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        r = np.array([rng.uniform(-spread, spread), rng.uniform(0.6, 1.1) - sink, rng.uniform(-spread, spread)])
+        scenes.add_body(eng, "b%d" % i, box if i % 2 else sph, r, q=q, v=rng.normal(scale=0.3, size=3))
+    return eng
+
+
+def gpu_step(eng, dt, flags=0):
+    st = eng.stepper
+    st._prepare(eng)
+    st.ctx.step([st.scene], 1, dt, flags)
+    c = st.ctx.get_contacts(st.scene)
+    r, q, v, w = st.ctx.get_state(st.scene, len(eng.bodies))
+    lg = st.ctx.get_log(st.scene, eng.params.max_iterations)
+    # keep the engine in sync with the device for the next step
+    for k, b in enumerate(eng.bodies.values()):
+        b.r, b.q, b.v, b.w = r[k].copy(), q[k].copy(), v[k].copy(), w[k].copy()
+    eng.step_index += 1
+    return c, (r, q, v, w), lg
+
+
+EPS_FP64 = 1e-9
+
+
+def assert_fp64_parity(c, state, lg, out, sc, margins=None):
+    """Returns True when the contact sets were identical (and everything was compared)."""
+    oc = out["contacts"]
+    if margins is not None:
+        kg = {(int(a), int(b), int(t)) for a, b, t in zip(c["ia"], c["ib"], c["tri"])}
+        ko = {(int(a), int(b), int(t)) for a, b, t in zip(oc.ia, oc.ib, oc.tri)}
+        diff = kg ^ ko
+        assert all(margins.get(k, np.inf) < EPS_FP64 for k in diff), \
+            sorted((k, margins.get(k)) for k in diff if margins.get(k, np.inf) >= EPS_FP64)[:5]
+        if diff:
+            return False
+    assert len(c["ia"]) == oc.K
+    assert np.array_equal(c["ia"], oc.ia) and np.array_equal(c["ib"], oc.ib) and np.array_equal(c["tri"], oc.tri)
+    if oc.K == 0:
+        return True
+    assert np.allclose(c["p"], oc.p, atol=1e-12) and np.allclose(c["n"], oc.n, atol=1e-12)
+    assert np.allclose(c["gap"], oc.gap, atol=1e-12)
+    if out["colors"] is not None:
+        assert np.array_equal(c["color"], out["colors"])
+    P = out["problem"]
+    scale = max(1.0, np.abs(P.sol).max())
+    assert np.abs(c["impulse"] - P.sol).max() <= 1e-9 * scale
+    r, q, v, w = state
+    assert np.abs(r - sc.r).max() <= 1e-10 and np.abs(v - sc.v).max() <= 1e-9
+    assert np.abs(q - sc.q).max() <= 1e-10 and np.abs(w - sc.w).max() <= 1e-8
+    assert lg["rejects"] == out["log"]["rejects"]
+    assert lg["iterations"] == out["log"]["iterations"] and lg["exitcode"] == out["log"]["exitcode"]
+    ro, rg = out["log"]["residuals"], lg["residuals"]
+    fin = np.isfinite(ro)
+    assert np.array_equal(fin, np.isfinite(rg))
+    assert np.allclose(rg[fin], ro[fin], rtol=1e-8, atol=1e-14)
+    return True
+
+
+def resync(eng, sc):
+    for k, b in enumerate(eng.bodies.values()):
+        b.r, b.q, b.v, b.w = sc.r[k].copy(), sc.q[k].copy(), sc.v[k].copy(), sc.w[k].copy()
+
+
+def test_sdf_kernel_matches_oracle(bank):
+    from oracle.grid import Grid, nodes_array
+    from oracle.sdf import signed_distance
+    from rainbow_amd.stepper import shared_context
+    ctx = shared_context(0, "fp64")
+    for V, T in (MESH.create_sphere(0.5, 16, 16), MESH.create_box(1.0, 2.0, 0.5)):
+        mn, mx = V.min(0) - 0.2, V.max(0) + 0.2
+        got = ctx.sdf_build(V, T, mn, mx, 12, 13, 14)
+        ref = signed_distance(nodes_array(Grid(mn, mx, 12, 13, 14)), V, T)
+        assert np.abs(got - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("flags", [native.STEP_FORCE_SINGLE, native.STEP_FORCE_MULTI])
+def test_box_stack_fp64(bank, flags):
+    eng = scenes.box_stack(bank, params=dict(precision="fp64"))
+    sc = OS.from_engine(eng)
+    for _ in range(3):
+        out = OST.step(sc, 0.01)
+        c, state, lg = gpu_step(eng, 0.01, flags | native.STEP_COUNT_OVERLAPS)
+        assert lg["number_of_overlaps"] == out["log"]["number_of_overlaps"]
+        assert_fp64_parity(c, state, lg, out, sc)
+        assert len(c["ia"]) > 0
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_mixed_scene_fp64(bank, seed):
+    eng = mixed_scene(bank, 8, seed, params=dict(precision="fp64", max_iterations=40))
+    sc = OS.from_engine(eng)
+    total = exact = 0
+    for _ in range(4):
+        m = {}
+        out = OST.step(sc, 0.01, m)
+        c, state, lg = gpu_step(eng, 0.01)
+        exact += assert_fp64_parity(c, state, lg, out, sc, m)
+        total += len(c["ia"])
+        resync(eng, sc)
+    assert total > 0 and exact >= 3
+
+
+@pytest.mark.parametrize("scheme,friction", [("gauss_seidel", "isotropic"), ("parallel_jacobi", "isotropic"),
+                                             ("parallel_jacobi_hybrid", "isotropic"),
+                                             ("parallel_gauss_seidel", "anisotropic")])
+def test_schemes_fp64(bank, scheme, friction):
+    eng = mixed_scene(bank, 6, 3, params=dict(precision="fp64", max_iterations=30, proximal_solver=scheme,
+                                               friction_model=friction))
+    if friction == "anisotropic":
+        beh = eng.surfaces_interactions.storage[("default", "default")]
+        beh.mu = np.array([0.6, 0.3, 0.1])
+    sc = OS.from_engine(eng)
+    exact = 0
+    for _ in range(3):
+        m = {}
+        out = OST.step(sc, 0.01, m)
+        c, state, lg = gpu_step(eng, 0.01)
+        exact += assert_fp64_parity(c, state, lg, out, sc, m)
+        resync(eng, sc)
+    assert exact >= 2
+
+
+def test_fp32_against_fp64_oracle(bank):
+    eng = mixed_scene(bank, 10, 5, params=dict(precision="fp32", max_iterations=40))
+    sc = OS.from_engine(eng)
+    for _ in range(2):
+        start = sc.copy()
+        margins = {}
+        out = OST.step(sc, 0.01, margins)
+        c, state, lg = gpu_step(eng, 0.01)
+        oc = out["contacts"]
+        kg = {(int(a), int(b), int(t)): i for i, (a, b, t) in enumerate(zip(c["ia"], c["ib"], c["tri"]))}
+        ko = {(int(a), int(b), int(t)): i for i, (a, b, t) in enumerate(zip(oc.ia, oc.ib, oc.tri))}
+        diff = set(kg) ^ set(ko)
+        exempt = {k for k in diff if margins.get(k, np.inf) < EPS_MARGIN}
+
+        def info(k):
+            src, arr = (c["p"][kg[k]], "gpu") if k in kg else (oc.p[ko[k]], "oracle")
+            near = min((np.abs(oc.p[j] - src).max() for kk, j in ko.items() if kk != k
+                        and {kk[0], kk[1]} == {k[0], k[1]}), default=None)
+            return k, arr, margins.get(k), near
+        assert diff == exempt, [info(k) for k in sorted(diff - exempt)][:5]
+        for k in set(kg) & set(ko):
+            if margins.get(k, np.inf) < EPS_MARGIN:
+                continue
+            i, j = kg[k], ko[k]
+            assert np.allclose(c["p"][i], oc.p[j], atol=2e-5)
+            assert np.allclose(c["n"][i], oc.n[j], atol=2e-4)
+        if oc.K:
+            # an FP32 solve takes its own path through the same iteration count;
+            # it must converge as far as the FP64 one, within 25 %
+            assert lg["natural_residual"] <= 1.25 * out["log"]["natural_residual"] + 1e-5
+        if not diff:
+            r, q, v, w = state
+            assert np.abs(r - sc.r).max() < 1e-4 and np.abs(v - sc.v).max() < 2e-3
+        # the next step starts both from the oracle's state
+        for k, b in enumerate(eng.bodies.values()):
+            b.r, b.q, b.v, b.w = sc.r[k].copy(), sc.q[k].copy(), sc.v[k].copy(), sc.w[k].copy()
+        del start
+
+
+def test_schedules_bitwise_fp32(bank):
+    res = []
+    for flags in (native.STEP_FORCE_SINGLE, native.STEP_FORCE_MULTI):
+        eng = mixed_scene(bank, 12, 7, params=dict(precision="fp32", max_iterations=30))
+        arrs = []
+        for _ in range(3):
+            c, state, lg = gpu_step(eng, 0.01, flags)
+            arrs.append([c["impulse"], c["color"]] + list(state))
+        res.append(arrs)
+    for a, b in zip(*res):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_batched_equals_alone(bank):
+    from rainbow_amd.stepper import engine_desc, gather_state
+    engines = [scenes.mixed_drop(3000 + s, bank, nmax=24, params=dict(precision="fp32")) for s in range(3)]
+    for e in engines:
+        for b in e.bodies.values():
+            if not b.is_fixed:
+                b.r = b.r - np.array([0.0, 0.7 * b.r[1] - 0.3, 0.0])   # start near the ground
+    ctx = native.Context(0, "fp32")
+    ids = []
+    for e in engines:
+        desc, keep = engine_desc(e, ctx)
+        sid = ctx.scene_create(desc)
+        ctx.set_state(sid, *gather_state(list(e.bodies.values())))
+        ids.append(sid)
+    ctx.step(ids, 5, 0.01)
+    batched = [ctx.get_state(s, len(e.bodies)) for s, e in zip(ids, engines)]
+    bc = ctx.get_contacts(ids[1])
+    for i, e in enumerate(engines):
+        c2 = native.Context(0, "fp32")
+        desc, keep = engine_desc(e, c2)
+        sid = c2.scene_create(desc)
+        c2.set_state(sid, *gather_state(list(e.bodies.values())))
+        c2.step([sid], 5, 0.01)
+        alone = c2.get_state(sid, len(e.bodies))
+        for x, y in zip(batched[i], alone):
+            assert np.array_equal(x, y)
+        if i == 1:
+            ac = c2.get_contacts(sid)
+            assert np.array_equal(ac["impulse"], bc["impulse"]) and np.array_equal(ac["tri"], bc["tri"])
+        c2.close()
+
+
+def test_graph_replay_equals_eager(bank):
+    from rainbow_amd.stepper import engine_desc, gather_state
+    engines = [scenes.mixed_drop(3100 + s, bank, nmax=16, params=dict(precision="fp32")) for s in range(2)]
+    out = []
+    for flags in (0, native.STEP_GRAPH):
+        ctx = native.Context(0, "fp32")
+        ids = []
+        for e in engines:
+            desc, keep = engine_desc(e, ctx)
+            sid = ctx.scene_create(desc)
+            ctx.set_state(sid, *gather_state(list(e.bodies.values())))
+            ids.append(sid)
+        states = []
+        for rep in range(4):     # several replays with host work and syncs in between
+            ctx.step(ids, 30, 0.01, flags)
+            states.append([ctx.get_state(s, len(e.bodies)) for s, e in zip(ids, engines)])
+        out.append(states)
+        ctx.close()
+    for a, b in zip(*out):
+        for sa, sb in zip(a, b):
+            for x, y in zip(sa, sb):
+                assert np.array_equal(x, y)
+
+
+def test_simulate_and_log_keys(bank):
+    eng = scenes.box_stack(bank, params=dict(max_iterations=20))
+    API.simulate(eng, 0.03, profiling_on=True)
+    log = API.get_log(eng)
+    assert len(log) == 3
+    keys = {"update_bvh", "narrow_phase", "number_of_overlaps", "model_space_update", "contact_optimization",
+            "contact_point_generation", "contact_determination", "contact_point_reduction",
+            "collision_detection_time", "residuals", "iterations", "exitcode", "solver_time", "stepper_time", "dt",
+            "contact_points", "kinetic_energy", "potential_energy", "max_penetration"}
+    assert keys <= set(log[0])
+    assert len(eng.contact_points) == log[-1]["contact_points"] > 0
+    cp = eng.contact_points[0]
+    assert abs(np.linalg.norm(cp.n) - 1) < 1e-5 and cp.bodyA.name in eng.bodies
+    API.simulate(eng, 0.05)   # no profiling: steps batched on the device
+    assert len(API.get_log(eng)) == 3 and eng.step_index == 8

@@ -1,0 +1,117 @@
+"""Host-side API (the reference's api.py surface), meshes and mass properties."""
+import numpy as np
+import pytest
+
+import rainbow_amd.api as API
+from rainbow_amd import mass as MASS
+from rainbow_amd import math3 as Q
+from rainbow_amd import mesh as MESH
+
+
+def test_engine_without_stepper_cannot_simulate():
+    eng = API.Engine()
+    with pytest.raises(ValueError):
+        API.simulate(eng, 0.1)
+    eng = API.create_engine(backend=None)
+    with pytest.raises(ValueError):
+        API.simulate(eng, 1.0)
+    with pytest.raises(ValueError):
+        API.simulate(API.create_engine(backend=None), -1.0)
+
+
+def test_api_errors_and_names():
+    eng = API.create_engine(backend=None)
+    API.create_rigid_body(eng, "a")
+    with pytest.raises(RuntimeError):
+        API.create_rigid_body(eng, "a")
+    with pytest.raises(RuntimeError):
+        API.set_position(eng, "nope", np.zeros(3))
+    with pytest.raises(RuntimeError):
+        API.create_gravity_force(eng, "g", -1.0, np.array([0, 1, 0.0]))
+    with pytest.raises(RuntimeError):
+        API.create_damping_force(eng, "d", 0.0, 1.0)
+    with pytest.raises(RuntimeError):
+        API.create_surfaces_interaction(eng, "default", "default", 0.0, [0.5, 0.5, 0.5])
+    with pytest.raises(RuntimeError):
+        API.create_surfaces_interaction(eng, "a", "b", -0.1, [0.5, 0.5, 0.5])
+    with pytest.raises(RuntimeError):
+        API.create_surfaces_interaction(eng, "a", "b", 0.1, [0.5, -0.5, 0.5])
+    API.create_surfaces_interaction(eng, "b", "a", 0.2, np.array([0.1, 0.2, 0.3]))
+    assert eng.surfaces_interactions.get_interaction("a", "b").epsilon == 0.2
+    with pytest.raises(RuntimeError):
+        API.set_body_material(eng, "a", "unknown")
+    API.set_body_material(eng, "a", "b")
+    with pytest.raises(RuntimeError):
+        API.set_body_type(eng, "a", "floating")
+    API.set_body_type(eng, "a", "fixed")
+    assert API.is_fixed_body(eng, "a")
+    with pytest.raises(RuntimeError):
+        API.set_mass_properties(eng, "a", 1.0)      # no shape
+    n = API.generate_unique_name("x y.z:w-v")
+    assert all(c not in n for c in " .:-")
+
+
+def test_box_mass_properties():
+    # test_mass.py:33-130: a 4 x 5 x 6 box has mass 120 and the box inertia
+    # closed form, under translation and rotation
+    w, h, d = 4.0, 5.0, 6.0
+    I = 120.0 / 12.0 * np.array([h * h + d * d, w * w + d * d, w * w + h * h])
+    for xf in ("translate", "rotate"):
+        V, T = MESH.create_box(w, h, d)
+        if xf == "translate":
+            V = V + np.array([2.0, 10.0, -1.0])
+        else:
+            V = Q.rotate_array(Q.Rx(0.3 * np.pi), V)
+        mesh = API.create_mesh(V, T)
+        p = MASS.compute_mass_properties(mesh.V, mesh.T, 1.0)
+        assert p.mass == pytest.approx(120.0, rel=1e-12)
+        r, q, m, Ib = MASS.xform_model_2_body_space(p)
+        assert np.allclose(sorted(Ib), sorted(I), rtol=1e-10)
+        assert list(Ib) == sorted(Ib, reverse=True)
+        R = Q.to_matrix(q)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-12) and np.linalg.det(R) > 0
+
+
+def test_mesh_tables():
+    # surface_mesh.py:286-329 (box) and 85-171 (profile sweep counts)
+    V, T = MESH.create_box(1.0, 1.0, 1.0)
+    assert V.shape == (8, 3) and T.shape == (12, 3)
+    assert T[0].tolist() == [0, 2, 3] and T[11].tolist() == [3, 6, 7]
+    V, T = MESH.create_sphere(0.5, 16, 16)
+    assert len(V) == (16 - 2) * 16 + 2 and len(T) == 2 * 16 * (16 - 2)
+    assert np.allclose(np.linalg.norm(V, axis=1), 0.5)
+    # closed: every edge is shared by exactly two triangles
+    edges = {}
+    for t in T:
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            edges[tuple(sorted((a, b)))] = edges.get(tuple(sorted((a, b))), 0) + 1
+    assert set(edges.values()) == {2}
+    p = MASS.compute_mass_properties(V, T, 1.0)
+    assert 0.5 < p.mass < 4 / 3 * np.pi * 0.125
+
+
+def test_direction_of_most_variance_row_convention():
+    # functions.py:40-48 returns vectors[argmax(values)], a row of the eigenvector
+    # matrix (test_math_functions.py:102-115)
+    rng = np.random.default_rng(0)
+    P = rng.normal(size=(100, 3)) * np.array([3.0, 1.0, 0.5])
+    mean = P.mean(axis=0)
+    vals, vecs = np.linalg.eig(np.cov((P - mean).T))
+    assert np.array_equal(MESH.direction_of_most_variance(P), vecs[np.argmax(vals)])
+
+
+def test_axis_sort_permutes_consistently():
+    V, T = MESH.create_sphere(0.5, 8, 8)
+    m = API.create_mesh(V, T)
+    tri_old = {tuple(sorted(map(tuple, np.round(V[t], 12)))) for t in T}
+    tri_new = {tuple(sorted(map(tuple, np.round(m.V[t], 12)))) for t in m.T}
+    assert tri_old == tri_new
+    axis = MESH.direction_of_most_variance(V)
+    d = m.V @ (axis / np.linalg.norm(axis))
+    assert np.all(np.diff(d) >= -1e-15)
+    assert np.all(np.diff(m.T.mean(axis=1)) >= 0)
+
+
+def test_substeps_match_reference_loop():
+    assert API.substeps(0.025, 0.01) == pytest.approx([0.01, 0.01, 0.005])
+    assert len(API.substeps(1.0, 0.005)) >= 200

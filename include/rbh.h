@@ -178,16 +178,29 @@ int rbh_get_log(rbh_ctx* ctx, int scene_id, rbh_step_log* log, double* residuals
 int rbh_get_contacts(rbh_ctx* ctx, int scene_id, int cap, int* count, int* ia, int* ib, int* tri, double* p,
                      double* n, double* gap, double* impulse, int* color);
 
-/* GPU milliseconds per stage of the last rbh_step with RBH_STEP_TIMING:
+/* GPU milliseconds per stage, averaged over the steps of the last rbh_step
+ * call made with RBH_STEP_TIMING (its last 512 steps at most):
  * [0] body prep + refit (update_bvh), [1] broad phase + candidates (narrow_phase),
  * [2] Frank-Wolfe + reduction (contact_determination), [3] assembly,
  * [4] colouring + order, [5] solve (solver_time), [6] integration,
- * [7] total (stepper_time).  Returns the number of entries written. */
+ * [7] total (stepper_time).  With RBH_STEP_GRAPH only [7] is measured (per
+ * replay).  The stage names in parentheses are the profile keys of
+ * steppers.py:55-100 and collision_detection.py:10-80, 204-252.  Returns the
+ * number of entries written. */
 int rbh_get_timings(rbh_ctx* ctx, double* ms, int cap);
 
 /* Batch-level counters of the last step: [0] body pairs, [1] candidate slots,
  * [2] candidate triangles, [3] contacts. */
 int rbh_get_counters(rbh_ctx* ctx, long long* out, int cap);
+
+/* Running totals of a scene since its creation or the last clear (clear != 0
+ * zeroes them after reading): [0] committed steps, [1] contact-iterations
+ * (contacts x sweeps, rejected sweeps included), [2] contacts, [3] candidate
+ * triangles, [4] trilinear cell evaluations in Frank-Wolfe, [5] sweeps.
+ * Kept on the device, so reading them is the only synchronisation.  No
+ * reference counterpart: the reference's profiler (steppers.py:95-106) keeps
+ * per-step values only.  Returns the number of entries written. */
+int rbh_scene_totals(rbh_ctx* ctx, int scene_id, unsigned long long* out, int cap, int clear);
 
 #ifdef __cplusplus
 }

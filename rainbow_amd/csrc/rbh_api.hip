@@ -131,8 +131,10 @@ struct rbh_ctx {
   double graph_dt = 0;
   int graph_flags = -1;
   // timing
-  hipEvent_t ev[ST_N]{};
+  // timing: ST_N events per step, for up to TIMED_STEPS steps of one call
+  std::vector<hipEvent_t> evp;
   double timings[ST_N]{};
+  int timed_steps = 0;
   long long counters[4]{};
 };
 
@@ -479,11 +481,27 @@ template <typename T> void scan(rbh_ctx* c, const T* in, int* out, const int* n_
   hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, part, out);
 }
 
-template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, bool timing, bool multi, int multi_grid) {
+// Events of step i of a call of `todo` steps: the last TIMED_STEPS steps are
+// timed, each on its own row of ST_N events (created on first use).
+constexpr int TIMED_STEPS = 512;
+hipEvent_t* timing_row(rbh_ctx* c, int i, int todo) {
+  const int first = std::max(0, todo - TIMED_STEPS);
+  if (i < first) return nullptr;
+  const size_t need = (size_t)std::min(todo, TIMED_STEPS) * ST_N;
+  while (c->evp.size() < need) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->evp.push_back(e);
+  }
+  return &c->evp[(size_t)(i - first) * ST_N];
+}
+
+// ev: ST_N events to record the stage boundaries into, or nullptr.
+template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, hipEvent_t* ev, bool multi, int multi_grid) {
   Dev<R> d = make_dev<R>(c, dt, flags);
   hipStream_t s = c->stream;
   const int nb = c->nb, ns = c->ns;
-  auto mark = [&](int st) { if (timing) (void)hipEventRecord(c->ev[st], s); };
+  auto mark = [&](int st) { if (ev) (void)hipEventRecord(ev[st], s); };
   mark(ST_BEGIN);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, d.err, 16, 0);
   hipLaunchKernelGGL(k_scene_reset<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
@@ -525,9 +543,9 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, b
   hipLaunchKernelGGL(k_color_units<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
   if (multi) {
     hipLaunchKernelGGL(k_order_units<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
-    mark(ST_COLOR);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(d.merit), 2 * c->max_iter_batch, 0);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(d.bar), 16, 0);
+    mark(ST_COLOR);
     void* args[] = {&d};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<void*>(k_solve_multi<R>), dim3(multi_grid),
                                               dim3(SOLVE_BLOCK), args, 0, s);
@@ -583,7 +601,7 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
       if (!c->gexec || c->graph_key_gen != c->arena_gen || c->graph_dt != dt || c->graph_flags != kflags) {
         drop_graph(c);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t le = launch_step<R>(c, dt, kflags, false, false, 1);
+        hipError_t le = launch_step<R>(c, dt, kflags, nullptr, false, 1);
         if (le != hipSuccess) {
           (void)hipStreamEndCapture(c->stream, &c->graph);
           return fail(c, RBH_E_HIP, std::string("step launch: ") + hipGetErrorString(le));
@@ -594,13 +612,19 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
         c->graph_dt = dt;
         c->graph_flags = kflags;
       }
-      for (int i = 0; i < todo; ++i) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+      for (int i = 0; i < todo; ++i) {
+        hipEvent_t* ev = timing ? timing_row(c, i, todo) : nullptr;
+        if (ev) HIPCHK(c, hipEventRecord(ev[ST_BEGIN], c->stream));
+        HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        if (ev) HIPCHK(c, hipEventRecord(ev[ST_INT], c->stream));
+      }
     } else {
       for (int i = 0; i < todo; ++i) {
-        hipError_t le = launch_step<R>(c, dt, kflags, timing && i == todo - 1, multi, multi_grid);
+        hipError_t le = launch_step<R>(c, dt, kflags, timing ? timing_row(c, i, todo) : nullptr, multi, multi_grid);
         if (le != hipSuccess) return fail(c, RBH_E_HIP, std::string("step launch: ") + hipGetErrorString(le));
       }
     }
+    c->timed_steps = timing ? std::min(std::min(todo, TIMED_STEPS), (int)(c->evp.size() / ST_N)) : 0;
     int e[8];
     rc = read_err(c, e);
     if (rc) return rc;
@@ -640,15 +664,22 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
   HIPCHK(c, hipMemcpyAsync(&h[3], c->n_contacts.p, 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < 4; ++i) c->counters[i] = h[i];
-  if (timing && !use_graph) {
-    for (int i = 1; i < ST_N; ++i) {
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, c->ev[i - 1], c->ev[i]);
-      c->timings[i - 1] = ms;
+  if (timing) {
+    // average over the timed steps of the last attempt; a graph replay has no stage marks
+    for (int i = 0; i < ST_N; ++i) c->timings[i] = 0.0;
+    for (int k = 0; k < c->timed_steps; ++k) {
+      hipEvent_t* ev = &c->evp[(size_t)k * ST_N];
+      if (!use_graph)
+        for (int i = 1; i < ST_N; ++i) {
+          float ms = 0;
+          HIPCHK(c, hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
+          c->timings[i - 1] += ms;
+        }
+      float tot = 0;
+      HIPCHK(c, hipEventElapsedTime(&tot, ev[ST_BEGIN], ev[ST_INT]));
+      c->timings[ST_N - 1] += tot;
     }
-    float tot = 0;
-    (void)hipEventElapsedTime(&tot, c->ev[ST_BEGIN], c->ev[ST_INT]);
-    c->timings[ST_N - 1] = tot;
+    for (int i = 0; i < ST_N; ++i) c->timings[i] /= std::max(1, c->timed_steps);
   }
   for (size_t i = 0; i < c->batch.size(); ++i) {
     c->scenes[c->batch[i]].ws_gen = c->ws_gen;
@@ -804,7 +835,6 @@ int rbh_ctx_create(int device, int precision_bytes, unsigned long long hbm_budge
     delete c;
     return RBH_E_HIP;
   }
-  for (int i = 0; i < ST_N; ++i) (void)hipEventCreate(&c->ev[i]);
   c->bodies.init(0);
   c->tris.init(0);
   c->res.init(0);
@@ -818,7 +848,7 @@ int rbh_ctx_destroy(rbh_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   free_all(c);
-  for (int i = 0; i < ST_N; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  for (hipEvent_t e : c->evp) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1126,6 +1156,25 @@ int rbh_get_counters(rbh_ctx* c, long long* out, int cap) {
   if (!c || !out) return RBH_E_ARG;
   int m = std::min(cap, 4);
   for (int i = 0; i < m; ++i) out[i] = c->counters[i];
+  return m;
+}
+
+int rbh_scene_totals(rbh_ctx* c, int id, unsigned long long* out, int cap, int clear) {
+  if (!c || !valid_scene(c, id) || (cap > 0 && !out)) return fail(c, RBH_E_ARG, "unknown scene or null output");
+  HIPCHK(c, hipSetDevice(c->device));
+  SceneState st;
+  SceneState* dst = reinterpret_cast<SceneState*>(c->d_ss.p) + id;
+  HIPCHK(c, hipMemcpyAsync(&st, dst, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const unsigned long long t[6] = {st.tot_steps, st.tot_contact_iters, st.tot_contacts, st.tot_cand,
+                                   st.tot_fw_evals, st.tot_sweeps};
+  int m = std::max(0, std::min(cap, 6));
+  for (int i = 0; i < m; ++i) out[i] = t[i];
+  if (clear) {
+    st.tot_steps = st.tot_contact_iters = st.tot_contacts = st.tot_cand = st.tot_fw_evals = st.tot_sweeps = 0;
+    HIPCHK(c, hipMemcpyAsync(dst, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   return m;
 }
 

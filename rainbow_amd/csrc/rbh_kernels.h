@@ -39,6 +39,8 @@ struct SceneState {
   unsigned long long fw_evals;   // distinct trilinear cell evaluations in Frank-Wolfe
   double nat_r, nat_x;     // natural-map residual: max |x - prox(...)| and max |x| (FP64)
   double max_pen, final_merit;
+  // running totals since the scene was created or they were last cleared
+  unsigned long long tot_steps, tot_contact_iters, tot_contacts, tot_cand, tot_fw_evals, tot_sweeps;
 };
 
 constexpr int PALW = 64;                 // palette words per contact
@@ -374,6 +376,7 @@ template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
   if (aborted(d.err)) return;
   const int nc = *d.n_cand;
   unsigned long long evals = 0;
+  int ncand = 0;
   int cur = -1;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
     int p = d.cand_pair[c];
@@ -386,9 +389,12 @@ template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
     const int slot = d.b_slot[own];
     if (slot != cur) {
       if (cur >= 0 && evals) atomicAdd(&d.ss[cur].fw_evals, evals);
+      if (cur >= 0 && ncand) atomicAdd(&d.ss[cur].n_cand, ncand);
       cur = slot;
       evals = 0;
+      ncand = 0;
     }
+    ++ncand;
     const SceneParams& sp = d.sp[slot];
     const ShapeInfo& sA = d.shapes[d.b_shape[own]];
     GridView<R> g = grid_of(d, d.b_shape[sdf]);
@@ -455,9 +461,11 @@ template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
     int s0 = __shfl(cur, leader, 64);
     bool mine = cur == s0 && cur >= 0;
     unsigned long long v = mine ? evals : 0ull;
+    int u = mine ? ncand : 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    for (int o = 32; o > 0; o >>= 1) { v += __shfl_xor(v, o, 64); u += __shfl_xor(u, o, 64); }
     if ((int)(threadIdx.x & 63) == leader && v) atomicAdd(&d.ss[s0].fw_evals, v);
+    if ((int)(threadIdx.x & 63) == leader && u) atomicAdd(&d.ss[s0].n_cand, u);
     live &= ~__ballot(mine);
   }
 }
@@ -634,7 +642,7 @@ template <typename R> __global__ void k_scene_reset(Dev<R> d) {
   if (s >= d.ns) return;
   SceneState& st = d.ss[d.bs_slot[s]];
   st.k_off = 0x7fffffff; st.k_cnt = 0; st.ncolor = 0; st.n_overlaps = 0; st.fw_evals = 0; st.max_pen = 0.0;
-  st.solved = 0; st.nat_r = 0.0; st.nat_x = 0.0;
+  st.solved = 0; st.nat_r = 0.0; st.nat_x = 0.0; st.n_cand = 0; st.sweeps = 0;
 }
 
 template <typename R> __global__ void k_scene_finalize(Dev<R> d) {
@@ -643,6 +651,13 @@ template <typename R> __global__ void k_scene_finalize(Dev<R> d) {
   SceneState& st = d.ss[d.bs_slot[s]];
   if (st.k_cnt == 0) st.k_off = 0;
   st.step_index += 1;
+  const unsigned long long sw = st.solved ? (unsigned long long)st.sweeps : 0ull;
+  st.tot_steps += 1;
+  st.tot_contact_iters += sw * (unsigned long long)st.k_cnt;
+  st.tot_contacts += (unsigned long long)st.k_cnt;
+  st.tot_cand += (unsigned long long)st.n_cand;
+  st.tot_fw_evals += st.fw_evals;
+  st.tot_sweeps += sw;
 }
 
 }  // namespace rbh

@@ -86,6 +86,7 @@ _PROTOS = {
                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "rbh_get_timings": [C.c_void_p, C.c_void_p, C.c_int],
     "rbh_get_counters": [C.c_void_p, C.c_void_p, C.c_int],
+    "rbh_scene_totals": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int],
 }
 
 
@@ -280,3 +281,11 @@ class Context:
         c = np.zeros(4, dtype=np.int64)
         self._check(self.lib.rbh_get_counters(self.h, _ptr(c), 4))
         return dict(zip(["body_pairs", "candidate_slots", "candidates", "contacts"], c.tolist()))
+
+    TOTAL_KEYS = ("steps", "contact_iters", "contacts", "candidates", "fw_cell_evals", "sweeps")
+
+    def scene_totals(self, sid, clear=False):
+        """Running work totals of a scene (kept on the device)."""
+        t = np.zeros(len(self.TOTAL_KEYS), dtype=np.uint64)
+        self._check(self.lib.rbh_scene_totals(self.h, int(sid), _ptr(t), len(t), 1 if clear else 0))
+        return dict(zip(self.TOTAL_KEYS, (int(x) for x in t)))

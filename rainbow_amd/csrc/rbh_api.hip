@@ -119,8 +119,16 @@ struct rbh_ctx {
   DBuf c_ia, c_ib, c_tri, c_pair, c_p, c_n, n_contacts, c_rA, c_rB, c_b, c_r0, c_mu, c_imp, c_color;
   DBuf f_bs, uf, isl_cnt, isl_off, isl_fill, isl_max, c_lab, uperm, u_cnt, u_base, n_units, u_start, u_end, u_ncol,
       u_bs, body_claim, body_loc, bc_cnt, bc_off, bc_fill, bc_list, pal, g_pc, g_done, col_off, merit, bar;
+  DBuf u_T, isl_unit, u_nbody, u_boff, f_loc, f_unit, u_blist, s_loc, s_W;
+  DBuf l_cnt, l_base, n_lanes, l_start, l_end, l_ncol, l_bs;
   DBuf perm, s_id, s_n, s_rA, s_rB, s_b, s_r0, x0, x1, errbuf, scan_part;
-  int cap_pairs = 0, cap_slots = 0, cap_cand = 0, cap_contacts = 0, cap_units = 0, unit_T = 2048;
+  int cap_pairs = 0, cap_slots = 0, cap_cand = 0, cap_contacts = 0, cap_units = 0;
+  int unit_T = 64;          // contacts per colouring unit (a scene's units are windows of this size)
+  int unit_split = 0;       // > 0: a scene solved by one workgroup is cut into about this many units
+  int unit_T_tuned = 0;     // unit size set through rbh_set_tuning (0: default)
+  // resident solve groups: LDS capacities (contacts, bodies, colour offsets) and the launch's dynamic LDS
+  int res_cap_c = 0, res_cap_b = 0, res_cap_o = 0;
+  size_t solve_lds = 0;
   int max_iter_batch = 1, coop_max = 0, last_units = 0;
   bool multi_ok = false;
   long long ws_gen = 0;
@@ -187,7 +195,9 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->c_lab, &c->uperm, &c->u_cnt, &c->u_base, &c->n_units, &c->u_start, &c->u_end, &c->u_ncol,
           &c->u_bs, &c->body_claim, &c->body_loc, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->pal,
           &c->g_pc, &c->g_done, &c->col_off, &c->merit, &c->bar, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
-          &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part};
+          &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part, &c->u_T, &c->isl_unit,
+          &c->u_nbody, &c->u_boff, &c->f_loc, &c->f_unit, &c->u_blist, &c->s_loc, &c->s_W, &c->l_cnt,
+          &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs};
 }
 
 size_t total_bytes(rbh_ctx* c) {
@@ -355,7 +365,14 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags) {
   d.f_bs = P(const int*, f_bs); d.uf = P(int*, uf); d.isl_cnt = P(int*, isl_cnt); d.isl_off = P(int*, isl_off);
   d.isl_fill = P(int*, isl_fill); d.isl_max = P(int*, isl_max); d.c_lab = P(int*, c_lab); d.uperm = P(int*, uperm);
   d.u_cnt = P(int*, u_cnt); d.u_base = P(int*, u_base); d.n_units = P(int*, n_units); d.cap_units = c->cap_units;
-  d.unit_T = c->unit_T; d.u_start = P(int*, u_start); d.u_end = P(int*, u_end); d.u_ncol = P(int*, u_ncol);
+  d.unit_T = c->unit_T; d.unit_split = c->unit_split; d.u_T = P(int*, u_T);
+  d.isl_unit = P(int*, isl_unit); d.u_nbody = P(int*, u_nbody); d.u_boff = P(int*, u_boff);
+  d.f_loc = P(int*, f_loc); d.f_unit = P(int*, f_unit); d.u_blist = P(int*, u_blist);
+  d.s_loc = P(int2*, s_loc); d.s_W = P(V4<R>*, s_W);
+  d.res_cap_c = c->res_cap_c; d.res_cap_b = c->res_cap_b; d.res_cap_o = c->res_cap_o;
+  d.l_cnt = P(int*, l_cnt); d.l_base = P(int*, l_base); d.n_lanes = P(int*, n_lanes);
+  d.l_start = P(int*, l_start); d.l_end = P(int*, l_end); d.l_ncol = P(int*, l_ncol); d.l_bs = P(int*, l_bs);
+  d.u_start = P(int*, u_start); d.u_end = P(int*, u_end); d.u_ncol = P(int*, u_ncol);
   d.u_bs = P(int*, u_bs); d.body_claim = P(int*, body_claim); d.body_loc = P(int2*, body_loc);
   d.bc_cnt = P(int*, bc_cnt); d.bc_off = P(int*, bc_off); d.bc_fill = P(int*, bc_fill);
   d.bc_list = P(int*, bc_list); d.pal = P(unsigned*, pal); d.g_pc = P(int*, g_pc);
@@ -379,7 +396,7 @@ int size_workspace(rbh_ctx* c) {
   const size_t nb = std::max(c->nb, 1), ns = std::max(c->ns, 1);
   c->cap_units = (int)std::min<long long>((long long)(c->cap_contacts + c->unit_T - 1) / c->unit_T + c->ns + 1, 1 << 20);
   const size_t nu = c->cap_units, nl = (size_t)c->nb + c->ns + 2;
-  int maxcap = std::max({c->cap_pairs, c->cap_slots, c->cap_cand, c->cap_contacts, c->nb + c->ns + 2, c->ns});
+  int maxcap = std::max({c->cap_pairs, c->cap_slots, c->cap_cand, c->cap_contacts, c->nb + c->ns + 2, c->ns, c->cap_units});
   struct E { DBuf* b; size_t bytes; } el[] = {
       {&c->row_cnt, nb * 4}, {&c->row_off, (nb + 1) * 4},
       {&c->pairs, cp * 8}, {&c->pair_sz, cp * 4}, {&c->slot_off, (cp + 1) * 4}, {&c->n_pairs, 16},
@@ -395,7 +412,11 @@ int size_workspace(rbh_ctx* c) {
       {&c->u_end, nu * 4}, {&c->u_ncol, nu * 4}, {&c->u_bs, nu * 4}, {&c->body_claim, nb * 4},
       {&c->body_loc, nb * 8}, {&c->bc_cnt, nb * 4}, {&c->bc_off, (nb + 1) * 4}, {&c->bc_fill, nb * 4},
       {&c->bc_list, 2 * ck * 4}, {&c->pal, ck * PALW * 4}, {&c->g_pc, ck * 4}, {&c->g_done, ck + 16},
-      {&c->col_off, std::max(ns, nu) * (PALMAX + 1) * 4}, {&c->merit, (size_t)std::max(c->max_iter_batch, 1) * 8},
+      {&c->col_off, (2 * (ck + nu) + 16) * 4}, {&c->merit, (size_t)std::max(c->max_iter_batch, 1) * 8},
+      {&c->u_T, ns * 4}, {&c->isl_unit, nl * 4}, {&c->u_nbody, nu * 4}, {&c->u_boff, (nu + 1) * 4},
+      {&c->f_loc, nb * 4}, {&c->f_unit, nb * 4}, {&c->u_blist, nb * 4}, {&c->s_loc, ck * 8},
+      {&c->s_W, 4 * ck * V}, {&c->l_cnt, ns * 4}, {&c->l_base, (ns + 1) * 4}, {&c->n_lanes, 16},
+      {&c->l_start, nu * 4}, {&c->l_end, nu * 4}, {&c->l_ncol, nu * 4}, {&c->l_bs, nu * 4},
       {&c->bar, 64},
       {&c->perm, ck * 4}, {&c->s_id, ck * 8}, {&c->s_n, ck * V}, {&c->s_rA, ck * V}, {&c->s_rB, ck * V},
       {&c->s_b, ck * V}, {&c->s_r0, ck * V}, {&c->x0, ck * V}, {&c->x1, ck * V}, {&c->errbuf, 64},
@@ -447,7 +468,8 @@ int build_batch(rbh_ctx* c, const int* ids, int n) {
   // workgroups; batches: one workgroup per scene
   c->max_iter_batch = max_iter;
   c->multi_ok = (n == 1 && !serial);
-  c->unit_T = c->multi_ok ? 128 : UNIT_CAP;
+  c->unit_T = c->unit_T_tuned > 0 ? c->unit_T_tuned : 64;
+  c->unit_split = c->multi_ok ? 0 : SOLVE_BLOCK / 64;
   c->last_units = 0;
   // initial capacities; they grow when a step reports an overflow
   long long est_pairs = std::min<long long>(pairs_all, 64LL * c->nb + 4096);
@@ -480,6 +502,9 @@ template <typename T> void scan(rbh_ctx* c, const T* in, int* out, const int* n_
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, c->stream, part, n_dev, cap, total);
   hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, part, out);
 }
+
+// steps between re-sizings of a single scene's cooperative solve grid
+constexpr int GRID_REFRESH = 16;
 
 // Events of step i of a call of `todo` steps: the last TIMED_STEPS steps are
 // timed, each on its own row of ST_N events (created on first use).
@@ -535,25 +560,30 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_isl_fill<R>, dim3(G), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_unit_count<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
   scan<int>(c, d.u_cnt, d.u_base, d.d_nb + 2, ns, d.n_units);
+  scan<int>(c, d.l_cnt, d.l_base, d.d_nb + 2, ns, d.n_lanes);
   hipLaunchKernelGGL(k_unit_reset<R>, dim3(grid_for(c->cap_units, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_unit_assign<R>, dim3(grid_for(nb + ns, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_csr_count<R>, dim3(G), dim3(256), 0, s, d);
   scan<int>(c, d.bc_cnt, d.bc_off, d.d_nb, nb, nullptr);
   hipLaunchKernelGGL(k_csr_fill<R>, dim3(G), dim3(256), 0, s, d);
+  // free bodies -> slots of their unit
+  hipLaunchKernelGGL(k_unit_bodies<R>, dim3(grid_for(nb, 256)), dim3(256), 0, s, d);
+  scan<int>(c, d.u_nbody, d.u_boff, d.n_units, c->cap_units, nullptr);
+  hipLaunchKernelGGL(k_unit_blist<R>, dim3(grid_for(nb, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_color_small<R>, dim3(c->cap_units), dim3(64), 0, s, d);
   hipLaunchKernelGGL(k_color_units<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_order_lanes<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
   if (multi) {
-    hipLaunchKernelGGL(k_order_units<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(d.merit), 2 * c->max_iter_batch, 0);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(d.bar), 16, 0);
     mark(ST_COLOR);
     void* args[] = {&d};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<void*>(k_solve_multi<R>), dim3(multi_grid),
-                                              dim3(SOLVE_BLOCK), args, 0, s);
+                                              dim3(SOLVE_BLOCK), args, c->solve_lds, s);
     if (e != hipSuccess) return e;
   } else {
-    hipLaunchKernelGGL(k_order<R>, dim3(ns), dim3(COLOR_BLOCK), 0, s, d);
     mark(ST_COLOR);
-    hipLaunchKernelGGL(k_solve<R>, dim3(ns), dim3(SOLVE_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_solve<R>, dim3(ns), dim3(SOLVE_BLOCK), c->solve_lds, s, d);
   }
   mark(ST_SOLVE);
   hipLaunchKernelGGL(k_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
@@ -577,20 +607,47 @@ int first_step_index(rbh_ctx* c, int slot, int* out) {
   return 0;
 }
 
+// Dynamic LDS of the solve kernels: the resident-group capacities (FP32:
+// 512 contacts, 512 bodies; FP64: half) or the velocity-only layout,
+// whichever is larger; the kernels' limit is raised to it.
+template <typename R> int setup_solve_lds(rbh_ctx* c) {
+  const int scale = sizeof(R) == 4 ? 2 : 1;
+  c->res_cap_c = 256 * scale;
+  c->res_cap_b = 256 * scale;
+  c->res_cap_o = 1024 * scale;
+  size_t bytes = std::max<size_t>(SOLVE_LDS_BYTES, res_lds_bytes<R>(c->res_cap_c, c->res_cap_b, c->res_cap_o,
+                                                                    RES_LANES));
+  const void* ks[2] = {reinterpret_cast<const void*>(k_solve<R>), reinterpret_cast<const void*>(k_solve_multi<R>)};
+  for (const void* k : ks)
+    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, RBH_E_HIP, "solve kernels cannot use " + std::to_string(bytes) + " bytes of LDS");
+    }
+  c->solve_lds = bytes;
+  c->coop_max = 0;
+  return 0;
+}
+
 template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flags) {
   const bool timing = (flags & RBH_STEP_TIMING) != 0;
   const bool use_graph = (flags & RBH_STEP_GRAPH) != 0;
   const int kflags = flags & RBH_STEP_COUNT_OVERLAPS;
+  if (c->solve_lds == 0) {
+    int rc = setup_solve_lds<R>(c);
+    if (rc) return rc;
+  }
   if (c->coop_max == 0) {
     int per_cu = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_multi<R>),
-                                                       SOLVE_BLOCK, 0);
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_multi<R>),
+                                                           SOLVE_BLOCK, c->solve_lds));
     c->coop_max = std::max(1, per_cu) * c->prop.multiProcessorCount;
   }
-  bool multi = c->multi_ok && !use_graph && c->last_units != 1;
+  bool multi = c->multi_ok && !use_graph;
   if (flags & RBH_STEP_FORCE_SINGLE) multi = false;
   if ((flags & RBH_STEP_FORCE_MULTI) && c->multi_ok && !use_graph) multi = true;
-  const int multi_grid = std::max(1, std::min(c->last_units > 0 ? c->last_units : 128, c->coop_max));
+  // cooperative grid: about one unit per wavefront, from the unit count of the last step seen
+  auto grid_of = [&](int units) { return std::max(1, std::min((units + SOLVE_BLOCK / 64 - 1) / (SOLVE_BLOCK / 64), c->coop_max)); };
+  int multi_grid = grid_of(c->last_units > 1 ? c->last_units : 64);
   int done = 0;
   for (int attempt = 0; attempt < 16 && done < n_steps; ++attempt) {
     int si0 = 0;
@@ -622,6 +679,13 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
       for (int i = 0; i < todo; ++i) {
         hipError_t le = launch_step<R>(c, dt, kflags, timing ? timing_row(c, i, todo) : nullptr, multi, multi_grid);
         if (le != hipSuccess) return fail(c, RBH_E_HIP, std::string("step launch: ") + hipGetErrorString(le));
+        if (multi && (i + 1) % GRID_REFRESH == 0 && i + 1 < todo) {
+          // re-size the cooperative grid to the scene's current unit count
+          int nu = 0;
+          HIPCHK(c, hipMemcpyAsync(&nu, c->n_units.p, 4, hipMemcpyDeviceToHost, c->stream));
+          HIPCHK(c, hipStreamSynchronize(c->stream));
+          if (nu > 0) multi_grid = grid_of(nu);
+        }
       }
     }
     c->timed_steps = timing ? std::min(std::min(todo, TIMED_STEPS), (int)(c->evp.size() / ST_N)) : 0;

@@ -90,18 +90,28 @@ template <typename R> struct Dev {
   int* isl_cnt; int* isl_off; int* isl_fill; int* isl_max;   // per island label [nb + ns (+1)]
   int* c_lab;                          // island label per contact
   int* uperm;                          // unit-order position -> contact
-  int* u_cnt; int* u_base; int* n_units; int cap_units; int unit_T;
+  int* u_cnt; int* u_base; int* n_units; int cap_units; int unit_T; int unit_split;
+  int* u_T;                            // sweep-lane size of each batch scene
+  int* l_cnt; int* l_base; int* n_lanes;   // sweep lanes per batch scene, their scan, total
+  int* l_start; int* l_end; int* l_ncol; int* l_bs;   // per lane (capacity cap_units: lanes <= units)
   int* u_start; int* u_end; int* u_ncol; int* u_bs;
+  int* isl_unit;                       // island label -> unit
+  int* u_nbody; int* u_boff;           // free bodies per unit, and their exclusive scan (slot base)
+  int* f_loc; int* f_unit;             // flat body -> slot within its unit (-1: no contact), unit
+  int* u_blist;                        // slot (u_boff order) -> arena body
   int* body_claim; int2* body_loc;     // per flat body, LDS staging of body lists
   int* bc_cnt; int* bc_off; int* bc_fill; int* bc_list;   // flat body -> unit-order positions
   unsigned* pal;                       // PALW words per unit-order position
   int* g_pc; unsigned char* g_done;    // colouring state of units too large for LDS
-  int* col_off;                        // (PALMAX + 1) per batch scene or per unit
+  int* col_off;                        // colour offsets of lane l at 2 * (l_start[l] + l)
   double* merit;                       // per-sweep merit of a scene solved by several workgroups
   unsigned* bar;                       // grid barrier counters
   // solver arrays (colour order)
   int* perm; int2* s_id; V4<R>* s_n; V4<R>* s_rA; V4<R>* s_rB; V4<R>* s_b; V4<R>* s_r0; V4<R>* x0; V4<R>* x1;
-  int* err;                            // [0] flags, [1..] needed capacities
+  int2* s_loc;                         // body slots (u_blist index) of the two bodies (-1: fixed)
+  int res_cap_c, res_cap_b, res_cap_o; // LDS capacities of a resident solve group (contacts, bodies, colour offsets)
+  V4<R>* s_W;                          // 4 per position: W0, W1 of body A, then of body B (zero if fixed)
+  int* err;                           // [0] flags, [1..] needed capacities
   double dt;                           // > 0: use for every scene, else the scene's time_step
   int flags;
 };

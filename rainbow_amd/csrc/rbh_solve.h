@@ -89,7 +89,11 @@ template <typename R> __global__ void k_isl_fill(Dev<R> d) {
   }
 }
 
-// units per batch scene: ceil(K_s / unit_T)
+// Colouring units of batch scene bs: windows of unit_T contact positions; an
+// island joins the window its first position falls in.  Sweep lanes: the
+// same with windows of TL_s = unit_T, or, when unit_split > 0, at least
+// ceil(K_s / unit_split) positions, so that a scene solved by one workgroup
+// has about one lane per wavefront.  Both are unions of whole islands.
 template <typename R> __device__ __forceinline__ void scene_pos_range(const Dev<R>& d, int bs, int& p0, int& p1) {
   p0 = d.isl_off[d.bs_foff[bs] + bs];
   p1 = d.isl_off[d.bs_foff[bs + 1] + bs + 1];
@@ -100,7 +104,11 @@ template <typename R> __global__ void k_unit_count(Dev<R> d) {
     int p0, p1;
     scene_pos_range(d, bs, p0, p1);
     int K = aborted(d.err) ? 0 : p1 - p0;
-    d.u_cnt[bs] = (K + d.unit_T - 1) / d.unit_T;
+    int T = d.unit_T, TL = T;
+    if (d.unit_split > 0) TL = max(T, (K + d.unit_split - 1) / d.unit_split);
+    d.u_cnt[bs] = (K + T - 1) / T;
+    d.u_T[bs] = TL;
+    d.l_cnt[bs] = (K + TL - 1) / TL;
   }
 }
 
@@ -110,6 +118,10 @@ template <typename R> __global__ void k_unit_reset(Dev<R> d) {
     d.u_end[u] = 0;
     d.u_ncol[u] = 0;
     d.u_bs[u] = -1;
+    d.u_nbody[u] = 0;
+    d.l_start[u] = 0x7fffffff;
+    d.l_end[u] = 0;
+    d.l_bs[u] = -1;
   }
 }
 
@@ -139,6 +151,34 @@ template <typename R> __global__ void k_unit_assign(Dev<R> d) {
     atomicMin(&d.u_start[u], start);
     atomicMax(&d.u_end[u], start + cnt);
     d.u_bs[u] = bs;
+    d.isl_unit[lab] = u;
+    int jl = min(d.l_cnt[bs] - 1, (start - p0) / d.u_T[bs]);
+    int l = d.l_base[bs] + jl;
+    atomicMin(&d.l_start[l], start);
+    atomicMax(&d.l_end[l], start + cnt);
+    d.l_bs[l] = bs;
+  }
+}
+
+// Free bodies with contacts lie in exactly one island, so in one unit; each
+// gets a slot in its unit (where its velocity change lives while the unit is
+// solved; the slot order changes no result).
+template <typename R> __global__ void k_unit_bodies(Dev<R> d) {
+  if (aborted(d.err) || *d.n_units > d.cap_units) return;
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < d.nb; f += gridDim.x * blockDim.x) {
+    if (d.bc_cnt[f] == 0) { d.f_loc[f] = -1; continue; }
+    int root = uf_find(d.uf, f);
+    int u = d.isl_unit[root + d.f_bs[root]];
+    d.f_unit[f] = u;
+    d.f_loc[f] = atomicAdd(&d.u_nbody[u], 1);
+  }
+}
+
+template <typename R> __global__ void k_unit_blist(Dev<R> d) {
+  if (aborted(d.err) || *d.n_units > d.cap_units) return;
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < d.nb; f += gridDim.x * blockDim.x) {
+    int l = d.f_loc[f];
+    if (l >= 0) d.u_blist[d.u_boff[d.f_unit[f]] + l] = d.bb_body[f];
   }
 }
 
@@ -168,27 +208,35 @@ template <typename R> __global__ void k_csr_fill(Dev<R> d) {
 // ------------------------------------------------------------ colouring
 // Vivace random colouring per island (proximal_contact/blocking.py:62-111 in
 // the deterministic form of oracle/coloring.py).  Node = contact; two
-// contacts are adjacent if they share a free body.
+// contacts are adjacent if they share a free body.  Units of up to
+// SMALL_UNIT contacts are coloured by one wavefront with the palettes in LDS
+// (ceil((2K + 2) / 32) words per contact: colours of an island stay below
+// twice its size); larger units by a workgroup of four wavefronts, palettes
+// in global memory (PALW words per contact).
 constexpr int COLOR_BLOCK = 256;
-constexpr int UNIT_CAP = 2048;        // contacts of a unit staged in LDS
+constexpr int UNIT_CAP = 2048;        // contacts of a large unit staged in LDS
 constexpr int LIST_CAP = 4096;        // body-list entries staged in LDS
+constexpr int SMALL_UNIT = 256;
+constexpr int SMALL_LIST = 1024;
+constexpr int SMALL_PAL = SMALL_UNIT * ((2 * SMALL_UNIT + 2 + 31) / 32);
 
-struct ColorLds {
-  int key[UNIT_CAP];                  // contact index within its scene (random draws, ranks)
-  int isl[UNIT_CAP];
-  int fa[UNIT_CAP], fb[UNIT_CAP];     // free flat bodies (-1 if fixed)
-  unsigned short ao[UNIT_CAP], ac[UNIT_CAP], bo[UNIT_CAP], bcn[UNIT_CAP];
-  unsigned short col[UNIT_CAP];
-  unsigned short pc[UNIT_CAP];        // palette sizes
-  unsigned char done[UNIT_CAP];       // 0 uncoloured, 1 coloured, 2 won this round
-  unsigned short list[LIST_CAP];
+template <int NT, int CAP, int LCAP, int PAL> struct ColorLds {
+  int key[CAP];                       // contact index within its scene (random draws, ranks)
+  int isl[CAP];
+  int fa[CAP], fb[CAP];               // free flat bodies (-1 if fixed)
+  unsigned short ao[CAP], ac[CAP], bo[CAP], bcn[CAP];
+  unsigned short col[CAP];
+  unsigned short pc[CAP];             // palette sizes
+  unsigned char done[CAP];            // 0 uncoloured, 1 coloured, 2 won this round
+  unsigned short list[LCAP];
+  unsigned pal[PAL > 0 ? PAL : 1];    // palettes (PAL > 0)
   int nlist, flag, cnt, mx;
-  int sh[COLOR_BLOCK / 64];
+  int sh[NT / 64];
 };
 
 // colouring state in LDS (unit fits) ...
-struct LdsAcc {
-  ColorLds* s;
+template <typename S> struct LdsAcc {
+  S* s;
   __device__ int key(int i) const { return s->key[i]; }
   __device__ int isl(int i) const { return s->isl[i]; }
   __device__ int col(int i) const { return s->col[i]; }
@@ -248,8 +296,9 @@ template <typename R> struct GlbAcc {
   }
 };
 
-__device__ __forceinline__ int pal_select(const unsigned* p, int idx) {
-  for (int w = 0; w < PALW; ++w) {
+// idx-th set bit of a palette of pw words, or -1
+__device__ __forceinline__ int pal_select(const unsigned* p, int pw, int idx) {
+  for (int w = 0; w < pw; ++w) {
     int c = __popc(p[w]);
     if (idx < c) {
       unsigned m = p[w];
@@ -261,30 +310,33 @@ __device__ __forceinline__ int pal_select(const unsigned* p, int idx) {
   return -1;
 }
 
-__device__ __forceinline__ int block_sum(int v, int* sh) {
+template <int NT> __device__ __forceinline__ int block_sum(int v, int* sh) {
+  if (NT == 64) return wave_sum(v);
   int tot;
   block_exclusive_scan(v, sh, tot);
   return tot;
 }
 
-// rounds of blocking.py:78-104 over the K contacts of one unit
-template <typename R, typename A>
-__device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, unsigned long long seed,
+// rounds of blocking.py:78-104 over the K contacts of one unit by NT threads;
+// pal: pw words per contact
+template <int NT, typename R, typename A>
+__device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw, unsigned long long seed,
                              unsigned long long stepk, int* sh, int* s_flag, int* s_mx) {
+  const int pcap = 32 * pw;
   // palettes {0, ..., deg-1}
-  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+  for (int i = threadIdx.x; i < K; i += NT) {
     int deg = 0;
     a.nb(i, [&](int) { ++deg; });
-    unsigned* p = pal + (size_t)i * PALW;
-    for (int w = 0; w < PALW; ++w) {
+    unsigned* p = pal + (size_t)i * pw;
+    for (int w = 0; w < pw; ++w) {
       int lo = w * 32;
       unsigned m = 0;
       if (deg >= lo + 32) m = 0xffffffffu;
       else if (deg > lo) m = (1u << (deg - lo)) - 1u;
       p[w] = m;
     }
-    if (deg > PALMAX) atomicOr(d.err, ERR_COLORS);
-    a.set_pc(i, min(deg, PALMAX));
+    if (deg > pcap) atomicOr(d.err, ERR_COLORS);
+    a.set_pc(i, min(deg, pcap));
     a.set_col(i, 0);
     a.set_done(i, 0);
   }
@@ -292,18 +344,18 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, unsign
   int rnd = 0;
   for (; rnd < MAX_ROUNDS; ++rnd) {
     int left = 0;
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    for (int i = threadIdx.x; i < K; i += NT) {
       if (a.done(i)) continue;
       ++left;
       int n = a.pc(i);
       if (n > 0)
-        a.set_col(i, pal_select(pal + (size_t)i * PALW,
+        a.set_col(i, pal_select(pal + (size_t)i * pw, pw,
                                 (int)color_draw(seed, stepk, (unsigned long long)a.key(i), (unsigned long long)rnd, (unsigned)n)));
     }
-    if (block_sum(left, sh) == 0) break;
+    if (block_sum<NT>(left, sh) == 0) break;
     __syncthreads();
     // keep the draw if no uncoloured neighbour drew the same colour
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    for (int i = threadIdx.x; i < K; i += NT) {
       if (a.done(i)) continue;
       int ci = a.col(i);
       bool ok = true;
@@ -312,23 +364,23 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, unsign
     }
     __syncthreads();
     // winners leave their colour out of the palettes of neighbours uncoloured at round start
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    for (int i = threadIdx.x; i < K; i += NT) {
       if (a.done(i) != 2) continue;
       int ci = a.col(i);
       unsigned bit = 1u << (ci & 31);
       a.nb(i, [&](int j) {
         if (a.done(j) != 1) {
-          unsigned old = atomicAnd(&pal[(size_t)j * PALW + (ci >> 5)], ~bit);
+          unsigned old = atomicAnd(&pal[(size_t)j * pw + (ci >> 5)], ~bit);
           if (old & bit) a.dec_pc(j);
         }
       });
     }
     __syncthreads();
     // commit winners; island maxima over kept and tentative colours
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) d.isl_max[a.isl(i)] = -1;
+    for (int i = threadIdx.x; i < K; i += NT) d.isl_max[a.isl(i)] = -1;
     if (threadIdx.x == 0) *s_flag = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    for (int i = threadIdx.x; i < K; i += NT) {
       if (a.done(i) == 2) a.set_done(i, 1);
       atomicMax(&d.isl_max[a.isl(i)], a.col(i));
       if (!a.done(i) && a.pc(i) == 0) atomicOr(s_flag, 1);
@@ -337,61 +389,58 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, unsign
     if (*s_flag) {
       // rare: empty palettes get max+1, max+2, ... in increasing contact order
       // per island; ranks read only done/pc/isl/key, which stay fixed here
-      for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      for (int i = threadIdx.x; i < K; i += NT) {
         if (a.done(i) || a.pc(i) != 0) continue;
         int il = a.isl(i), ki = a.key(i), rank = 0;
         for (int j = 0; j < K; ++j)
           if (j != i && !a.done(j) && a.pc(j) == 0 && a.isl(j) == il && a.key(j) < ki) ++rank;
         int c = d.isl_max[il] + 1 + rank;
-        if (c >= PALMAX) { atomicOr(d.err, ERR_COLORS); continue; }
-        pal[(size_t)i * PALW + (c >> 5)] |= 1u << (c & 31);
+        if (c >= pcap) { atomicOr(d.err, ERR_COLORS); continue; }
+        pal[(size_t)i * pw + (c >> 5)] |= 1u << (c & 31);
       }
       __syncthreads();
-      for (int i = threadIdx.x; i < K; i += COLOR_BLOCK)
+      for (int i = threadIdx.x; i < K; i += NT)
         if (!a.done(i) && a.pc(i) == 0) a.set_pc(i, 1);
       __syncthreads();
     }
   }
   if (rnd >= MAX_ROUNDS) {
     // leftovers get fresh colours max+1, ... in increasing contact order per island
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) d.isl_max[a.isl(i)] = -1;
+    for (int i = threadIdx.x; i < K; i += NT) d.isl_max[a.isl(i)] = -1;
     __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicMax(&d.isl_max[a.isl(i)], a.col(i));
+    for (int i = threadIdx.x; i < K; i += NT) atomicMax(&d.isl_max[a.isl(i)], a.col(i));
     __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    for (int i = threadIdx.x; i < K; i += NT) {
       if (a.done(i)) continue;
       int il = a.isl(i), ki = a.key(i), rank = 0;
       for (int j = 0; j < K; ++j)
         if (j != i && !a.done(j) && a.isl(j) == il && a.key(j) < ki) ++rank;
       int c = d.isl_max[il] + 1 + rank;
-      if (c >= PALMAX) atomicOr(d.err, ERR_COLORS);
-      a.set_col(i, min(c, PALMAX - 1));
+      if (c >= pcap) atomicOr(d.err, ERR_COLORS);
+      a.set_col(i, min(c, pcap - 1));
     }
     __syncthreads();
   }
   int mx = 0;
-  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) mx = max(mx, a.col(i) + 1);
+  for (int i = threadIdx.x; i < K; i += NT) mx = max(mx, a.col(i) + 1);
   if (threadIdx.x == 0) *s_mx = 0;
   __syncthreads();
   atomicMax(s_mx, mx);
   __syncthreads();
 }
 
-template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color_units(Dev<R> d) {
-  __shared__ ColorLds s;
-  if (aborted(d.err)) return;
-  const int u = blockIdx.x;
-  if (u >= *d.n_units) return;
+// one unit by NT threads with the state S in LDS (global fallback when its
+// body lists do not fit)
+template <typename R, int NT, int CAP, int LCAP, int PAL>
+__device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>& s) {
   const int p0 = d.u_start[u], p1 = d.u_end[u];
-  if (p1 <= p0) return;
   const int bs = d.u_bs[u];
   const int slot = d.bs_slot[bs];
   const SceneParams& sp = d.sp[slot];
   const SceneState& st = d.ss[slot];
   const int K = p1 - p0;
-  unsigned* pal = d.pal + (size_t)p0 * PALW;
   if (sp.scheme == SCHEME_GS) {   // the reference's serial order: contact i is colour i
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    for (int i = threadIdx.x; i < K; i += NT) {
       int k = d.uperm[p0 + i];
       d.c_color[k] = k - st.k_off;
     }
@@ -399,12 +448,12 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color_uni
     return;
   }
   const unsigned long long seed = sp.seed, stepk = (unsigned long long)st.step_index;
-  bool fits = K <= UNIT_CAP;
+  bool fits = K <= CAP;
   if (fits) {
     // stage contacts and the bodies' contact lists (claimed once per body)
     if (threadIdx.x == 0) { s.nlist = 0; s.flag = 0; }
     __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    for (int i = threadIdx.x; i < K; i += NT) {
       int k = d.uperm[p0 + i];
       int A = d.c_ia[k], B = d.c_ib[k];
       int fa = d.b_fixed[A] ? -1 : d.b2f[A];
@@ -417,13 +466,13 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color_uni
       if (fb >= 0) atomicCAS(&d.body_claim[fb], -1, 2 * i + 1);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    for (int i = threadIdx.x; i < K; i += NT) {
       for (int side = 0; side < 2; ++side) {
         int f = side ? s.fb[i] : s.fa[i];
         if (f < 0 || d.body_claim[f] != 2 * i + side) continue;
         int n = d.bc_cnt[f];
         int off = atomicAdd(&s.nlist, n);
-        if (off + n > LIST_CAP) { atomicOr(&s.flag, 1); continue; }
+        if (off + n > LCAP) { atomicOr(&s.flag, 1); continue; }
         for (int e = 0; e < n; ++e) s.list[off + e] = (unsigned short)(d.bc_list[d.bc_off[f] + e] - p0);
         d.body_loc[f] = make_int2(off, n);
       }
@@ -432,96 +481,169 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color_uni
     __syncthreads();
     fits = s.flag == 0;
     if (fits) {
-      for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      for (int i = threadIdx.x; i < K; i += NT) {
         int2 la = s.fa[i] >= 0 ? d.body_loc[s.fa[i]] : make_int2(0, 0);
         int2 lb = s.fb[i] >= 0 ? d.body_loc[s.fb[i]] : make_int2(0, 0);
         s.ao[i] = (unsigned short)la.x; s.ac[i] = (unsigned short)la.y;
         s.bo[i] = (unsigned short)lb.x; s.bcn[i] = (unsigned short)lb.y;
       }
       __syncthreads();
-      LdsAcc a{&s};
-      color_rounds(d, a, K, pal, seed, stepk, s.sh, &s.flag, &s.mx);
-      for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) d.c_color[d.uperm[p0 + i]] = s.col[i];
+      LdsAcc<ColorLds<NT, CAP, LCAP, PAL>> a{&s};
+      const int pw = (2 * K + 2 + 31) / 32;
+      if (PAL > 0 && K * pw <= PAL)
+        color_rounds<NT>(d, a, K, s.pal, pw, seed, stepk, s.sh, &s.flag, &s.mx);
+      else
+        color_rounds<NT>(d, a, K, d.pal + (size_t)p0 * PALW, PALW, seed, stepk, s.sh, &s.flag, &s.mx);
+      for (int i = threadIdx.x; i < K; i += NT) d.c_color[d.uperm[p0 + i]] = s.col[i];
     }
   }
   if (!fits) {
     GlbAcc<R> a{&d, p0, st.k_off, d.c_color};
-    color_rounds(d, a, K, pal, seed, stepk, s.sh, &s.flag, &s.mx);
+    color_rounds<NT>(d, a, K, d.pal + (size_t)p0 * PALW, PALW, seed, stepk, s.sh, &s.flag, &s.mx);
   }
   if (threadIdx.x == 0) {
+    if (s.mx > 2 * K + 1) atomicOr(d.err, ERR_COLORS);   // beyond the lane's colour offset space
     d.u_ncol[u] = s.mx;
     atomicMax(&d.ss[slot].ncolor, s.mx);
   }
 }
 
-// ------------------------------------------------------------ colour order
-// Gather one contact into the solver arrays at position `pos`.
+// units of up to SMALL_UNIT contacts: one wavefront each
+template <typename R> __global__ void __launch_bounds__(64) k_color_small(Dev<R> d) {
+  __shared__ ColorLds<64, SMALL_UNIT, SMALL_LIST, SMALL_PAL> s;
+  if (aborted(d.err)) return;
+  const int u = blockIdx.x;
+  if (u >= *d.n_units) return;
+  const int K = d.u_end[u] - d.u_start[u];
+  if (K <= 0 || K > SMALL_UNIT) return;
+  color_unit<R>(d, u, s);
+}
+
+// larger units: a workgroup each
+template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color_units(Dev<R> d) {
+  __shared__ ColorLds<COLOR_BLOCK, UNIT_CAP, LIST_CAP, 0> s;
+  if (aborted(d.err)) return;
+  const int u = blockIdx.x;
+  if (u >= *d.n_units) return;
+  const int K = d.u_end[u] - d.u_start[u];
+  if (K <= SMALL_UNIT) return;
+  color_unit<R>(d, u, s);
+}
+
+// ------------------------------------------------------------ sweep order
+// Gather one contact into the solver arrays at position `pos`; s_loc holds
+// the bodies' slots (index into u_blist; -1 for a fixed body).
 template <typename R> __device__ __forceinline__ void gather_contact(const Dev<R>& d, int pos, int k) {
-  V4<R> mu = d.c_mu[k];
-  V4<R> n = d.c_n[k], rA = d.c_rA[k], rB = d.c_rB[k];
+  const V4<R> mu = d.c_mu[k];
+  const V4<R> n = d.c_n[k], rA = d.c_rA[k], rB = d.c_rB[k];
+  const int A = d.c_ia[k], B = d.c_ib[k];
+  const bool fa = !d.b_fixed[A], fb = !d.b_fixed[B];
+  int sa = -1, sb = -1;
+  if (fa) { const int f = d.b2f[A]; sa = d.u_boff[d.f_unit[f]] + d.f_loc[f]; }
+  if (fb) { const int f = d.b2f[B]; sb = d.u_boff[d.f_unit[f]] + d.f_loc[f]; }
   d.perm[pos] = k;
-  d.s_id[pos] = make_int2(d.c_ia[k], d.c_ib[k]);
+  d.s_id[pos] = make_int2(A, B);
+  d.s_loc[pos] = make_int2(sa, sb);
   d.s_n[pos] = V4<R>{n.x, n.y, n.z, mu.x};
   d.s_rA[pos] = V4<R>{rA.x, rA.y, rA.z, mu.y};
   d.s_rB[pos] = V4<R>{rB.x, rB.y, rB.z, mu.z};
   d.s_b[pos] = d.c_b[k];
   d.s_r0[pos] = d.c_r0[k];
-  V4<R> z = {R(0), R(0), R(0), R(0)};
+  const V4<R> z = {R(0), R(0), R(0), R(0)};
+  V4<R>* W = d.s_W + 4 * (size_t)pos;
+  W[0] = fa ? d.b_W0[A] : z;
+  W[1] = fa ? d.b_W1[A] : z;
+  W[2] = fb ? d.b_W0[B] : z;
+  W[3] = fb ? d.b_W1[B] : z;
   d.x0[pos] = z;
   d.x1[pos] = z;
 }
 
-// counting sort of `K` contacts (source: idx(i)) by colour into [dst, dst + K)
-template <typename R, typename Idx>
-__device__ void order_by_color(const Dev<R>& d, int K, int nc, int dst, int* coff, Idx idx, int* s_cnt) {
-  for (int c = threadIdx.x; c <= nc; c += COLOR_BLOCK) coff[c] = 0;
-  for (int c = threadIdx.x; c < nc; c += COLOR_BLOCK) s_cnt[c] = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicAdd(&coff[d.c_color[idx(i)] + 1], 1);
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int c = 0; c < nc; ++c) coff[c + 1] += coff[c];
-  __syncthreads();
-  // the order inside a colour does not change any result: LDS atomics place them
-  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
-    int k = idx(i);
-    int c = d.c_color[k];
-    gather_contact(d, dst + coff[c] + atomicAdd(&s_cnt[c], 1), k);
-  }
-  __syncthreads();
+// colour offsets of lane l: 2 K_l + 2 entries (an island's colours stay below
+// twice its size, which the colouring checks)
+template <typename R> __device__ __forceinline__ const int* lane_coff(const Dev<R>& d, int l) {
+  return d.col_off + 2 * ((size_t)d.l_start[l] + l);
 }
 
-// one workgroup per batch scene: scene-order positions [k_off, k_off + K)
-template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order(Dev<R> d) {
-  __shared__ int s_cnt[PALMAX];
-  const int bs = blockIdx.x;
+// One workgroup per sweep lane: its positions [l_start, l_end) in sweep
+// order -- by colour (counting sort; the order inside a colour changes no
+// result), or for the reference's serial Gauss-Seidel by contact index
+// (contacts of different islands commute, so the order inside each island
+// is all that matters, and that is the contact order).
+template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order_lanes(Dev<R> d) {
+  __shared__ int s_cnt[PALMAX + 1];
+  __shared__ int s_nc;
   if (aborted(d.err)) return;
-  const int slot = d.bs_slot[bs];
-  const SceneState& st = d.ss[slot];
-  const int K = st.k_cnt, k0 = st.k_off;
-  if (K == 0) return;
-  int* coff = d.col_off + (size_t)bs * (PALMAX + 1);
-  if (d.sp[slot].scheme == SCHEME_GS) {
-    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) gather_contact(d, k0 + i, k0 + i);
+  const int l = blockIdx.x;
+  if (l >= *d.n_lanes) return;
+  const int p0 = d.l_start[l], p1 = d.l_end[l];
+  if (p1 <= p0) return;
+  const int K = p1 - p0;
+  if (d.sp[d.bs_slot[d.l_bs[l]]].scheme == SCHEME_GS) {
+    for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+      const int k = d.uperm[p0 + i];
+      int rank = 0;
+      for (int j = 0; j < K; ++j) rank += d.uperm[p0 + j] < k ? 1 : 0;
+      gather_contact(d, p0 + rank, k);
+    }
+    if (threadIdx.x == 0) d.l_ncol[l] = 0;
     return;
   }
-  order_by_color(d, K, st.ncolor, k0, coff, [&](int i) { return k0 + i; }, s_cnt);
-}
-
-// one workgroup per unit: unit-order positions [u_start, u_end)
-template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order_units(Dev<R> d) {
-  __shared__ int s_cnt[PALMAX];
-  if (aborted(d.err)) return;
-  const int u = blockIdx.x;
-  if (u >= *d.n_units) return;
-  const int p0 = d.u_start[u], p1 = d.u_end[u];
-  if (p1 <= p0) return;
-  int* coff = d.col_off + (size_t)u * (PALMAX + 1);
-  order_by_color(d, p1 - p0, d.u_ncol[u], p0, coff, [&](int i) { return d.uperm[p0 + i]; }, s_cnt);
+  if (threadIdx.x == 0) s_nc = 0;
+  __syncthreads();
+  int mx = 0;
+  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) mx = max(mx, d.c_color[d.uperm[p0 + i]] + 1);
+  atomicMax(&s_nc, mx);
+  __syncthreads();
+  const int nc = s_nc;
+  int* coff = d.col_off + 2 * ((size_t)p0 + l);
+  for (int c = threadIdx.x; c <= nc; c += COLOR_BLOCK) s_cnt[c] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicAdd(&s_cnt[d.c_color[d.uperm[p0 + i]]], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int c = 0; c < nc; ++c) {
+      const int v = s_cnt[c];
+      s_cnt[c] = acc;
+      coff[c] = acc;
+      acc += v;
+    }
+    coff[nc] = acc;
+    d.l_ncol[l] = nc;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+    const int k = d.uperm[p0 + i];
+    gather_contact(d, p0 + atomicAdd(&s_cnt[d.c_color[k]], 1), k);
+  }
 }
 
 // ------------------------------------------------------------ solve
+// A workgroup solves a group of sweep lanes: the lanes of one batch scene, or
+// a block of the lanes of a single scene spread over a cooperative grid (the
+// scene-wide merit then meets in a grid barrier once per sweep).  A group's
+// bodies are the slot range [b_lo, b_hi) of u_blist.  Three layouts, chosen
+// per group:
+//  * resident: everything a sweep touches -- contact rows, both iterates,
+//    colour offsets, the bodies' velocity changes and inverse masses -- is
+//    staged in LDS once per solve; each wavefront sweeps its own lanes colour
+//    by colour with wavefront-level syncs, and a sweep issues no global
+//    memory access;
+//  * velocity-only: the bodies' velocity changes in LDS, contact rows read
+//    from global memory with the next colour's rows prefetched;
+//  * global: everything in global memory, lanes swept by the whole workgroup.
+// Lanes larger than WAVE_LANE_MAX contacts are swept by the whole workgroup.
+// The sweep order inside an island is the same in every layout and schedule,
+// so all of them give bitwise the same iterates.
 constexpr int SOLVE_BLOCK = 256;
+constexpr int WAVE_LANE_MAX = 512;
+constexpr int SOLVE_LDS_BYTES = 65536 - 512;
+
+// bodies whose velocity changes fit the velocity-only layout
+template <typename R> __host__ __device__ constexpr int lds_body_cap() {
+  return SOLVE_LDS_BYTES / (4 * (int)sizeof(V4<R>));
+}
 
 template <typename R> struct ContactRow {
   V3<R> lin[4], angA[4], angB[4];
@@ -561,24 +683,24 @@ __device__ __forceinline__ V4<R> contact_prox(const SceneParams& sp, const Conta
   return V4<R>{xn, os, ot, orr};
 }
 
-// w_body += W J_body^T dx (contacts.py:107)
+// w += W J^T dx for one body (contacts.py:107); W0 = (I^-1 xx, yy, zz, xy),
+// W1 = (I^-1 xz, yz, 1/m, 0); l, g: linear and angular parts of w
 template <typename R>
-__device__ __forceinline__ void apply_dw(const Dev<R>& d, int body, R sign, const ContactRow<R>& J, const V3<R>* ang,
-                                         V4<R> dx) {
-  R im, W[6];
-  load_W(d, body, im, W);
+__device__ __forceinline__ void dw_body(V4<R> W0, V4<R> W1, R sign, const ContactRow<R>& J, const V3<R>* ang,
+                                        V4<R> dx, V4<R>& l, V4<R>& g) {
+  const R im = W1.z;
+  const R W[6] = {W0.x, W0.y, W0.z, W0.w, W1.x, W1.y};
   R dl[3] = {R(0), R(0), R(0)};
   V3<R> a = {R(0), R(0), R(0)};
-  R dd[4] = {dx.x, dx.y, dx.z, dx.w};
+  const R dd[4] = {dx.x, dx.y, dx.z, dx.w};
   for (int i = 0; i < 4; ++i) {
-    R s = sign * dd[i];
+    const R s = sign * dd[i];
     dl[0] += J.lin[i].x * s; dl[1] += J.lin[i].y * s; dl[2] += J.lin[i].z * s;
     a = a + s * ang[i];
   }
-  V3<R> da = sym_mul(W, a);
-  V4<R> l = d.ww0[body], g = d.ww1[body];
-  d.ww0[body] = V4<R>{l.x + im * dl[0], l.y + im * dl[1], l.z + im * dl[2], R(0)};
-  d.ww1[body] = V4<R>{g.x + da.x, g.y + da.y, g.z + da.z, R(0)};
+  const V3<R> da = sym_mul(W, a);
+  l = V4<R>{l.x + im * dl[0], l.y + im * dl[1], l.z + im * dl[2], R(0)};
+  g = V4<R>{g.x + da.x, g.y + da.y, g.z + da.z, R(0)};
 }
 
 __device__ __forceinline__ double absmax4(double a, double b, double c, double e) {
@@ -595,111 +717,6 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
   for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sh[i]);
   __syncthreads();
   return m;
-}
-
-// One sweep over the contacts at positions [p0, p0 + K) sorted by colour
-// (coff: colour offsets, or colour i = position i in serial mode).  Returns
-// this thread's max |x_new - x_old|.
-template <typename R>
-__device__ double sweep_range(const Dev<R>& d, const SceneParams& sp, int p0, int K, int nc, const int* coff,
-                              bool serial, const V4<R>* xa, V4<R>* xn, R rs) {
-  const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
-  double mloc = 0.0;
-  if (!jacobi) {
-    for (int c = 0; c < nc; ++c) {
-      const int c_lo = serial ? c : coff[c], c_hi = serial ? c + 1 : coff[c + 1];
-      for (int pos = c_lo + (int)threadIdx.x; pos < c_hi; pos += SOLVE_BLOCK) {
-        int kk = p0 + pos;
-        int2 id = d.s_id[kk];
-        V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
-        ContactRow<R> J;
-        contact_rows(n4, rA4, rB4, J);
-        V4<R> r0 = d.s_r0[kk];
-        V4<R> rr = {r0.x * rs, r0.y * rs, r0.z * rs, r0.w * rs};
-        V4<R> xo = xa[kk];
-        V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
-        V4<R> x = contact_prox(sp, J, xo, d.s_b[kk], rr, mu, xyz(d.ww0[id.x]), xyz(d.ww1[id.x]), xyz(d.ww0[id.y]),
-                               xyz(d.ww1[id.y]), false);
-        xn[kk] = x;
-        V4<R> dx = {x.x - xo.x, x.y - xo.y, x.z - xo.z, x.w - xo.w};
-        mloc = fmax(mloc, absmax4((double)x.x - (double)xo.x, (double)x.y - (double)xo.y,
-                                  (double)x.z - (double)xo.z, (double)x.w - (double)xo.w));
-        if (!d.b_fixed[id.x]) apply_dw(d, id.x, R(-1), J, J.angA, dx);
-        if (!d.b_fixed[id.y]) apply_dw(d, id.y, R(1), J, J.angB, dx);
-      }
-      __syncthreads();
-    }
-  } else {
-    // jacobi_solver.py:6-57: every contact from the accepted w; then the
-    // velocity change is applied colour by colour (deterministic sums)
-    const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
-    for (int pos = threadIdx.x; pos < K; pos += SOLVE_BLOCK) {
-      int kk = p0 + pos;
-      int2 id = d.s_id[kk];
-      V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
-      ContactRow<R> J;
-      contact_rows(n4, rA4, rB4, J);
-      V4<R> r0 = d.s_r0[kk];
-      V4<R> rr = {r0.x * rs, r0.y * rs, r0.z * rs, r0.w * rs};
-      V4<R> xo = xa[kk];
-      V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
-      V4<R> x = contact_prox(sp, J, xo, d.s_b[kk], rr, mu, xyz(d.ws0[id.x]), xyz(d.ws1[id.x]), xyz(d.ws0[id.y]),
-                             xyz(d.ws1[id.y]), !hybrid);
-      xn[kk] = x;
-      mloc = fmax(mloc, absmax4((double)x.x - (double)xo.x, (double)x.y - (double)xo.y, (double)x.z - (double)xo.z,
-                                (double)x.w - (double)xo.w));
-    }
-    __syncthreads();
-    for (int c = 0; c < nc; ++c) {
-      const int c_lo = serial ? c : coff[c], c_hi = serial ? c + 1 : coff[c + 1];
-      for (int pos = c_lo + (int)threadIdx.x; pos < c_hi; pos += SOLVE_BLOCK) {
-        int kk = p0 + pos;
-        int2 id = d.s_id[kk];
-        ContactRow<R> J;
-        contact_rows(d.s_n[kk], d.s_rA[kk], d.s_rB[kk], J);
-        V4<R> x = xn[kk], xo = xa[kk];
-        V4<R> dx = {x.x - xo.x, x.y - xo.y, x.z - xo.z, x.w - xo.w};
-        if (!d.b_fixed[id.x]) apply_dw(d, id.x, R(-1), J, J.angA, dx);
-        if (!d.b_fixed[id.y]) apply_dw(d, id.y, R(1), J, J.angB, dx);
-      }
-      __syncthreads();
-    }
-  }
-  return mloc;
-}
-
-// accepted sweep: w_sol <- w_work, else w_work <- w_sol, for the bodies of [p0, p0 + K)
-template <typename R> __device__ void settle_w(const Dev<R>& d, int p0, int K, bool accept) {
-  for (int pos = threadIdx.x; pos < K; pos += SOLVE_BLOCK) {
-    int2 id = d.s_id[p0 + pos];
-    for (int side = 0; side < 2; ++side) {
-      int b = side ? id.y : id.x;
-      if (d.b_fixed[b]) continue;
-      if (accept) { d.ws0[b] = d.ww0[b]; d.ws1[b] = d.ww1[b]; }
-      else { d.ww0[b] = d.ws0[b]; d.ww1[b] = d.ws1[b]; }
-    }
-  }
-}
-
-// impulses back to contact order; natural-map residual parts (FP64)
-template <typename R>
-__device__ void finish_range(const Dev<R>& d, const SceneParams& sp, int p0, int K, const V4<R>* xs, double& rloc,
-                             double& xmax) {
-  for (int pos = threadIdx.x; pos < K; pos += SOLVE_BLOCK) {
-    int kk = p0 + pos;
-    V4<R> x = xs[kk];
-    d.c_imp[d.perm[kk]] = x;
-    int2 id = d.s_id[kk];
-    V4<R> n4 = d.s_n[kk], rA4 = d.s_rA[kk], rB4 = d.s_rB[kk];
-    ContactRow<R> J;
-    contact_rows(n4, rA4, rB4, J);
-    V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
-    V4<R> y = contact_prox(sp, J, x, d.s_b[kk], d.s_r0[kk], mu, xyz(d.ws0[id.x]), xyz(d.ws1[id.x]), xyz(d.ws0[id.y]),
-                           xyz(d.ws1[id.y]), false);
-    rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
-                              (double)x.w - (double)y.w));
-    xmax = fmax(xmax, absmax4(x.x, x.y, x.z, x.w));
-  }
 }
 
 // gauss_seidel_solver.py:56-97: accept/reject, r-factor, stopping tests.
@@ -738,47 +755,6 @@ template <typename R> __device__ void write_solve_state(const Dev<R>& d, SceneSt
   st.final_merit = c.last_merit;
 }
 
-// One workgroup per batch scene (scene-order positions).
-template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve(Dev<R> d) {
-  __shared__ double sh_red[SOLVE_BLOCK / 64];
-  const int bs = blockIdx.x;
-  if (aborted(d.err)) return;
-  const int slot = d.bs_slot[bs];
-  const SceneParams& sp = d.sp[slot];
-  SceneState& st = d.ss[slot];
-  const int K = st.k_cnt, k0 = st.k_off;
-  double* res = d.residuals + sp.res_off;
-  for (int i = threadIdx.x; i < sp.max_iter; i += SOLVE_BLOCK) res[i] = INFINITY;
-  if (K == 0) {
-    if (threadIdx.x == 0) { st.solved = 0; st.iterations = 0; st.exitcode = 0; st.sweeps = 0; st.rejects = 0;
-                            st.final_merit = 0.0; }
-    return;
-  }
-  const bool serial = sp.scheme == SCHEME_GS;
-  const int nc = st.ncolor;
-  const int* coff = d.col_off + (size_t)bs * (PALMAX + 1);
-  SolveCtl ctl;
-  ctl.iters = sp.max_iter;
-  for (int it = 0; it < sp.max_iter; ++it) {
-    const V4<R>* xa = ctl.acc ? d.x1 : d.x0;
-    V4<R>* xn = ctl.acc ? d.x0 : d.x1;
-    double mloc = sweep_range(d, sp, k0, K, nc, coff, serial, xa, xn, R(ctl.rscale));
-    double merit = block_max(mloc, sh_red);
-    int dec = ctl.decide(sp, merit, threadIdx.x == 0 ? res : nullptr, it);
-    settle_w(d, k0, K, dec & 1);
-    __syncthreads();
-    if (dec & 2) { ctl.iters = it; break; }
-  }
-  double rloc = 0.0, xmax = 0.0;
-  finish_range(d, sp, k0, K, ctl.acc ? d.x1 : d.x0, rloc, xmax);
-  double rm = block_max(rloc, sh_red), xm = block_max(xmax, sh_red);
-  if (threadIdx.x == 0) {
-    write_solve_state(d, st, ctl);
-    st.nat_r = rm;
-    st.nat_x = xm;
-  }
-}
-
 // grid-wide barrier for a cooperative launch; a bounded wait that flags an
 // error instead of spinning forever
 __device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, int* err, int* s_ok) {
@@ -805,62 +781,502 @@ __device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, in
   return *s_ok != 0;
 }
 
-// A single scene split over units: each workgroup sweeps its units, the
-// scene-wide merit meets in a grid barrier once per sweep.
+// Working (wl, wa) and accepted (sl, sa) velocity changes of a group's
+// bodies: LDS (slot - base) or the arena arrays (by body).
+template <typename R, bool LDS> struct VelStore {
+  V4<R>* wl; V4<R>* wa; V4<R>* sl; V4<R>* sa;
+  int base;
+  __device__ __forceinline__ int at(int slot, int body) const { return LDS ? slot - base : body; }
+};
+
+// the constant rows of one contact and its previous iterate
+template <typename R> struct Item {
+  int2 loc, id;
+  V4<R> n, rA, rB, b, r0, xo, W[4];
+};
+
+template <typename R> __device__ __forceinline__ void load_item(const Dev<R>& d, const V4<R>* xa, int kk, Item<R>& it) {
+  it.loc = d.s_loc[kk];
+  it.id = d.s_id[kk];
+  it.n = d.s_n[kk];
+  it.rA = d.s_rA[kk];
+  it.rB = d.s_rB[kk];
+  it.b = d.s_b[kk];
+  it.r0 = d.s_r0[kk];
+  it.xo = xa[kk];
+  const V4<R>* W = d.s_W + 4 * (size_t)kk;
+  it.W[0] = W[0]; it.W[1] = W[1]; it.W[2] = W[2]; it.W[3] = W[3];
+}
+
+// One contact (contacts.py:77-107).  Gauss-Seidel reads and updates the
+// working velocity changes; Jacobi (jacobi_solver.py:6-57) reads the
+// accepted ones and adds its change to the working ones.  Returns
+// max |x_new - x_old| in FP64.
+template <typename R, bool LDS>
+__device__ __forceinline__ double update_contact(const SceneParams& sp, const Item<R>& it, VelStore<R, LDS>& w,
+                                                 V4<R>* xn, int kk, R rs, bool jacobi, bool hybrid) {
+  ContactRow<R> J;
+  contact_rows(it.n, it.rA, it.rB, J);
+  const V4<R> rr = {it.r0.x * rs, it.r0.y * rs, it.r0.z * rs, it.r0.w * rs};
+  const V4<R> mu = {it.n.w, it.rA.w, it.rB.w, R(0)};
+  const bool fa = it.loc.x >= 0, fb = it.loc.y >= 0;
+  const int sa = w.at(it.loc.x, it.id.x), sb = w.at(it.loc.y, it.id.y);
+  const V4<R> z = {R(0), R(0), R(0), R(0)};
+  const V4<R>* rl = jacobi ? w.sl : w.wl;
+  const V4<R>* ra = jacobi ? w.sa : w.wa;
+  const V4<R> lA = fa ? rl[sa] : z, aA = fa ? ra[sa] : z;
+  const V4<R> lB = fb ? rl[sb] : z, aB = fb ? ra[sb] : z;
+  const V4<R> x = contact_prox(sp, J, it.xo, it.b, rr, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), jacobi && !hybrid);
+  xn[kk] = x;
+  const V4<R> dx = {x.x - it.xo.x, x.y - it.xo.y, x.z - it.xo.z, x.w - it.xo.w};
+  if (fa) {
+    V4<R> l = jacobi ? w.wl[sa] : lA, g = jacobi ? w.wa[sa] : aA;
+    dw_body(it.W[0], it.W[1], R(-1), J, J.angA, dx, l, g);
+    w.wl[sa] = l;
+    w.wa[sa] = g;
+  }
+  if (fb) {
+    V4<R> l = jacobi ? w.wl[sb] : lB, g = jacobi ? w.wa[sb] : aB;
+    dw_body(it.W[2], it.W[3], R(1), J, J.angB, dx, l, g);
+    w.wl[sb] = l;
+    w.wa[sb] = g;
+  }
+  return absmax4((double)x.x - (double)it.xo.x, (double)x.y - (double)it.xo.y, (double)x.z - (double)it.xo.z,
+                 (double)x.w - (double)it.xo.w);
+}
+
+template <int STRIDE> __device__ __forceinline__ void phase_sync() {
+  if (STRIDE == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// One sweep of one lane at positions p0 + [0, K) sorted by colour: STRIDE
+// lanes (a wavefront or the workgroup) take the contacts of a colour, in
+// passes of STRIDE; colours are separated by a sync.  coff: colour offsets
+// (serial: colour c is position c).  Returns this thread's max |dx|.
+template <typename R, bool LDS, int STRIDE>
+__device__ double sweep_lane(const Dev<R>& d, const SceneParams& sp, VelStore<R, LDS>& w, int p0, int nc,
+                             const int* coff, bool serial, bool jacobi, bool hybrid, const V4<R>* xa, V4<R>* xn,
+                             R rs, int lane) {
+  double m = 0.0;
+  if (nc <= 0) return m;
+  // pass state: colour c, first position pb of the pass, end hi of colour c, end hn of colour c + 1
+  int c = 0;
+  int pb = serial ? 0 : coff[0];
+  int hi = serial ? 1 : coff[1];
+  int hn = nc > 1 ? (serial ? 2 : coff[2]) : hi;
+  // Two item buffers used in turn (no register copies, so the prefetch of
+  // one pass stays in flight while the other is computed).
+  Item<R> ia, ib;
+  bool ha = pb + lane < hi;
+  if (ha) load_item(d, xa, p0 + pb + lane, ia);
+  auto step = [&](Item<R>& use, bool have, Item<R>& fill, bool& fhave) {
+    const bool same = pb + STRIDE < hi;
+    int c1, pb1, hi1, hn1;
+    if (same) {
+      c1 = c; pb1 = pb + STRIDE; hi1 = hi; hn1 = hn;
+    } else {
+      c1 = c + 1; pb1 = hi; hi1 = hn;
+      hn1 = c1 + 1 < nc ? (serial ? c1 + 2 : coff[c1 + 2]) : hi1;
+    }
+    fhave = c1 < nc && pb1 + lane < hi1;
+    if (fhave) load_item(d, xa, p0 + pb1 + lane, fill);
+    if (have) m = fmax(m, update_contact(sp, use, w, xn, p0 + pb + lane, rs, jacobi, hybrid));
+    if (!same) phase_sync<STRIDE>();
+    c = c1; pb = pb1; hi = hi1; hn = hn1;
+  };
+  bool hb = false;
+  while (true) {
+    step(ia, ha, ib, hb);
+    if (c >= nc) break;
+    step(ib, hb, ia, ha);
+    if (c >= nc) break;
+  }
+  return m;
+}
+
+// The lanes and bodies of one solve group.
+struct Group {
+  int l_lo, l_hi;     // lanes
+  int b_lo, b_hi;     // body slots
+  bool lead;          // writes the scene's solver state
+};
+
+template <typename R, bool LDS, bool COOP>
+__device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4<R>* smem, double* sh_red,
+                            int* s_ok) {
+  const SceneParams& sp = d.sp[slot];
+  SceneState& st = d.ss[slot];
+  const int b0 = gr.b_lo, nbody = gr.b_hi - gr.b_lo;
+  constexpr int CAP = lds_body_cap<R>();
+  VelStore<R, LDS> w;
+  if (LDS) { w.wl = smem; w.wa = smem + CAP; w.sl = smem + 2 * CAP; w.sa = smem + 3 * CAP; }
+  else { w.wl = d.ww0; w.wa = d.ww1; w.sl = d.ws0; w.sa = d.ws1; }
+  w.base = b0;
+  if (LDS) {
+    for (int i = threadIdx.x; i < nbody; i += blockDim.x) {
+      const int b = d.u_blist[b0 + i];
+      w.wl[i] = d.ww0[b]; w.wa[i] = d.ww1[b]; w.sl[i] = d.ws0[b]; w.sa[i] = d.ws1[b];
+    }
+    __syncthreads();
+  }
+  const bool serial = sp.scheme == SCHEME_GS;
+  const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
+  const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+  double* res = d.residuals + sp.res_off;
+  SolveCtl ctl;
+  ctl.iters = sp.max_iter;
+  for (int it = 0; it < sp.max_iter; ++it) {
+    const V4<R>* xa = ctl.acc ? d.x1 : d.x0;
+    V4<R>* xn = ctl.acc ? d.x0 : d.x1;
+    const R rs = R(ctl.rscale);
+    double m = 0.0;
+    if (LDS) {
+      for (int l = gr.l_lo + wave; l < gr.l_hi; l += nwave) {
+        const int p0 = d.l_start[l], K = d.l_end[l] - p0;
+        if (K <= 0 || (K > WAVE_LANE_MAX && !serial)) continue;
+        m = fmax(m, sweep_lane<R, LDS, 64>(d, sp, w, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial, jacobi,
+                                           hybrid, xa, xn, rs, lane));
+      }
+    }
+    for (int l = gr.l_lo; l < gr.l_hi; ++l) {
+      const int p0 = d.l_start[l], K = d.l_end[l] - p0;
+      if (K <= 0 || (LDS && (K <= WAVE_LANE_MAX || serial))) continue;
+      m = fmax(m, sweep_lane<R, LDS, SOLVE_BLOCK>(d, sp, w, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
+                                                  jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
+    }
+    double merit = block_max(m, sh_red);
+    if (COOP) {
+      if (threadIdx.x == 0 && merit > 0.0) atomic_max_nonneg(&d.merit[it], merit);
+      if (!grid_barrier(d.bar, gridDim.x, d.err, s_ok)) return;
+      merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0) ? res : nullptr, it);
+    // accepted sweep: ws <- ww, else ww <- ws
+    for (int i = threadIdx.x; i < nbody; i += blockDim.x) {
+      const int s = LDS ? i : d.u_blist[b0 + i];
+      if (dec & 1) { w.sl[s] = w.wl[s]; w.sa[s] = w.wa[s]; }
+      else { w.wl[s] = w.sl[s]; w.wa[s] = w.sa[s]; }
+    }
+    __syncthreads();
+    if (dec & 2) { ctl.iters = it; break; }
+  }
+  // impulses back to contact order; natural-map residual (FP64) with r0
+  const V4<R>* xs = ctl.acc ? d.x1 : d.x0;
+  double rloc = 0.0, xmax = 0.0;
+  const V4<R> z = {R(0), R(0), R(0), R(0)};
+  for (int l = gr.l_lo; l < gr.l_hi; ++l) {
+    const int p0 = d.l_start[l], K = d.l_end[l] - p0;
+    if (K <= 0) continue;
+    for (int pos = threadIdx.x; pos < K; pos += blockDim.x) {
+      const int kk = p0 + pos;
+      Item<R> c;
+      load_item(d, xs, kk, c);
+      const V4<R> x = c.xo;
+      d.c_imp[d.perm[kk]] = x;
+      ContactRow<R> J;
+      contact_rows(c.n, c.rA, c.rB, J);
+      const V4<R> mu = {c.n.w, c.rA.w, c.rB.w, R(0)};
+      const bool fa = c.loc.x >= 0, fb = c.loc.y >= 0;
+      const int sa = w.at(c.loc.x, c.id.x), sb = w.at(c.loc.y, c.id.y);
+      const V4<R> lA = fa ? w.sl[sa] : z, aA = fa ? w.sa[sa] : z;
+      const V4<R> lB = fb ? w.sl[sb] : z, aB = fb ? w.sa[sb] : z;
+      const V4<R> y = contact_prox(sp, J, x, c.b, c.r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
+      rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
+                                (double)x.w - (double)y.w));
+      xmax = fmax(xmax, absmax4(x.x, x.y, x.z, x.w));
+    }
+  }
+  if (LDS)
+    for (int i = threadIdx.x; i < nbody; i += blockDim.x) {
+      const int b = d.u_blist[b0 + i];
+      d.ws0[b] = w.sl[i];
+      d.ws1[b] = w.sa[i];
+    }
+  const double rm = block_max(rloc, sh_red), xm = block_max(xmax, sh_red);
+  if (threadIdx.x == 0) {
+    if (COOP) {
+      if (rm > 0.0) atomic_max_nonneg(&st.nat_r, rm);
+      if (xm > 0.0) atomic_max_nonneg(&st.nat_x, xm);
+    } else {
+      st.nat_r = rm;
+      st.nat_x = xm;
+    }
+    if (gr.lead) write_solve_state(d, st, ctl);
+  }
+}
+
+// ---- resident layout
+template <typename R> __host__ __device__ constexpr size_t res_lds_bytes(int cc, int cb, int co, int cl) {
+  return (size_t)cc * (7 * sizeof(V4<R>) + sizeof(int2)) + (size_t)cb * 6 * sizeof(V4<R>) + (size_t)co * 4 +
+         (size_t)cl * 16;
+}
+
+constexpr int RES_LANES = 64;   // lanes of a resident group
+
+template <typename R> struct ResLds {
+  V4<R> *n, *rA, *rB, *b, *r0, *x0, *x1;   // per contact (group order)
+  V4<R> *wl, *wa, *sl, *sa, *W0, *W1;      // per body (slot - b_lo)
+  int2* loc;                               // body of a contact's two bodies (slot - b_lo; -1: fixed)
+  int4* lt;                                // per lane: first contact, K, colours, first colour offset
+  int* coff;
+  __device__ ResLds(unsigned char* p, int cc, int cb) {
+    V4<R>* v = reinterpret_cast<V4<R>*>(p);
+    n = v; rA = n + cc; rB = rA + cc; b = rB + cc; r0 = b + cc; x0 = r0 + cc; x1 = x0 + cc;
+    wl = x1 + cc; wa = wl + cb; sl = wa + cb; sa = sl + cb; W0 = sa + cb; W1 = W0 + cb;
+    loc = reinterpret_cast<int2*>(W1 + cb);
+    lt = reinterpret_cast<int4*>(loc + cc);
+    coff = reinterpret_cast<int*>(lt + RES_LANES);
+  }
+};
+
+// sizes of a group's resident data (uniform over the workgroup)
+struct GroupSize { int p0, nc, no, nl; };
+
+template <typename R> __device__ GroupSize group_size(const Dev<R>& d, const Group& gr, bool serial) {
+  GroupSize g{0x7fffffff, 0, 0, 0};
+  int p1 = 0;
+  for (int l = gr.l_lo; l < gr.l_hi; ++l) {
+    const int a = d.l_start[l], e = d.l_end[l];
+    if (e <= a) continue;
+    g.p0 = min(g.p0, a);
+    p1 = max(p1, e);
+    g.no += serial ? 0 : d.l_ncol[l] + 1;
+    ++g.nl;
+  }
+  g.nc = g.nl ? p1 - g.p0 : 0;
+  if (!g.nl) g.p0 = 0;
+  return g;
+}
+
+template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d, const Group& gr, const GroupSize& g) {
+  return g.nc <= d.res_cap_c && gr.b_hi - gr.b_lo <= d.res_cap_b && g.no <= d.res_cap_o && g.nl <= RES_LANES;
+}
+
+// one contact of a resident group (the arithmetic of update_contact)
+template <typename R>
+__device__ __forceinline__ double res_update(const SceneParams& sp, ResLds<R>& L, const V4<R>* rl, const V4<R>* ra,
+                                             const V4<R>* xa, V4<R>* xn, int i, R rs, bool jacobi, bool hybrid) {
+  const V4<R> n4 = L.n[i], rA4 = L.rA[i], rB4 = L.rB[i], bb = L.b[i], r0 = L.r0[i], xo = xa[i];
+  const int2 lc = L.loc[i];
+  ContactRow<R> J;
+  contact_rows(n4, rA4, rB4, J);
+  const V4<R> rr = {r0.x * rs, r0.y * rs, r0.z * rs, r0.w * rs};
+  const V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
+  const V4<R> z = {R(0), R(0), R(0), R(0)};
+  const bool fa = lc.x >= 0, fb = lc.y >= 0;
+  const V4<R> lA = fa ? rl[lc.x] : z, aA = fa ? ra[lc.x] : z;
+  const V4<R> lB = fb ? rl[lc.y] : z, aB = fb ? ra[lc.y] : z;
+  const V4<R> x = contact_prox(sp, J, xo, bb, rr, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), jacobi && !hybrid);
+  xn[i] = x;
+  const V4<R> dx = {x.x - xo.x, x.y - xo.y, x.z - xo.z, x.w - xo.w};
+  if (fa) {
+    V4<R> l = jacobi ? L.wl[lc.x] : lA, g = jacobi ? L.wa[lc.x] : aA;
+    dw_body(L.W0[lc.x], L.W1[lc.x], R(-1), J, J.angA, dx, l, g);
+    L.wl[lc.x] = l;
+    L.wa[lc.x] = g;
+  }
+  if (fb) {
+    V4<R> l = jacobi ? L.wl[lc.y] : lB, g = jacobi ? L.wa[lc.y] : aB;
+    dw_body(L.W0[lc.y], L.W1[lc.y], R(1), J, J.angB, dx, l, g);
+    L.wl[lc.y] = l;
+    L.wa[lc.y] = g;
+  }
+  return absmax4((double)x.x - (double)xo.x, (double)x.y - (double)xo.y, (double)x.z - (double)xo.z,
+                 (double)x.w - (double)xo.w);
+}
+
+// one sweep of one resident lane by STRIDE lanes (a wavefront or the workgroup)
+template <typename R, int STRIDE>
+__device__ double res_sweep(const SceneParams& sp, ResLds<R>& L, int4 lt, bool serial, bool jacobi, bool hybrid,
+                            const V4<R>* xa, V4<R>* xn, R rs, int lane) {
+  const V4<R>* rl = jacobi ? L.sl : L.wl;
+  const V4<R>* ra = jacobi ? L.sa : L.wa;
+  const int c0 = lt.x, nc = serial ? lt.y : lt.z;
+  const int* co = L.coff + lt.w;
+  double m = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    const int lo = serial ? c : co[c], hi = serial ? c + 1 : co[c + 1];
+    for (int pos = lo + lane; pos < hi; pos += STRIDE)
+      m = fmax(m, res_update(sp, L, rl, ra, xa, xn, c0 + pos, rs, jacobi, hybrid));
+    phase_sync<STRIDE>();
+  }
+  return m;
+}
+
+template <typename R, bool COOP>
+__device__ void solve_group_resident(const Dev<R>& d, const int slot, const Group& gr, const GroupSize& gs,
+                                     unsigned char* smem, double* sh_red, int* s_ok) {
+  const SceneParams& sp = d.sp[slot];
+  SceneState& st = d.ss[slot];
+  const bool serial = sp.scheme == SCHEME_GS;
+  const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
+  const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
+  ResLds<R> L(smem, d.res_cap_c, d.res_cap_b);
+  const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
+  // stage: lane table, colour offsets, contacts, bodies
+  if (threadIdx.x == 0) {
+    int t = 0, o = 0;
+    for (int l = gr.l_lo; l < gr.l_hi; ++l) {
+      const int a = d.l_start[l], e = d.l_end[l];
+      if (e <= a) continue;
+      const int nc = serial ? 0 : d.l_ncol[l];
+      L.lt[t++] = make_int4(a - gs.p0, e - a, nc, o);
+      o += serial ? 0 : nc + 1;
+    }
+  }
+  __syncthreads();
+  if (!serial) {
+    int t = 0;
+    for (int l = gr.l_lo; l < gr.l_hi; ++l) {
+      if (d.l_end[l] <= d.l_start[l]) continue;
+      const int4 lt = L.lt[t++];
+      const int* src = lane_coff(d, l);
+      for (int c = threadIdx.x; c <= lt.z; c += blockDim.x) L.coff[lt.w + c] = src[c];
+    }
+  }
+  for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
+    const int pos = gs.p0 + i;
+    L.n[i] = d.s_n[pos]; L.rA[i] = d.s_rA[pos]; L.rB[i] = d.s_rB[pos]; L.b[i] = d.s_b[pos]; L.r0[i] = d.s_r0[pos];
+    L.x0[i] = d.x0[pos]; L.x1[i] = d.x1[pos];
+    const int2 lc = d.s_loc[pos];
+    L.loc[i] = make_int2(lc.x >= 0 ? lc.x - b0 : -1, lc.y >= 0 ? lc.y - b0 : -1);
+  }
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const int b = d.u_blist[b0 + i];
+    L.wl[i] = d.ww0[b]; L.wa[i] = d.ww1[b]; L.sl[i] = d.ws0[b]; L.sa[i] = d.ws1[b];
+    L.W0[i] = d.b_W0[b]; L.W1[i] = d.b_W1[b];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+  double* res = d.residuals + sp.res_off;
+  SolveCtl ctl;
+  ctl.iters = sp.max_iter;
+  for (int it = 0; it < sp.max_iter; ++it) {
+    const V4<R>* xa = ctl.acc ? L.x1 : L.x0;
+    V4<R>* xn = ctl.acc ? L.x0 : L.x1;
+    const R rs = R(ctl.rscale);
+    double m = 0.0;
+    for (int t = wave; t < gs.nl; t += nwave) {
+      const int4 lt = L.lt[t];
+      if (lt.y > WAVE_LANE_MAX && !serial) continue;
+      m = fmax(m, res_sweep<R, 64>(sp, L, lt, serial, jacobi, hybrid, xa, xn, rs, lane));
+    }
+    for (int t = 0; t < gs.nl; ++t) {
+      const int4 lt = L.lt[t];
+      if (lt.y <= WAVE_LANE_MAX || serial) continue;
+      m = fmax(m, res_sweep<R, SOLVE_BLOCK>(sp, L, lt, serial, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
+    }
+    double merit = block_max(m, sh_red);
+    if (COOP) {
+      if (threadIdx.x == 0 && merit > 0.0) atomic_max_nonneg(&d.merit[it], merit);
+      if (!grid_barrier(d.bar, gridDim.x, d.err, s_ok)) return;
+      merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0) ? res : nullptr, it);
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      if (dec & 1) { L.sl[i] = L.wl[i]; L.sa[i] = L.wa[i]; }
+      else { L.wl[i] = L.sl[i]; L.wa[i] = L.sa[i]; }
+    }
+    __syncthreads();
+    if (dec & 2) { ctl.iters = it; break; }
+  }
+  // impulses to contact order; natural-map residual (FP64) with r0; accepted velocity changes out
+  const V4<R>* xs = ctl.acc ? L.x1 : L.x0;
+  double rloc = 0.0, xmax = 0.0;
+  const V4<R> z = {R(0), R(0), R(0), R(0)};
+  for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
+    const V4<R> x = xs[i], n4 = L.n[i], rA4 = L.rA[i], rB4 = L.rB[i];
+    d.c_imp[d.perm[gs.p0 + i]] = x;
+    ContactRow<R> J;
+    contact_rows(n4, rA4, rB4, J);
+    const V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
+    const int2 lc = L.loc[i];
+    const V4<R> lA = lc.x >= 0 ? L.sl[lc.x] : z, aA = lc.x >= 0 ? L.sa[lc.x] : z;
+    const V4<R> lB = lc.y >= 0 ? L.sl[lc.y] : z, aB = lc.y >= 0 ? L.sa[lc.y] : z;
+    const V4<R> y = contact_prox(sp, J, x, L.b[i], L.r0[i], mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
+    rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
+                              (double)x.w - (double)y.w));
+    xmax = fmax(xmax, absmax4(x.x, x.y, x.z, x.w));
+  }
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const int b = d.u_blist[b0 + i];
+    d.ws0[b] = L.sl[i];
+    d.ws1[b] = L.sa[i];
+  }
+  const double rm = block_max(rloc, sh_red), xm = block_max(xmax, sh_red);
+  if (threadIdx.x == 0) {
+    if (COOP) {
+      if (rm > 0.0) atomic_max_nonneg(&st.nat_r, rm);
+      if (xm > 0.0) atomic_max_nonneg(&st.nat_x, xm);
+    } else {
+      st.nat_r = rm;
+      st.nat_x = xm;
+    }
+    if (gr.lead) write_solve_state(d, st, ctl);
+  }
+}
+
+template <typename R> __device__ __forceinline__ void no_contacts(SceneState& st) {
+  st.solved = 0; st.iterations = 0; st.exitcode = 0; st.sweeps = 0; st.rejects = 0; st.final_merit = 0.0;
+}
+
+template <typename R, bool COOP>
+__device__ __forceinline__ void solve_dispatch(const Dev<R>& d, int slot, const Group& gr, unsigned char* smem,
+                                               double* sh_red, int* s_ok) {
+  const GroupSize gs = group_size(d, gr, d.sp[slot].scheme == SCHEME_GS);
+  V4<R>* vs = reinterpret_cast<V4<R>*>(smem);
+  if (group_fits(d, gr, gs)) solve_group_resident<R, COOP>(d, slot, gr, gs, smem, sh_red, s_ok);
+  else if (gr.b_hi - gr.b_lo <= lds_body_cap<R>()) solve_group<R, true, COOP>(d, slot, gr, vs, sh_red, s_ok);
+  else solve_group<R, false, COOP>(d, slot, gr, vs, sh_red, s_ok);
+}
+
+// One workgroup per batch scene.
+template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve(Dev<R> d) {
+  extern __shared__ __align__(32) unsigned char solve_smem[];
+  __shared__ double sh_red[SOLVE_BLOCK / 64];
+  __shared__ int s_ok;
+  if (aborted(d.err)) return;
+  const int bs = blockIdx.x;
+  const int slot = d.bs_slot[bs];
+  const SceneParams& sp = d.sp[slot];
+  SceneState& st = d.ss[slot];
+  double* res = d.residuals + sp.res_off;
+  for (int i = threadIdx.x; i < sp.max_iter; i += SOLVE_BLOCK) res[i] = INFINITY;
+  if (st.k_cnt == 0) {
+    if (threadIdx.x == 0) no_contacts<R>(st);
+    return;
+  }
+  const int u0 = d.u_base[bs], u1 = u0 + d.u_cnt[bs];
+  const Group gr{d.l_base[bs], d.l_base[bs] + d.l_cnt[bs], d.u_boff[u0], d.u_boff[u1], true};
+  solve_dispatch<R, false>(d, slot, gr, solve_smem, sh_red, &s_ok);
+}
+
+// A single scene (its lanes are its colouring units): the lanes split into
+// gridDim.x contiguous blocks, one per workgroup of a cooperative launch.
 template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve_multi(Dev<R> d) {
+  extern __shared__ __align__(32) unsigned char solve_smem[];
   __shared__ double sh_red[SOLVE_BLOCK / 64];
   __shared__ int s_ok;
   if (aborted(d.err)) return;   // uniform: the flag is set before this launch or not at all
   const int slot = d.bs_slot[0];
   const SceneParams& sp = d.sp[slot];
   SceneState& st = d.ss[slot];
-  const int K = st.k_cnt;
-  const int nu = *d.n_units;
   double* res = d.residuals + sp.res_off;
   if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < sp.max_iter; i += SOLVE_BLOCK) res[i] = INFINITY;
-  if (K == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st.solved = 0; st.iterations = 0; st.exitcode = 0; st.sweeps = 0;
-                                               st.rejects = 0; st.final_merit = 0.0; }
+  if (st.k_cnt == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) no_contacts<R>(st);
     return;
   }
-  const bool serial = sp.scheme == SCHEME_GS;
-  SolveCtl ctl;
-  ctl.iters = sp.max_iter;
-  for (int it = 0; it < sp.max_iter; ++it) {
-    const V4<R>* xa = ctl.acc ? d.x1 : d.x0;
-    V4<R>* xn = ctl.acc ? d.x0 : d.x1;
-    double mloc = 0.0;
-    for (int u = blockIdx.x; u < nu; u += gridDim.x) {
-      int p0 = d.u_start[u], p1 = d.u_end[u];
-      if (p1 <= p0) continue;
-      const int* coff = d.col_off + (size_t)u * (PALMAX + 1);
-      mloc = fmax(mloc, sweep_range(d, sp, p0, p1 - p0, serial ? (p1 - p0) : d.u_ncol[u], coff, serial, xa, xn,
-                                    R(ctl.rscale)));
-    }
-    double m = block_max(mloc, sh_red);
-    if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&d.merit[it], m);
-    if (!grid_barrier(d.bar, gridDim.x, d.err, &s_ok)) return;
-    double merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    int dec = ctl.decide(sp, merit, (blockIdx.x == 0 && threadIdx.x == 0) ? res : nullptr, it);
-    for (int u = blockIdx.x; u < nu; u += gridDim.x) {
-      int p0 = d.u_start[u], p1 = d.u_end[u];
-      if (p1 > p0) settle_w(d, p0, p1 - p0, dec & 1);
-    }
-    __syncthreads();
-    if (dec & 2) { ctl.iters = it; break; }
-  }
-  double rloc = 0.0, xmax = 0.0;
-  for (int u = blockIdx.x; u < nu; u += gridDim.x) {
-    int p0 = d.u_start[u], p1 = d.u_end[u];
-    if (p1 > p0) finish_range(d, sp, p0, p1 - p0, ctl.acc ? d.x1 : d.x0, rloc, xmax);
-  }
-  double rm = block_max(rloc, sh_red), xm = block_max(xmax, sh_red);
-  if (threadIdx.x == 0) {
-    if (rm > 0.0) atomic_max_nonneg(&st.nat_r, rm);
-    if (xm > 0.0) atomic_max_nonneg(&st.nat_x, xm);
-    if (blockIdx.x == 0) write_solve_state(d, st, ctl);
-  }
+  const long long nl = *d.n_lanes, G = gridDim.x;
+  const int l_lo = (int)(blockIdx.x * nl / G), l_hi = (int)((blockIdx.x + 1) * nl / G);
+  const Group gr{l_lo, l_hi, d.u_boff[l_lo], d.u_boff[l_hi], blockIdx.x == 0};
+  solve_dispatch<R, true>(d, slot, gr, solve_smem, sh_red, &s_ok);
 }
-
-

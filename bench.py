@@ -21,6 +21,18 @@ of this same command, read from the committed summary under profiles/.
 cpu_baseline: the FP64 NumPy restatement of the step, single-threaded, timed
 on one step of the same scene from the GPU's end state.
 
+batched (on by default, --no-batched to skip): 1024 independent scenes of
+8..512 boxes and spheres (log-uniform counts, mu ~ U[0.1, 0.8], dt = 0.01,
+30 GS sweeps) advanced together by one hipGraph replay per step; scene-steps/s
+and per-step p50/p99 beside the plain path (one scene per launch) on a sample
+of the same scenes, whose end states must equal the batched ones.
+
+serving (on by default, --no-serving to skip): Poisson arrivals of
+sphere-drop requests (8..1024 bodies, 50..500 steps) admitted against an HBM
+budget and advanced by continuous batching; p50/p99 per-step latency, completed
+scene-steps/s, admission wait, and the alone-vs-mixed difference of a few
+requests.
+
 --copy-calibration BYTES runs only a plain device-to-device copy of BYTES
 bytes (the reference kernel for scaling the profiler's byte counters).
 """
@@ -90,6 +102,108 @@ def cpu_baseline(eng, stepper, steps_done):
                 natural_residual=float(lg.get("natural_residual", 0.0)))
 
 
+def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
+    """1024 mixed box/sphere scenes stepped as one batch vs one scene per launch."""
+    import numpy as np
+    from rainbow_amd import native, scenes
+    from rainbow_amd.stepper import engine_desc, gather_state, shared_context
+    t0 = time.perf_counter()
+    ctx = shared_context(0, "fp32")
+    bank = scenes.ShapeBank(resolution=32)
+    engs = [scenes.mixed_drop(3000 + s, bank) for s in range(n_scenes)]
+    sids, init = [], []
+    for e in engs:
+        desc, keep = engine_desc(e, ctx)
+        sid = ctx.scene_create(desc)
+        st0 = gather_state(list(e.bodies.values()))
+        ctx.set_state(sid, *st0)
+        sids.append(sid)
+        init.append((desc, keep, st0, len(e.bodies)))
+    n_bodies = sum(len(e.bodies) for e in engs)
+    log("batched: %d scenes, %d bodies built in %.1f s" % (n_scenes, n_bodies, time.perf_counter() - t0))
+    sample = list(range(0, n_scenes, max(1, n_scenes // plain_sample)))[:plain_sample]
+    # batch: warm-up steps, then the sample's states for the plain comparison
+    ctx.step(sids, warmup, 0.0, native.STEP_GRAPH)
+    ctx.sync()
+    ref = {i: ctx.get_state(sids[i], init[i][3]) for i in sample}
+    for sid in sids:
+        ctx.scene_totals(sid, clear=True)
+    t1 = time.perf_counter()
+    ctx.step(sids, steps, 0.0, native.STEP_GRAPH)
+    ctx.sync()
+    tb = time.perf_counter() - t1
+    lat = []
+    for _ in range(10):
+        t2 = time.perf_counter()
+        ctx.step(sids, 1, 0.0, native.STEP_GRAPH)
+        ctx.sync()
+        lat.append(time.perf_counter() - t2)
+    citers = sum(ctx.scene_totals(sid)["contact_iters"] for sid in sids)
+    # plain path: the sample scenes again from their initial state, one scene per launch
+    dup = []
+    for i in sample:
+        desc, keep, st0, n = init[i]
+        sid = ctx.scene_create(desc)
+        ctx.set_state(sid, *st0)
+        dup.append(sid)
+    for sid in dup:                      # first launch of each scene (workspace sizing) untimed
+        ctx.step([sid], 1, 0.0, 0)
+    ctx.sync()
+    t3 = time.perf_counter()
+    for _ in range(plain_steps):
+        for sid in dup:
+            ctx.step([sid], 1, 0.0, 0)
+    ctx.sync()
+    tp = time.perf_counter() - t3
+    # equal results: the plain runs took 1 + plain_steps steps; compare at equal step counts
+    diff = 0.0
+    if 1 + plain_steps == warmup:
+        for i, sid in zip(sample, dup):
+            got = ctx.get_state(sid, init[i][3])
+            diff = max(diff, max(float(np.abs(a - b).max()) for a, b in zip(got, ref[i])))
+    else:
+        diff = None
+    for sid in dup:
+        ctx.scene_destroy(sid)
+    for sid in sids:
+        ctx.scene_destroy(sid)
+    batched = n_scenes * steps / tb
+    plain = len(dup) * plain_steps / tp
+    return {"workload": "mixed_drop_%d_scenes" % n_scenes, "bodies": n_bodies, "steps": steps,
+            "scene_steps_per_s": batched, "ms_per_step": 1e3 * tb / steps,
+            "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
+            "contact_iters_per_s": citers / tb if tb > 0 else 0.0,
+            "plain": {"scene_steps_per_s": plain, "sample_scenes": len(dup), "steps": plain_steps,
+                      "ms_per_scene_step": 1e3 * tp / (len(dup) * plain_steps)},
+            "speedup_vs_plain": batched / plain if plain > 0 else None,
+            "max_abs_state_diff_vs_plain": diff}
+
+
+def serving_leg(n_req, rate, budget_gib, alone_checks):
+    """Poisson arrivals served by continuous batching."""
+    import numpy as np
+    from rainbow_amd import scenes, serving
+    t0 = time.perf_counter()
+    bank = scenes.ShapeBank(resolution=32)
+    rng = np.random.default_rng(4)
+    arrivals = serving.poisson_arrivals(rate, n_req, seed=4)
+    srv = serving.Server(budget=budget_gib << 30)
+    reqs = []
+    for i in range(n_req):
+        eng = scenes.serving_request(4000 + i, bank)
+        reqs.append(srv.prepare(serving.Request(i, eng, int(rng.integers(50, 501)), float(arrivals[i]))))
+    log("serving: %d requests (%d bodies) prepared in %.1f s" % (
+        n_req, sum(len(r.engine.bodies) for r in reqs), time.perf_counter() - t0))
+    rep = srv.run(reqs)
+    diffs = []
+    for r in reqs[:alone_checks]:
+        alone = srv.run_alone(r)
+        diffs.append(max(float(np.abs(a - b).max()) for a, b in zip(alone, r.state)))
+    rep.update(rate_per_s=rate, max_abs_diff_alone_vs_mixed=max(diffs) if diffs else None,
+               alone_checked=len(diffs))
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +212,11 @@ def main():
     ap.add_argument("--bodies", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--batched-scenes", type=int, default=1024)
+    ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--serving-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the step of one scene (or of one batch of scenes) is measured on "
@@ -178,6 +297,17 @@ def main():
         log("cpu baseline: %.3g contact-iters/s (%.1f s)" % (cb["value"], cb["seconds"]))
     else:
         result["cpu_baseline"] = None
+    if not args.no_batched:
+        log("batched leg ...")
+        result["batched"] = batched_leg(args.batched_scenes, 20, 3, 64, 2)
+        log("batched: %.0f scene-steps/s (plain %.0f)" % (result["batched"]["scene_steps_per_s"],
+                                                          result["batched"]["plain"]["scene_steps_per_s"]))
+    if not args.no_serving:
+        log("serving leg ...")
+        result["serving"] = serving_leg(args.serving_requests, args.serving_rate, 64, 3)
+        log("serving: %.0f scene-steps/s, p50 %.2f ms, p99 %.2f ms" % (
+            result["serving"]["scene_steps_per_s"], result["serving"]["p50_step_ms"],
+            result["serving"]["p99_step_ms"]))
     print(json.dumps(result), flush=True)
 
 

@@ -108,6 +108,16 @@ typedef struct rbh_step_log {
   double final_merit;          /* last accepted merit ||x - sol||_inf */
 } rbh_step_log;
 
+/* Launch shapes of the step (0 = default).  No reference counterpart: these
+ * are the knobs an autotuner sets. */
+typedef struct rbh_tuning {
+  int unit_size;          /* contact positions per colouring unit window (default 64) */
+  int lanes_per_scene;    /* sweep lanes of a batched scene, about one per wavefront (default 4) */
+  int resident_contacts;  /* LDS capacity of a resident solve group, contacts (multiple of 64) */
+  int resident_bodies;    /* LDS capacity of a resident solve group, bodies (multiple of 64) */
+  int lanes_per_group;    /* a single scene's cooperative grid: lanes per workgroup (default 4) */
+} rbh_tuning;
+
 typedef struct rbh_device_info {
   char name[256];
   int compute_units;
@@ -192,6 +202,11 @@ int rbh_get_timings(rbh_ctx* ctx, double* ms, int cap);
 /* Batch-level counters of the last step: [0] body pairs, [1] candidate slots,
  * [2] candidate triangles, [3] contacts. */
 int rbh_get_counters(rbh_ctx* ctx, long long* out, int cap);
+
+/* Set / read the launch shapes (rbh_tuning).  Takes effect at the next
+ * rbh_step; a captured graph is dropped.  No reference counterpart. */
+int rbh_set_tuning(rbh_ctx* ctx, const rbh_tuning* tuning);
+int rbh_get_tuning(rbh_ctx* ctx, rbh_tuning* tuning);
 
 /* Running totals of a scene since its creation or the last clear (clear != 0
  * zeroes them after reading): [0] committed steps, [1] contact-iterations

@@ -124,8 +124,7 @@ struct rbh_ctx {
   DBuf perm, s_id, s_n, s_rA, s_rB, s_b, s_r0, x0, x1, errbuf, scan_part;
   int cap_pairs = 0, cap_slots = 0, cap_cand = 0, cap_contacts = 0, cap_units = 0;
   int unit_T = 64;          // contacts per colouring unit (a scene's units are windows of this size)
-  int unit_split = 0;       // > 0: a scene solved by one workgroup is cut into about this many units
-  int unit_T_tuned = 0;     // unit size set through rbh_set_tuning (0: default)
+  rbh_tuning tune{};        // rbh_set_tuning (0 fields: defaults)
   // resident solve groups: LDS capacities (contacts, bodies, colour offsets) and the launch's dynamic LDS
   int res_cap_c = 0, res_cap_b = 0, res_cap_o = 0;
   size_t solve_lds = 0;
@@ -331,7 +330,8 @@ int check_params(rbh_ctx* c, const rbh_params& p) {
 }
 
 // ------------------------------------------------------------ batch workspace
-template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags) {
+// multi: a single scene solved by a cooperative grid (its lanes are its units)
+template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool multi) {
   Dev<R> d{};
 #define P(T, b) reinterpret_cast<T>(c->b.p)
   d.shapes = P(const ShapeInfo*, d_shapes); d.sV = P(const V4<R>*, d_sV); d.sT = P(const int4*, d_sT);
@@ -365,7 +365,9 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags) {
   d.f_bs = P(const int*, f_bs); d.uf = P(int*, uf); d.isl_cnt = P(int*, isl_cnt); d.isl_off = P(int*, isl_off);
   d.isl_fill = P(int*, isl_fill); d.isl_max = P(int*, isl_max); d.c_lab = P(int*, c_lab); d.uperm = P(int*, uperm);
   d.u_cnt = P(int*, u_cnt); d.u_base = P(int*, u_base); d.n_units = P(int*, n_units); d.cap_units = c->cap_units;
-  d.unit_T = c->unit_T; d.unit_split = c->unit_split; d.u_T = P(int*, u_T);
+  d.unit_T = c->unit_T;
+  d.unit_split = multi ? 0 : (c->tune.lanes_per_scene > 0 ? c->tune.lanes_per_scene : SOLVE_BLOCK / 64);
+  d.u_T = P(int*, u_T);
   d.isl_unit = P(int*, isl_unit); d.u_nbody = P(int*, u_nbody); d.u_boff = P(int*, u_boff);
   d.f_loc = P(int*, f_loc); d.f_unit = P(int*, f_unit); d.u_blist = P(int*, u_blist);
   d.s_loc = P(int2*, s_loc); d.s_W = P(V4<R>*, s_W);
@@ -468,8 +470,7 @@ int build_batch(rbh_ctx* c, const int* ids, int n) {
   // workgroups; batches: one workgroup per scene
   c->max_iter_batch = max_iter;
   c->multi_ok = (n == 1 && !serial);
-  c->unit_T = c->unit_T_tuned > 0 ? c->unit_T_tuned : 64;
-  c->unit_split = c->multi_ok ? 0 : SOLVE_BLOCK / 64;
+  c->unit_T = c->tune.unit_size > 0 ? c->tune.unit_size : 64;
   c->last_units = 0;
   // initial capacities; they grow when a step reports an overflow
   long long est_pairs = std::min<long long>(pairs_all, 64LL * c->nb + 4096);
@@ -523,7 +524,7 @@ hipEvent_t* timing_row(rbh_ctx* c, int i, int todo) {
 
 // ev: ST_N events to record the stage boundaries into, or nullptr.
 template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, hipEvent_t* ev, bool multi, int multi_grid) {
-  Dev<R> d = make_dev<R>(c, dt, flags);
+  Dev<R> d = make_dev<R>(c, dt, flags, multi);
   hipStream_t s = c->stream;
   const int nb = c->nb, ns = c->ns;
   auto mark = [&](int st) { if (ev) (void)hipEventRecord(ev[st], s); };
@@ -612,9 +613,9 @@ int first_step_index(rbh_ctx* c, int slot, int* out) {
 // whichever is larger; the kernels' limit is raised to it.
 template <typename R> int setup_solve_lds(rbh_ctx* c) {
   const int scale = sizeof(R) == 4 ? 2 : 1;
-  c->res_cap_c = 256 * scale;
-  c->res_cap_b = 256 * scale;
-  c->res_cap_o = 1024 * scale;
+  c->res_cap_c = c->tune.resident_contacts > 0 ? c->tune.resident_contacts : 256 * scale;
+  c->res_cap_b = c->tune.resident_bodies > 0 ? c->tune.resident_bodies : 256 * scale;
+  c->res_cap_o = 4 * c->res_cap_c;
   size_t bytes = std::max<size_t>(SOLVE_LDS_BYTES, res_lds_bytes<R>(c->res_cap_c, c->res_cap_b, c->res_cap_o,
                                                                     RES_LANES));
   const void* ks[2] = {reinterpret_cast<const void*>(k_solve<R>), reinterpret_cast<const void*>(k_solve_multi<R>)};
@@ -646,7 +647,8 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
   if (flags & RBH_STEP_FORCE_SINGLE) multi = false;
   if ((flags & RBH_STEP_FORCE_MULTI) && c->multi_ok && !use_graph) multi = true;
   // cooperative grid: about one unit per wavefront, from the unit count of the last step seen
-  auto grid_of = [&](int units) { return std::max(1, std::min((units + SOLVE_BLOCK / 64 - 1) / (SOLVE_BLOCK / 64), c->coop_max)); };
+  const int lpw = c->tune.lanes_per_group > 0 ? c->tune.lanes_per_group : SOLVE_BLOCK / 64;
+  auto grid_of = [&](int units) { return std::max(1, std::min((units + lpw - 1) / lpw, c->coop_max)); };
   int multi_grid = grid_of(c->last_units > 1 ? c->last_units : 64);
   int done = 0;
   for (int attempt = 0; attempt < 16 && done < n_steps; ++attempt) {
@@ -1221,6 +1223,34 @@ int rbh_get_counters(rbh_ctx* c, long long* out, int cap) {
   int m = std::min(cap, 4);
   for (int i = 0; i < m; ++i) out[i] = c->counters[i];
   return m;
+}
+
+int rbh_set_tuning(rbh_ctx* c, const rbh_tuning* t) {
+  if (!c || !t) return RBH_E_ARG;
+  if (t->unit_size < 0 || t->unit_size > 4096 || t->lanes_per_scene < 0 || t->lanes_per_scene > 64 ||
+      t->resident_contacts < 0 || t->resident_bodies < 0 || t->lanes_per_group < 0 || t->lanes_per_group > 64 ||
+      (t->resident_contacts % 64) || (t->resident_bodies % 64))
+    return fail(c, RBH_E_ARG, "tuning values out of range (resident capacities are multiples of 64)");
+  const size_t V = 4 * (size_t)c->prec;
+  const int cc = t->resident_contacts, cb = t->resident_bodies;
+  const size_t bytes = (size_t)cc * (7 * V + 8) + (size_t)cb * 6 * V + (size_t)4 * cc * 4 + RES_LANES * 16;
+  if (bytes > (size_t)c->prop.sharedMemPerBlock)
+    return fail(c, RBH_E_ARG, "resident capacities need " + std::to_string(bytes) + " bytes of LDS, more than " +
+                                  std::to_string(c->prop.sharedMemPerBlock));
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->tune = *t;
+  c->solve_lds = 0;          // recomputed (and the cooperative occupancy with it) by the next step
+  c->coop_max = 0;
+  c->batch_arena_gen = -1;   // the next step rebuilds the batch with the new unit sizes
+  drop_graph(c);
+  return 0;
+}
+
+int rbh_get_tuning(rbh_ctx* c, rbh_tuning* t) {
+  if (!c || !t) return RBH_E_ARG;
+  *t = c->tune;
+  return 0;
 }
 
 int rbh_scene_totals(rbh_ctx* c, int id, unsigned long long* out, int cap, int clear) {

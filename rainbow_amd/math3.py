@@ -22,15 +22,23 @@ def conjugate(q):
     return np.array([q[0], -q[1], -q[2], -q[3]], dtype=np.float64)
 
 
+def _prod(a0, a1, a2, a3, b0, b1, b2, b3):
+    # [a0 b0 - A.B, a0 B + b0 A + A x B] on Python floats
+    return (a0 * b0 - (a1 * b1 + a2 * b2 + a3 * b3),
+            a0 * b1 + a1 * b0 + (a2 * b3 - a3 * b2),
+            a0 * b2 + a2 * b0 + (a3 * b1 - a1 * b3),
+            a0 * b3 + a3 * b0 + (a1 * b2 - a2 * b1))
+
+
 def prod(a, b):
-    s = a[0] * b[0] - np.dot(a[1:], b[1:])
-    v = a[0] * b[1:] + a[1:] * b[0] + np.cross(a[1:], b[1:])
-    return np.array([s, v[0], v[1], v[2]], dtype=np.float64)
+    return np.array(_prod(float(a[0]), float(a[1]), float(a[2]), float(a[3]),
+                          float(b[0]), float(b[1]), float(b[2]), float(b[3])), dtype=np.float64)
 
 
 def rotate(q, r):
-    t = prod(prod(q, np.array([0.0, r[0], r[1], r[2]])), conjugate(q))
-    return t[1:].copy()
+    s, x, y, z = (float(c) for c in q)
+    t = _prod(*_prod(s, x, y, z, 0.0, float(r[0]), float(r[1]), float(r[2])), s, -x, -y, -z)
+    return np.array(t[1:], dtype=np.float64)
 
 
 def rotate_array(q, V):

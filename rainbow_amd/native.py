@@ -53,6 +53,11 @@ class StepLog(C.Structure):
                 ("natural_residual", C.c_double), ("max_penetration", C.c_double), ("final_merit", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("unit_size", "lanes_per_scene", "resident_contacts", "resident_bodies",
+                                        "lanes_per_group")]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 256), ("compute_units", C.c_int), ("lds_per_block", C.c_int),
                 ("hbm_bytes", C.c_ulonglong), ("precision_bytes", C.c_int), ("max_colors", C.c_int)]
@@ -87,6 +92,8 @@ _PROTOS = {
     "rbh_get_timings": [C.c_void_p, C.c_void_p, C.c_int],
     "rbh_get_counters": [C.c_void_p, C.c_void_p, C.c_int],
     "rbh_scene_totals": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int],
+    "rbh_set_tuning": [C.c_void_p, C.c_void_p],
+    "rbh_get_tuning": [C.c_void_p, C.c_void_p],
 }
 
 
@@ -281,6 +288,21 @@ class Context:
         c = np.zeros(4, dtype=np.int64)
         self._check(self.lib.rbh_get_counters(self.h, _ptr(c), 4))
         return dict(zip(["body_pairs", "candidate_slots", "candidates", "contacts"], c.tolist()))
+
+    def set_tuning(self, **kw):
+        """Launch shapes of the step (rbh_tuning fields; omitted ones keep their value)."""
+        t = Tuning()
+        self._check(self.lib.rbh_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        self._check(self.lib.rbh_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.lib.rbh_get_tuning(self.h, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in Tuning._fields_}
 
     TOTAL_KEYS = ("steps", "contact_iters", "contacts", "candidates", "fw_cell_evals", "sweeps")
 

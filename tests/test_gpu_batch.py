@@ -1,0 +1,112 @@
+"""GPU: launch shapes and the serving loop change no result.
+
+* Every tuning of the step (unit size, sweep lanes per scene, resident LDS
+  capacities small enough to force the velocity-only and global layouts)
+  gives bitwise the same contacts, impulses and states.
+* A request served by continuous batching, among other requests arriving
+  over time, ends in bitwise the same state as the same request run alone.
+"""
+import numpy as np
+import pytest
+
+from rainbow_amd import native, scenes, serving
+from rainbow_amd.stepper import engine_desc, gather_state
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = [
+    dict(),                                                   # defaults: resident groups
+    dict(unit_size=32, lanes_per_scene=2),
+    dict(unit_size=128, lanes_per_scene=8, lanes_per_group=2),
+    dict(resident_contacts=64, resident_bodies=64),           # groups too big: velocity-only layout
+]
+
+
+@pytest.fixture(scope="module")
+def bank():
+    return scenes.ShapeBank(resolution=32)
+
+
+def run_batch(bank, tuning, seeds, steps, single=False):
+    ctx = native.Context(0, "fp32")
+    try:
+        if tuning:
+            ctx.set_tuning(**tuning)
+        engines = [scenes.mixed_drop(s, bank, nmax=96, params=dict(precision="fp32")) for s in seeds]
+        for e in engines:
+            for b in e.bodies.values():
+                if not b.is_fixed:
+                    b.r = b.r - np.array([0.0, 0.8 * b.r[1] - 0.3, 0.0])   # start near the ground: contacts at once
+        ids = []
+        for e in engines:
+            desc, keep = engine_desc(e, ctx)
+            sid = ctx.scene_create(desc)
+            ctx.set_state(sid, *gather_state(list(e.bodies.values())))
+            ids.append(sid)
+        out = []
+        for _ in range(steps):
+            if single:
+                for sid in ids:
+                    ctx.step([sid], 1, 0.01)
+            else:
+                ctx.step(ids, 1, 0.01)
+            snap = []
+            for sid, e in zip(ids, engines):
+                c = ctx.get_contacts(sid)
+                snap.append([c["tri"], c["impulse"]] + list(ctx.get_state(sid, len(e.bodies))))
+            out.append(snap)
+        total = sum(ctx.scene_totals(s)["contacts"] for s in ids)
+        return out, total
+    finally:
+        ctx.close()
+
+
+def test_tunings_bitwise(bank):
+    ref, total = run_batch(bank, TUNINGS[0], (3200, 3201, 3202), 4)
+    assert total > 0
+    for t in TUNINGS[1:]:
+        got, _ = run_batch(bank, t, (3200, 3201, 3202), 4)
+        for a, b in zip(ref, got):
+            for sa, sb in zip(a, b):
+                for x, y in zip(sa, sb):
+                    assert np.array_equal(x, y), t
+
+
+def test_single_scene_schedules_bitwise(bank):
+    """A lone scene (cooperative grid) equals the same scene inside a batch."""
+    ref, _ = run_batch(bank, {}, (3203,), 4, single=True)
+    got, _ = run_batch(bank, {}, (3203, 3204), 4)
+    for a, b in zip(ref, got):
+        for x, y in zip(a[0], b[0]):
+            assert np.array_equal(x, y)
+
+
+def test_tuning_rejects_bad_values():
+    ctx = native.Context(0, "fp32")
+    try:
+        with pytest.raises(Exception):
+            ctx.set_tuning(resident_contacts=100)          # not a multiple of 64
+        with pytest.raises(Exception):
+            ctx.set_tuning(resident_contacts=1 << 16, resident_bodies=1 << 16)   # beyond LDS
+    finally:
+        ctx.close()
+
+
+def test_serving_alone_equals_mixed(bank):
+    ctx = native.Context(0, "fp32")
+    try:
+        srv = serving.Server(ctx=ctx, budget=1 << 30)
+        arrivals = serving.poisson_arrivals(400.0, 6, seed=7)
+        reqs = []
+        for i in range(6):
+            eng = scenes.serving_request(4100 + i, bank, nmax=40, params=dict(precision="fp32"))
+            reqs.append(srv.prepare(serving.Request(i, eng, 10 + 4 * i, float(arrivals[i]))))
+        rep = srv.run(reqs)
+        assert rep["scene_steps"] == sum(r.n_steps for r in reqs)
+        assert rep["ticks"] >= max(r.n_steps for r in reqs)
+        for r in reqs:
+            alone = srv.run_alone(r)
+            for x, y in zip(alone, r.state):
+                assert np.array_equal(x, y), r.rid
+    finally:
+        ctx.close()

@@ -21,7 +21,10 @@ the form this build defines for the GPU, so that a colouring is a function of
   node order, where max is the largest colour value present in the island
   (kept or tentative; 0 for nodes that never drew).
 * After 200 rounds, remaining nodes get fresh colours max+1, ... in node order
-  (blocking.py:100-104).
+  (blocking.py:100-104).  A round with no winner in which every uncoloured node
+  holds a one-colour palette (two adjacent nodes left with the same single
+  colour, say) repeats forever, so the rounds stop there; the result is that
+  of running all 200.
 * Sweep order: colours in increasing value; a colour's contacts are mutually
   independent, so their order inside the colour does not change the result.
 
@@ -95,8 +98,9 @@ def neighbours(ia, ib, fixed):
     return [sorted(s) for s in nb]
 
 
-def color(ia, ib, fixed, seed, step, max_rounds=MAX_ROUNDS):
-    """Colour per contact (int array)."""
+def color(ia, ib, fixed, seed, step, max_rounds=MAX_ROUNDS, shortcut=True):
+    """Colour per contact (int array).  shortcut=False runs every round up to
+    max_rounds (the literal loop); the result is the same."""
     K = len(ia)
     if K == 0:
         return np.zeros(0, dtype=np.int64)
@@ -123,11 +127,18 @@ def color(ia, ib, fixed, seed, step, max_rounds=MAX_ROUNDS):
         mx = {}
         for v in range(K):
             mx[isl[v]] = max(mx.get(isl[v], -1), int(C[v]))
+        filled = False
         for v in np.nonzero(U)[0]:
             if not pal[v]:
                 mx[isl[v]] += 1
                 pal[v] = {mx[isl[v]]}
+                filled = True
         rnd += 1
+        if shortcut and not acc and not filled and all(len(pal[v]) == 1 for v in np.nonzero(U)[0]):
+            # No winner and every palette a single colour: every later round
+            # draws the same colours and keeps none, so the remaining rounds
+            # change nothing; go to the leftovers directly.
+            break
     if U.any():
         mx = {}
         for v in range(K):

@@ -380,12 +380,18 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
     for (int i = threadIdx.x; i < K; i += NT) d.isl_max[a.isl(i)] = -1;
     if (threadIdx.x == 0) *s_flag = 0;
     __syncthreads();
+    int moving = 0;   // a winner, or an uncoloured node whose next draw is not forced
     for (int i = threadIdx.x; i < K; i += NT) {
-      if (a.done(i) == 2) a.set_done(i, 1);
+      if (a.done(i) == 2) { a.set_done(i, 1); moving = 1; }
       atomicMax(&d.isl_max[a.isl(i)], a.col(i));
       if (!a.done(i) && a.pc(i) == 0) atomicOr(s_flag, 1);
+      if (!a.done(i) && a.pc(i) != 1) moving = 1;
     }
     __syncthreads();
+    // No winner and every uncoloured palette a single colour: every later
+    // round draws the same colours and keeps none, so the rounds up to
+    // MAX_ROUNDS would change nothing; go to the leftovers directly.
+    if (block_sum<NT>(moving, sh) == 0) { rnd = MAX_ROUNDS; break; }
     if (*s_flag) {
       // rare: empty palettes get max+1, max+2, ... in increasing contact order
       // per island; ranks read only done/pc/isl/key, which stay fixed here
@@ -675,8 +681,9 @@ __device__ __forceinline__ V4<R> contact_prox(const SceneParams& sp, const Conta
   R xr = use_old_xn ? xo.x : xn;
   R os, ot, orr;
   if (sp.friction == 1) {
-    EllipsoidParams ep{sp.ell_tol, sp.ell_expansion, sp.ell_max_iter};
-    prox_ellipsoid<R>(z1, z2, z3, mu.x, mu.y, mu.z, xr, ep, os, ot, orr);
+    R o[3];
+    prox_ellipsoid_call<R>(z1, z2, z3, mu.x, mu.y, mu.z, xr, sp.ell_tol, sp.ell_expansion, sp.ell_max_iter, o);
+    os = o[0]; ot = o[1]; orr = o[2];
   } else {
     prox_sphere<R>(z1, z2, z3, mu.x, xr, os, ot, orr);
   }
@@ -1013,24 +1020,33 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
 }
 
 // ---- resident layout
+// Per contact i the four rows r of J are staged once (row 4i + r):
+//   JL = (lin_r, b_r), JA = (angA_r, r0_r), JB = (angB_r, mu_r) with mu_1..3 =
+//   the friction coefficients (mu_0 unused), and the iterates X0/X1[4i + r].
+// A contact is updated by a quad of four lanes, lane q holding row q; the
+// quad exchanges values with DPP quad broadcasts and every lane repeats the
+// serial update's operations on the same operands in the same order, so the
+// results are bitwise those of update_contact.
 template <typename R> __host__ __device__ constexpr size_t res_lds_bytes(int cc, int cb, int co, int cl) {
-  return (size_t)cc * (7 * sizeof(V4<R>) + sizeof(int2)) + (size_t)cb * 6 * sizeof(V4<R>) + (size_t)co * 4 +
-         (size_t)cl * 16;
+  return (size_t)cc * (12 * sizeof(V4<R>) + 8 * sizeof(R) + sizeof(int2)) + (size_t)cb * 6 * sizeof(V4<R>) +
+         (size_t)co * 4 + (size_t)cl * 16;
 }
 
 constexpr int RES_LANES = 64;   // lanes of a resident group
 
 template <typename R> struct ResLds {
-  V4<R> *n, *rA, *rB, *b, *r0, *x0, *x1;   // per contact (group order)
+  V4<R> *JL, *JA, *JB;                     // per contact row (4 per contact)
+  R *x0, *x1;                              // per contact row
   V4<R> *wl, *wa, *sl, *sa, *W0, *W1;      // per body (slot - b_lo)
-  int2* loc;                               // body of a contact's two bodies (slot - b_lo; -1: fixed)
+  int2* loc;                               // a contact's two bodies (slot - b_lo; -1: fixed)
   int4* lt;                                // per lane: first contact, K, colours, first colour offset
   int* coff;
   __device__ ResLds(unsigned char* p, int cc, int cb) {
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
-    n = v; rA = n + cc; rB = rA + cc; b = rB + cc; r0 = b + cc; x0 = r0 + cc; x1 = x0 + cc;
-    wl = x1 + cc; wa = wl + cb; sl = wa + cb; sa = sl + cb; W0 = sa + cb; W1 = W0 + cb;
-    loc = reinterpret_cast<int2*>(W1 + cb);
+    JL = v; JA = JL + 4 * cc; JB = JA + 4 * cc;
+    wl = JB + 4 * cc; wa = wl + cb; sl = wa + cb; sa = sl + cb; W0 = sa + cb; W1 = W0 + cb;
+    x0 = reinterpret_cast<R*>(W1 + cb); x1 = x0 + 4 * cc;
+    loc = reinterpret_cast<int2*>(x1 + 4 * cc);
     lt = reinterpret_cast<int4*>(loc + cc);
     coff = reinterpret_cast<int*>(lt + RES_LANES);
   }
@@ -1059,52 +1075,95 @@ template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d
   return g.nc <= d.res_cap_c && gr.b_hi - gr.b_lo <= d.res_cap_b && g.no <= d.res_cap_o && g.nl <= RES_LANES;
 }
 
-// one contact of a resident group (the arithmetic of update_contact)
-template <typename R>
-__device__ __forceinline__ double res_update(const SceneParams& sp, ResLds<R>& L, const V4<R>* rl, const V4<R>* ra,
-                                             const V4<R>* xa, V4<R>* xn, int i, R rs, bool jacobi, bool hybrid) {
-  const V4<R> n4 = L.n[i], rA4 = L.rA[i], rB4 = L.rB[i], bb = L.b[i], r0 = L.r0[i], xo = xa[i];
-  const int2 lc = L.loc[i];
-  ContactRow<R> J;
-  contact_rows(n4, rA4, rB4, J);
-  const V4<R> rr = {r0.x * rs, r0.y * rs, r0.z * rs, r0.w * rs};
-  const V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
-  const V4<R> z = {R(0), R(0), R(0), R(0)};
-  const bool fa = lc.x >= 0, fb = lc.y >= 0;
-  const V4<R> lA = fa ? rl[lc.x] : z, aA = fa ? ra[lc.x] : z;
-  const V4<R> lB = fb ? rl[lc.y] : z, aB = fb ? ra[lc.y] : z;
-  const V4<R> x = contact_prox(sp, J, xo, bb, rr, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), jacobi && !hybrid);
-  xn[i] = x;
-  const V4<R> dx = {x.x - xo.x, x.y - xo.y, x.z - xo.z, x.w - xo.w};
-  if (fa) {
-    V4<R> l = jacobi ? L.wl[lc.x] : lA, g = jacobi ? L.wa[lc.x] : aA;
-    dw_body(L.W0[lc.x], L.W1[lc.x], R(-1), J, J.angA, dx, l, g);
-    L.wl[lc.x] = l;
-    L.wa[lc.x] = g;
-  }
-  if (fb) {
-    V4<R> l = jacobi ? L.wl[lc.y] : lB, g = jacobi ? L.wa[lc.y] : aB;
-    dw_body(L.W0[lc.y], L.W1[lc.y], R(1), J, J.angB, dx, l, g);
-    L.wl[lc.y] = l;
-    L.wa[lc.y] = g;
-  }
-  return absmax4((double)x.x - (double)xo.x, (double)x.y - (double)xo.y, (double)x.z - (double)xo.z,
-                 (double)x.w - (double)xo.w);
+// value of lane K of each quad (DPP quad_perm(K, K, K, K))
+template <int K> __device__ __forceinline__ float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xf, 0xf, false));
+}
+template <int K> __device__ __forceinline__ double quad_bcast(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), K * 0x55, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), K * 0x55, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
 }
 
-// one sweep of one resident lane by STRIDE lanes (a wavefront or the workgroup)
+// One contact by one quad; lane q holds row q.  rl/ra: velocity changes the
+// rows read (working for Gauss-Seidel, accepted for Jacobi).  Returns
+// |x_q - xo_q| in FP64.
+template <typename R>
+__device__ __forceinline__ double res_update(const SceneParams& sp, ResLds<R>& L, const V4<R>* rl, const V4<R>* ra,
+                                             const R* xa, R* xn, int i, int q, R rs, bool jacobi, bool hybrid) {
+  const int row = 4 * i + q;
+  const V4<R> jl = L.JL[row], ja = L.JA[row], jb = L.JB[row];
+  const R xo = xa[row];
+  const int2 lc = L.loc[i];
+  const V4<R> z4 = {R(0), R(0), R(0), R(0)};
+  const bool fa = lc.x >= 0, fb = lc.y >= 0;
+  const V4<R> lA = fa ? rl[lc.x] : z4, aA = fa ? ra[lc.x] : z4;
+  const V4<R> lB = fb ? rl[lc.y] : z4, aB = fb ? ra[lc.y] : z4;
+  // row q of z = x - r (J w + b)  (contacts.py:77-92)
+  const R Jw = -dot(xyz(jl), xyz(lA)) - dot(xyz(ja), xyz(aA)) + dot(xyz(jl), xyz(lB)) + dot(xyz(jb), xyz(aB));
+  const R z = xo - (ja.w * rs) * (Jw + jl.w);
+  const R z0 = quad_bcast<0>(z), z1 = quad_bcast<1>(z), z2 = quad_bcast<2>(z), z3 = quad_bcast<3>(z);
+  const R xo0 = quad_bcast<0>(xo);
+  const R mu0 = quad_bcast<1>(jb.w), mu1 = quad_bcast<2>(jb.w), mu2 = quad_bcast<3>(jb.w);
+  // projections (contacts.py:92-95), the same in every lane of the quad
+  const R xnrm = fmax(R(0), z0);
+  const R xr = (jacobi && !hybrid) ? xo0 : xnrm;
+  R os, ot, orr;
+  if (sp.friction == 1) {
+    R o[3];
+    prox_ellipsoid_call<R>(z1, z2, z3, mu0, mu1, mu2, xr, sp.ell_tol, sp.ell_expansion, sp.ell_max_iter, o);
+    os = o[0]; ot = o[1]; orr = o[2];
+  } else {
+    prox_sphere<R>(z1, z2, z3, mu0, xr, os, ot, orr);
+  }
+  const R xq = q == 0 ? xnrm : (q == 1 ? os : (q == 2 ? ot : orr));
+  xn[row] = xq;
+  const R dq = xq - xo;
+  const R d0 = quad_bcast<0>(dq), d1 = quad_bcast<1>(dq), d2 = quad_bcast<2>(dq), d3 = quad_bcast<3>(dq);
+  // w += W J^T dx (contacts.py:107): lane 0 linear of A, 1 angular of A, 2 linear of B, 3 angular of B
+  const bool side_b = q >= 2, ang = q & 1;
+  const int slot = side_b ? lc.y : lc.x;
+  if (slot >= 0) {
+    const V4<R>* src = ang ? (side_b ? L.JB : L.JA) : L.JL;
+    const R sign = side_b ? R(1) : R(-1);
+    const R dd[4] = {d0, d1, d2, d3};
+    V3<R> acc = {R(0), R(0), R(0)};
+    for (int k = 0; k < 4; ++k) {
+      const V4<R> v = src[4 * i + k];
+      const R sk = sign * dd[k];
+      acc.x = acc.x + v.x * sk; acc.y = acc.y + v.y * sk; acc.z = acc.z + v.z * sk;
+    }
+    const V4<R> W0 = L.W0[slot], W1 = L.W1[slot];
+    V4<R>* dst = ang ? L.wa : L.wl;
+    const V4<R> old = dst[slot];
+    V3<R> t;
+    if (ang) {
+      const R W[6] = {W0.x, W0.y, W0.z, W0.w, W1.x, W1.y};
+      t = sym_mul(W, acc);
+    } else {
+      t = V3<R>{W1.z * acc.x, W1.z * acc.y, W1.z * acc.z};
+    }
+    dst[slot] = V4<R>{old.x + t.x, old.y + t.y, old.z + t.z, R(0)};
+  }
+  return fabs((double)xq - (double)xo);
+}
+
+// one sweep of one resident lane by STRIDE threads (a wavefront or the
+// workgroup): STRIDE / 4 contacts per pass
 template <typename R, int STRIDE>
 __device__ double res_sweep(const SceneParams& sp, ResLds<R>& L, int4 lt, bool serial, bool jacobi, bool hybrid,
-                            const V4<R>* xa, V4<R>* xn, R rs, int lane) {
+                            const R* xa, R* xn, R rs, int tid) {
   const V4<R>* rl = jacobi ? L.sl : L.wl;
   const V4<R>* ra = jacobi ? L.sa : L.wa;
   const int c0 = lt.x, nc = serial ? lt.y : lt.z;
   const int* co = L.coff + lt.w;
+  const int q = tid & 3, j = tid >> 2;
+  constexpr int PER = STRIDE / 4;
   double m = 0.0;
   for (int c = 0; c < nc; ++c) {
     const int lo = serial ? c : co[c], hi = serial ? c + 1 : co[c + 1];
-    for (int pos = lo + lane; pos < hi; pos += STRIDE)
-      m = fmax(m, res_update(sp, L, rl, ra, xa, xn, c0 + pos, rs, jacobi, hybrid));
+    for (int pos = lo + j; pos < hi; pos += PER)
+      m = fmax(m, res_update(sp, L, rl, ra, xa, xn, c0 + pos, q, rs, jacobi, hybrid));
     phase_sync<STRIDE>();
   }
   return m;
@@ -1120,7 +1179,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
   ResLds<R> L(smem, d.res_cap_c, d.res_cap_b);
   const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
-  // stage: lane table, colour offsets, contacts, bodies
+  // stage: lane table, colour offsets, contact rows, bodies
   if (threadIdx.x == 0) {
     int t = 0, o = 0;
     for (int l = gr.l_lo; l < gr.l_hi; ++l) {
@@ -1143,8 +1202,20 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   }
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
     const int pos = gs.p0 + i;
-    L.n[i] = d.s_n[pos]; L.rA[i] = d.s_rA[pos]; L.rB[i] = d.s_rB[pos]; L.b[i] = d.s_b[pos]; L.r0[i] = d.s_r0[pos];
-    L.x0[i] = d.x0[pos]; L.x1[i] = d.x1[pos];
+    const V4<R> n4 = d.s_n[pos], rA4 = d.s_rA[pos], rB4 = d.s_rB[pos], bb = d.s_b[pos], r0 = d.s_r0[pos];
+    ContactRow<R> J;
+    contact_rows(n4, rA4, rB4, J);
+    const R b[4] = {bb.x, bb.y, bb.z, bb.w}, rr[4] = {r0.x, r0.y, r0.z, r0.w};
+    const R mu[4] = {R(0), n4.w, rA4.w, rB4.w};
+    const V4<R> xa0 = d.x0[pos], xa1 = d.x1[pos];
+    const R x0v[4] = {xa0.x, xa0.y, xa0.z, xa0.w}, x1v[4] = {xa1.x, xa1.y, xa1.z, xa1.w};
+    for (int r = 0; r < 4; ++r) {
+      L.JL[4 * i + r] = V4<R>{J.lin[r].x, J.lin[r].y, J.lin[r].z, b[r]};
+      L.JA[4 * i + r] = V4<R>{J.angA[r].x, J.angA[r].y, J.angA[r].z, rr[r]};
+      L.JB[4 * i + r] = V4<R>{J.angB[r].x, J.angB[r].y, J.angB[r].z, mu[r]};
+      L.x0[4 * i + r] = x0v[r];
+      L.x1[4 * i + r] = x1v[r];
+    }
     const int2 lc = d.s_loc[pos];
     L.loc[i] = make_int2(lc.x >= 0 ? lc.x - b0 : -1, lc.y >= 0 ? lc.y - b0 : -1);
   }
@@ -1159,8 +1230,8 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   SolveCtl ctl;
   ctl.iters = sp.max_iter;
   for (int it = 0; it < sp.max_iter; ++it) {
-    const V4<R>* xa = ctl.acc ? L.x1 : L.x0;
-    V4<R>* xn = ctl.acc ? L.x0 : L.x1;
+    const R* xa = ctl.acc ? L.x1 : L.x0;
+    R* xn = ctl.acc ? L.x0 : L.x1;
     const R rs = R(ctl.rscale);
     double m = 0.0;
     for (int t = wave; t < gs.nl; t += nwave) {
@@ -1188,19 +1259,26 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     if (dec & 2) { ctl.iters = it; break; }
   }
   // impulses to contact order; natural-map residual (FP64) with r0; accepted velocity changes out
-  const V4<R>* xs = ctl.acc ? L.x1 : L.x0;
+  const R* xs = ctl.acc ? L.x1 : L.x0;
   double rloc = 0.0, xmax = 0.0;
   const V4<R> z = {R(0), R(0), R(0), R(0)};
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
-    const V4<R> x = xs[i], n4 = L.n[i], rA4 = L.rA[i], rB4 = L.rB[i];
+    const V4<R> x = {xs[4 * i], xs[4 * i + 1], xs[4 * i + 2], xs[4 * i + 3]};
     d.c_imp[d.perm[gs.p0 + i]] = x;
     ContactRow<R> J;
-    contact_rows(n4, rA4, rB4, J);
-    const V4<R> mu = {n4.w, rA4.w, rB4.w, R(0)};
+    V4<R> bb, r0;
+    R* bp = &bb.x;
+    R* rp = &r0.x;
+    for (int r = 0; r < 4; ++r) {
+      const V4<R> a = L.JL[4 * i + r], e = L.JA[4 * i + r], f = L.JB[4 * i + r];
+      J.lin[r] = xyz(a); J.angA[r] = xyz(e); J.angB[r] = xyz(f);
+      bp[r] = a.w; rp[r] = e.w;
+    }
+    const V4<R> mu = {L.JB[4 * i + 1].w, L.JB[4 * i + 2].w, L.JB[4 * i + 3].w, R(0)};
     const int2 lc = L.loc[i];
     const V4<R> lA = lc.x >= 0 ? L.sl[lc.x] : z, aA = lc.x >= 0 ? L.sa[lc.x] : z;
     const V4<R> lB = lc.y >= 0 ? L.sl[lc.y] : z, aB = lc.y >= 0 ? L.sa[lc.y] : z;
-    const V4<R> y = contact_prox(sp, J, x, L.b[i], L.r0[i], mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
+    const V4<R> y = contact_prox(sp, J, x, bb, r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
     rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
                               (double)x.w - (double)y.w));
     xmax = fmax(xmax, absmax4(x.x, x.y, x.z, x.w));

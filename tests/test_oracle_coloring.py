@@ -104,3 +104,23 @@ def test_draw_is_uniform():
     n = 7
     cnt = np.bincount([C.draw(5, 3, v, 2, n) for v in range(70000)], minlength=n)
     assert np.all(np.abs(cnt / 10000.0 - 1) < 0.05)
+
+
+def test_stuck_rounds_shortcut_equals_full_run():
+    """Stopping at a round that can only repeat gives the colouring of all 200 rounds."""
+    rng = np.random.default_rng(11)
+    cases = [(np.array([0] * 61), np.array([1] * 61), np.array([True, False])),     # a 61-clique
+             (np.array([0, 0]), np.array([1, 1]), np.array([True, False]))]           # a stuck pair
+    for _ in range(6):
+        nbod = int(rng.integers(3, 9))
+        K = int(rng.integers(5, 40))
+        ia = rng.integers(0, nbod, K)
+        ib = (ia + rng.integers(1, nbod, K)) % nbod
+        fixed = rng.uniform(size=nbod) < 0.3
+        cases.append((ia, ib, fixed))
+    for ia, ib, fixed in cases:
+        for step in (0, 5):
+            fast = C.color(ia, ib, fixed, 3, step)
+            full = C.color(ia, ib, fixed, 3, step, shortcut=False)
+            assert np.array_equal(fast, full)
+            assert C.is_proper(fast, ia, ib, fixed)

@@ -101,7 +101,7 @@ struct rbh_ctx {
   RangeAlloc bodies, tris, res;
   DBuf r, q, v, w, b_shape, b_slot, b_fixed, b_mat, b_tri, b_mass, b_inertia, b_grav, b_damp;
   DBuf W0, W1, du0, du1, lo, hi, ws0, ws1, ww0, ww1;
-  DBuf tlo, thi;
+  DBuf tlo, thi, clo, chi;
   // scenes
   std::vector<SceneHost> scenes;
   std::vector<SceneParams> sp_host;
@@ -184,7 +184,7 @@ int ensure(rbh_ctx* c, DBuf& b, size_t bytes, bool keep = false) {
 std::vector<DBuf*> all_bufs(rbh_ctx* c) {
   return {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
           &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
-          &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi,
+          &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi, &c->clo, &c->chi,
           &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
           &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs,
           &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
@@ -251,6 +251,8 @@ int grow_tris(rbh_ctx* c, int need) {
   size_t V = 4 * (size_t)c->prec;
   int rc = ensure(c, c->tlo, (size_t)cap * V, true);
   if (!rc) rc = ensure(c, c->thi, (size_t)cap * V, true);
+  if (!rc) rc = ensure(c, c->clo, (size_t)cap * V, true);
+  if (!rc) rc = ensure(c, c->chi, (size_t)cap * V, true);
   if (rc) return rc;
   c->tris.grow(cap);
   c->arena_gen++;
@@ -345,6 +347,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.b_W0 = P(V4<R>*, W0); d.b_W1 = P(V4<R>*, W1); d.b_du0 = P(V4<R>*, du0); d.b_du1 = P(V4<R>*, du1);
   d.b_lo = P(V4<R>*, lo); d.b_hi = P(V4<R>*, hi); d.ws0 = P(V4<R>*, ws0); d.ws1 = P(V4<R>*, ws1);
   d.ww0 = P(V4<R>*, ww0); d.ww1 = P(V4<R>*, ww1); d.tlo = P(V4<R>*, tlo); d.thi = P(V4<R>*, thi);
+  d.clo = P(V4<R>*, clo); d.chi = P(V4<R>*, chi);
   d.nb = c->nb; d.ns = c->ns;
   d.bb_body = P(const int*, bb_body); d.bs_slot = P(const int*, bs_slot); d.bs_foff = P(const int*, bs_foff);
   d.b2f = P(const int*, b2f); d.f_end = P(const int*, f_end); d.d_nb = P(const int*, d_nb);
@@ -989,8 +992,35 @@ int rbh_shape_upload(rbh_ctx* c, const double* V, int nv, const int* T, int nt, 
   for (int i = 0; i < nv; ++i) {
     c->sV.push_back(V[3 * i]); c->sV.push_back(V[3 * i + 1]); c->sV.push_back(V[3 * i + 2]); c->sV.push_back(0.0);
   }
+  // the fourth int of triangle record p holds the triangle at position p of
+  // the cluster order: triangles sorted by the Morton code of their centroid
+  // (10 bits per axis over the shape's box), so each run of CLUSTER
+  // positions is a compact patch of the surface
+  double lo[3] = {V[0], V[1], V[2]}, hi[3] = {V[0], V[1], V[2]};
+  for (int i = 0; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], V[3 * i + a]); hi[a] = std::max(hi[a], V[3 * i + a]); }
+  auto spread = [](unsigned x) {   // 10 bits -> every third bit
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+  };
+  std::vector<std::pair<unsigned, int>> key(nt);
   for (int t = 0; t < nt; ++t) {
-    c->sT.push_back(T[3 * t]); c->sT.push_back(T[3 * t + 1]); c->sT.push_back(T[3 * t + 2]); c->sT.push_back(0);
+    unsigned q[3];
+    for (int a = 0; a < 3; ++a) {
+      const double cen = (V[3 * T[3 * t] + a] + V[3 * T[3 * t + 1] + a] + V[3 * T[3 * t + 2] + a]) / 3.0;
+      const double u = hi[a] > lo[a] ? (cen - lo[a]) / (hi[a] - lo[a]) : 0.0;
+      q[a] = (unsigned)std::min(1023.0, std::max(0.0, u * 1024.0));
+    }
+    key[t] = {spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2), t};
+  }
+  std::stable_sort(key.begin(), key.end());
+  for (int t = 0; t < nt; ++t) {
+    c->sT.push_back(T[3 * t]); c->sT.push_back(T[3 * t + 1]); c->sT.push_back(T[3 * t + 2]);
+    c->sT.push_back(key[t].second);
   }
   c->sdf.insert(c->sdf.end(), sdf, sdf + (size_t)I * J * K);
   c->shapes.push_back(s);

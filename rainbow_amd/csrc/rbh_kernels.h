@@ -66,6 +66,7 @@ template <typename R> struct Dev {
   V4<R>* ws0; V4<R>* ws1;              // accepted W J^T sol
   V4<R>* ww0; V4<R>* ww1;              // working copy during a sweep
   V4<R>* tlo; V4<R>* thi;              // per body-instance triangle AABBs
+  V4<R>* clo; V4<R>* chi;              // per body-instance cluster AABBs (CLUSTER triangles each) at b_tri + c
   // batch
   int nb, ns;
   const int* bb_body;                  // flat batch body -> arena body
@@ -138,6 +139,12 @@ template <typename R> __device__ __forceinline__ bool box_overlap(V4<R> alo, V4<
   return !(ahi.x < blo.x || bhi.x < alo.x || ahi.y < blo.y || bhi.y < alo.y || ahi.z < blo.z || bhi.z < alo.z);
 }
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -181,6 +188,13 @@ template <typename R> __global__ void k_body_prep(Dev<R> d) {
 // ========================================================== BVH leaf refit
 // collision_detection.py:10-44 + kdop.py:53-75: world vertices, per-triangle
 // AABB grown by the envelope, body AABB = union.  One wave per body.
+// Triangles in clusters of CLUSTER consecutive positions of the shape's
+// cluster order (sT[t_off + p].w is the triangle at position p; the order
+// follows the Morton code of the centroids).  A cluster's box bounds its
+// triangles' boxes, so a cluster that misses a volume holds no triangle that
+// meets it.
+constexpr int CLUSTER = 32;
+
 template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
   int f = blockIdx.x;
   if (f >= d.nb || aborted(d.err)) return;
@@ -192,19 +206,37 @@ template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
   const R inf = R(INFINITY);
   R lx = inf, ly = inf, lz = inf, hx = -inf, hy = -inf, hz = -inf;
   const int tb = d.b_tri[b];
-  for (int t = threadIdx.x; t < sh.nt; t += 64) {
-    int4 tri = d.sT[sh.t_off + t];
-    V3<R> p0 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.x])) + r;
-    V3<R> p1 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.y])) + r;
-    V3<R> p2 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.z])) + r;
-    V4<R> lo = {fmin(fmin(p0.x, p1.x), p2.x) - env, fmin(fmin(p0.y, p1.y), p2.y) - env,
-                fmin(fmin(p0.z, p1.z), p2.z) - env, R(0)};
-    V4<R> hi = {fmax(fmax(p0.x, p1.x), p2.x) + env, fmax(fmax(p0.y, p1.y), p2.y) + env,
-                fmax(fmax(p0.z, p1.z), p2.z) + env, R(0)};
-    d.tlo[tb + t] = lo;
-    d.thi[tb + t] = hi;
-    lx = fmin(lx, lo.x); ly = fmin(ly, lo.y); lz = fmin(lz, lo.z);
-    hx = fmax(hx, hi.x); hy = fmax(hy, hi.y); hz = fmax(hz, hi.z);
+  const int lane = threadIdx.x;
+  for (int base = 0; base < sh.nt; base += 64) {
+    const int t = base + lane;
+    R cx = inf, cy = inf, cz = inf, dx = -inf, dy = -inf, dz = -inf;
+    if (t < sh.nt) {
+      const int tt = d.sT[sh.t_off + t].w;   // triangle at cluster position t
+      int4 tri = d.sT[sh.t_off + tt];
+      V3<R> p0 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.x])) + r;
+      V3<R> p1 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.y])) + r;
+      V3<R> p2 = qrotate_array(q, xyz(d.sV[sh.v_off + tri.z])) + r;
+      V4<R> lo = {fmin(fmin(p0.x, p1.x), p2.x) - env, fmin(fmin(p0.y, p1.y), p2.y) - env,
+                  fmin(fmin(p0.z, p1.z), p2.z) - env, R(0)};
+      V4<R> hi = {fmax(fmax(p0.x, p1.x), p2.x) + env, fmax(fmax(p0.y, p1.y), p2.y) + env,
+                  fmax(fmax(p0.z, p1.z), p2.z) + env, R(0)};
+      d.tlo[tb + tt] = lo;
+      d.thi[tb + tt] = hi;
+      cx = lo.x; cy = lo.y; cz = lo.z; dx = hi.x; dy = hi.y; dz = hi.z;
+    }
+    // cluster boxes: the 32 lanes of each half-wave hold one cluster
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+      cx = fmin(cx, __shfl_xor(cx, o, 64)); cy = fmin(cy, __shfl_xor(cy, o, 64)); cz = fmin(cz, __shfl_xor(cz, o, 64));
+      dx = fmax(dx, __shfl_xor(dx, o, 64)); dy = fmax(dy, __shfl_xor(dy, o, 64)); dz = fmax(dz, __shfl_xor(dz, o, 64));
+    }
+    if ((lane & (CLUSTER - 1)) == 0 && t < sh.nt) {
+      const int c = t / CLUSTER;
+      d.clo[tb + c] = V4<R>{cx, cy, cz, R(0)};
+      d.chi[tb + c] = V4<R>{dx, dy, dz, R(0)};
+    }
+    lx = fmin(lx, cx); ly = fmin(ly, cy); lz = fmin(lz, cz);
+    hx = fmax(hx, dx); hy = fmax(hy, dy); hz = fmax(hz, dz);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -262,70 +294,117 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
 // ========================================================== candidate triangles
 // Candidate set of X w.r.t. Y: triangles of X whose AABB overlaps the AABB of
 // some triangle of Y (the BVH traversal result, collision_detection.py:71,
-// kdop_bvh.py:328-345, reduced by np.unique at :234-246).  One workgroup per
-// body pair; Y's triangles overlapping X's root volume are staged in LDS.
+// kdop_bvh.py:328-345, reduced by np.unique at :234-246).  One wavefront per
+// (body pair, side): Y's triangles that overlap X's root volume are staged in
+// LDS, then every triangle of X that overlaps Y's root volume is tested
+// against them.  Clusters whose box misses the other root volume are skipped
+// whole (none of their triangles can meet it).
 constexpr int CAND_BLOCK = 256;
+constexpr int CAND_WAVES = CAND_BLOCK / 64;
 
 template <typename R> struct CandLds {
-  static constexpr int CAP = (sizeof(R) == 4) ? 2000 : 1000;   // 2 x CAP x sizeof(V4<R>) <= 64000 B
+  static constexpr int CAP = (sizeof(R) == 4) ? 512 : 256;   // staged triangles per wavefront (16 KB)
 };
+
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+constexpr int CAND_CL = 64;   // staged clusters per wavefront
 
 template <typename R>
 __device__ void cand_side(const Dev<R>& d, int X, int Y, int slot_base, bool count_all, V4<R>* lo_s, V4<R>* hi_s,
-                          int* sh_scan, unsigned long long& overlaps) {
+                          V4<R>* clo_s, V4<R>* chi_s, int2* crange_s, unsigned long long& overlaps) {
   const int cap = CandLds<R>::CAP;
+  const int lane = threadIdx.x & 63;
   const ShapeInfo& sX = d.shapes[d.b_shape[X]];
   const ShapeInfo& sY = d.shapes[d.b_shape[Y]];
   const int tX = d.b_tri[X], tY = d.b_tri[Y];
   const V4<R> rXlo = d.b_lo[X], rXhi = d.b_hi[X], rYlo = d.b_lo[Y], rYhi = d.b_hi[Y];
-  int ycur = 0;
+  int ybase = 0;
   bool first = true;
   while (true) {
-    // stage the next run of Y triangles that overlap root(X)
-    int nfill = 0;
-    while (ycur < sY.nt) {
-      int t = ycur + threadIdx.x;
+    // stage the next run of Y triangles that overlap root(X), two clusters
+    // (64 positions) at a time, with the boxes of their clusters
+    int nfill = 0, ncl = 0;
+    while (ybase < sY.nt) {
+      const int c0 = ybase / CLUSTER;
+      const bool has1 = ybase + CLUSTER < sY.nt;
+      const V4<R> b0lo = d.clo[tY + c0], b0hi = d.chi[tY + c0];
+      const V4<R> b1lo = has1 ? d.clo[tY + c0 + 1] : b0lo, b1hi = has1 ? d.chi[tY + c0 + 1] : b0hi;
+      const bool o0 = box_overlap(b0lo, b0hi, rXlo, rXhi);
+      const bool o1 = has1 && box_overlap(b1lo, b1hi, rXlo, rXhi);
+      if (!o0 && !o1) { ybase += 64; continue; }
+      const int p = ybase + lane;
       V4<R> lo, hi;
       bool in = false;
-      if (t < sY.nt) {
+      if (p < sY.nt && (lane < CLUSTER ? o0 : o1)) {
+        const int t = d.sT[sY.t_off + p].w;
         lo = d.tlo[tY + t];
         hi = d.thi[tY + t];
         in = box_overlap(lo, hi, rXlo, rXhi);
       }
-      int tot;
-      int pos = block_exclusive_scan(in ? 1 : 0, sh_scan, tot);
-      if (nfill + tot > cap) break;   // uniform: tot is block-wide
-      if (in) { lo_s[nfill + pos] = lo; hi_s[nfill + pos] = hi; }
-      nfill += tot;
-      ycur += CAND_BLOCK;
+      const unsigned long long m = __ballot(in);
+      const int n0 = __popcll(m & 0xffffffffull), n1 = __popcll(m >> 32);
+      if (nfill + n0 + n1 > cap || ncl + 2 > CAND_CL) break;   // uniform: stage full, test what is staged
+      if (in) { const int pos = nfill + lanes_below(m); lo_s[pos] = lo; hi_s[pos] = hi; }
+      if (lane == 0) {
+        int k = ncl;
+        if (n0) { clo_s[k] = b0lo; chi_s[k] = b0hi; crange_s[k] = make_int2(nfill, n0); ++k; }
+        if (n1) { clo_s[k] = b1lo; chi_s[k] = b1hi; crange_s[k] = make_int2(nfill + n0, n1); }
+      }
+      ncl += (n0 ? 1 : 0) + (n1 ? 1 : 0);
+      nfill += n0 + n1;
+      ybase += 64;
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < sX.nt; t += CAND_BLOCK) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int xbase = 0; xbase < sX.nt; xbase += 64) {
+      const int c0 = xbase / CLUSTER;
+      const bool o0 = box_overlap(d.clo[tX + c0], d.chi[tX + c0], rYlo, rYhi);
+      const bool o1 = xbase + CLUSTER < sX.nt && box_overlap(d.clo[tX + c0 + 1], d.chi[tX + c0 + 1], rYlo, rYhi);
+      const int p = xbase + lane;
+      if (p >= sX.nt) continue;
+      const int t = d.sT[sX.t_off + p].w;
+      if (!(lane < CLUSTER ? o0 : o1)) {
+        if (first) d.slot_flag[slot_base + t] = 0;
+        continue;
+      }
       unsigned char flag = first ? 0 : d.slot_flag[slot_base + t];
       if (flag && !count_all) continue;
       V4<R> lo = d.tlo[tX + t], hi = d.thi[tX + t];
       if (!box_overlap(lo, hi, rYlo, rYhi)) { if (first) d.slot_flag[slot_base + t] = 0; continue; }
       unsigned cnt = 0;
-      for (int e = 0; e < nfill; ++e) {
-        if (box_overlap(lo, hi, lo_s[e], hi_s[e])) {
-          flag = 1;
-          if (!count_all) break;
-          ++cnt;
+      for (int k = 0; k < ncl && !(flag && !count_all); ++k) {
+        if (!box_overlap(lo, hi, clo_s[k], chi_s[k])) continue;
+        const int2 rg = crange_s[k];
+        for (int e = rg.x; e < rg.x + rg.y; ++e) {
+          if (box_overlap(lo, hi, lo_s[e], hi_s[e])) {
+            flag = 1;
+            if (!count_all) break;
+            ++cnt;
+          }
         }
       }
       overlaps += cnt;
       d.slot_flag[slot_base + t] = flag;
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     first = false;
-    if (ycur >= sY.nt) break;
+    if (ybase >= sY.nt) break;
   }
 }
 
+// one wavefront per (pair, side): item 2p is A's candidates w.r.t. B, 2p + 1 B's w.r.t. A
 template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags(Dev<R> d) {
-  __shared__ V4<R> lo_s[CandLds<R>::CAP];
-  __shared__ V4<R> hi_s[CandLds<R>::CAP];
-  __shared__ int sh_scan[CAND_BLOCK / 64];
+  __shared__ V4<R> lo_s[CAND_WAVES][CandLds<R>::CAP];
+  __shared__ V4<R> hi_s[CAND_WAVES][CandLds<R>::CAP];
+  __shared__ V4<R> clo_s[CAND_WAVES][CAND_CL];
+  __shared__ V4<R> chi_s[CAND_WAVES][CAND_CL];
+  __shared__ int2 crange_s[CAND_WAVES][CAND_CL];
   const int np = *d.n_pairs;
   if (np > d.cap_pairs) return;
   if (aborted(d.err)) return;
@@ -335,15 +414,21 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
     return;
   }
   const bool count_all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
-  for (int p = blockIdx.x; p < np; p += gridDim.x) {
-    int2 ab = d.pairs[p];
-    int base = d.slot_off[p];
+  const int wave = threadIdx.x >> 6;
+  for (int item = blockIdx.x * CAND_WAVES + wave; item < 2 * np; item += gridDim.x * CAND_WAVES) {
+    const int p = item >> 1, side = item & 1;
+    const int2 ab = d.pairs[p];
+    const int base = d.slot_off[p];
     unsigned long long ov = 0;
-    cand_side(d, ab.x, ab.y, base, count_all, lo_s, hi_s, sh_scan, ov);
-    int ntA = d.shapes[d.b_shape[ab.x]].nt;
-    unsigned long long ov2 = 0;
-    cand_side(d, ab.y, ab.x, base + ntA, false, lo_s, hi_s, sh_scan, ov2);
-    if (count_all && ov) atomicAdd(&d.ss[d.b_slot[ab.x]].n_overlaps, ov);
+    if (side == 0) {
+      cand_side(d, ab.x, ab.y, base, count_all, lo_s[wave], hi_s[wave], clo_s[wave], chi_s[wave], crange_s[wave], ov);
+      const unsigned long long tot = wave_sum_u64(ov);
+      if (count_all && (threadIdx.x & 63) == 0 && tot) atomicAdd(&d.ss[d.b_slot[ab.x]].n_overlaps, tot);
+    } else {
+      const int ntA = d.shapes[d.b_shape[ab.x]].nt;
+      cand_side(d, ab.y, ab.x, base + ntA, false, lo_s[wave], hi_s[wave], clo_s[wave], chi_s[wave], crange_s[wave],
+                ov);
+    }
   }
 }
 

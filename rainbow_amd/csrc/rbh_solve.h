@@ -1091,14 +1091,22 @@ template <int K> __device__ __forceinline__ double quad_bcast(double v) {
 template <typename R>
 __device__ __forceinline__ double res_update(const SceneParams& sp, ResLds<R>& L, const V4<R>* rl, const V4<R>* ra,
                                              const R* xa, R* xn, int i, int q, R rs, bool jacobi, bool hybrid) {
+  // every load first (none depends on this contact's results; the only
+  // writes come at the end, to this contact's own iterate and bodies)
   const int row = 4 * i + q;
+  const int2 lc = L.loc[i];
   const V4<R> jl = L.JL[row], ja = L.JA[row], jb = L.JB[row];
   const R xo = xa[row];
-  const int2 lc = L.loc[i];
+  const bool side_b = q >= 2, ang = q & 1;   // lane 0: linear A, 1: angular A, 2: linear B, 3: angular B
+  const int slot = side_b ? lc.y : lc.x;
+  const V4<R>* src = ang ? (side_b ? L.JB : L.JA) : L.JL;
+  const V4<R> v0 = src[4 * i], v1 = src[4 * i + 1], v2 = src[4 * i + 2], v3 = src[4 * i + 3];
+  V4<R>* dst = ang ? L.wa : L.wl;
   const V4<R> z4 = {R(0), R(0), R(0), R(0)};
-  const bool fa = lc.x >= 0, fb = lc.y >= 0;
+  const bool fa = lc.x >= 0, fb = lc.y >= 0, own = slot >= 0;
   const V4<R> lA = fa ? rl[lc.x] : z4, aA = fa ? ra[lc.x] : z4;
   const V4<R> lB = fb ? rl[lc.y] : z4, aB = fb ? ra[lc.y] : z4;
+  const V4<R> W0 = own ? L.W0[slot] : z4, W1 = own ? L.W1[slot] : z4, old = own ? dst[slot] : z4;
   // row q of z = x - r (J w + b)  (contacts.py:77-92)
   const R Jw = -dot(xyz(jl), xyz(lA)) - dot(xyz(ja), xyz(aA)) + dot(xyz(jl), xyz(lB)) + dot(xyz(jb), xyz(aB));
   const R z = xo - (ja.w * rs) * (Jw + jl.w);
@@ -1120,22 +1128,15 @@ __device__ __forceinline__ double res_update(const SceneParams& sp, ResLds<R>& L
   xn[row] = xq;
   const R dq = xq - xo;
   const R d0 = quad_bcast<0>(dq), d1 = quad_bcast<1>(dq), d2 = quad_bcast<2>(dq), d3 = quad_bcast<3>(dq);
-  // w += W J^T dx (contacts.py:107): lane 0 linear of A, 1 angular of A, 2 linear of B, 3 angular of B
-  const bool side_b = q >= 2, ang = q & 1;
-  const int slot = side_b ? lc.y : lc.x;
-  if (slot >= 0) {
-    const V4<R>* src = ang ? (side_b ? L.JB : L.JA) : L.JL;
+  // w += W J^T dx (contacts.py:107), this lane's block
+  if (own) {
     const R sign = side_b ? R(1) : R(-1);
-    const R dd[4] = {d0, d1, d2, d3};
+    const R s0 = sign * d0, s1 = sign * d1, s2 = sign * d2, s3 = sign * d3;
     V3<R> acc = {R(0), R(0), R(0)};
-    for (int k = 0; k < 4; ++k) {
-      const V4<R> v = src[4 * i + k];
-      const R sk = sign * dd[k];
-      acc.x = acc.x + v.x * sk; acc.y = acc.y + v.y * sk; acc.z = acc.z + v.z * sk;
-    }
-    const V4<R> W0 = L.W0[slot], W1 = L.W1[slot];
-    V4<R>* dst = ang ? L.wa : L.wl;
-    const V4<R> old = dst[slot];
+    acc.x = acc.x + v0.x * s0; acc.y = acc.y + v0.y * s0; acc.z = acc.z + v0.z * s0;
+    acc.x = acc.x + v1.x * s1; acc.y = acc.y + v1.y * s1; acc.z = acc.z + v1.z * s1;
+    acc.x = acc.x + v2.x * s2; acc.y = acc.y + v2.y * s2; acc.z = acc.z + v2.z * s2;
+    acc.x = acc.x + v3.x * s3; acc.y = acc.y + v3.y * s3; acc.z = acc.z + v3.z * s3;
     V3<R> t;
     if (ang) {
       const R W[6] = {W0.x, W0.y, W0.z, W0.w, W1.x, W1.y};

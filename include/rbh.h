@@ -113,8 +113,8 @@ typedef struct rbh_step_log {
 typedef struct rbh_tuning {
   int unit_size;          /* contact positions per colouring unit window (default 64) */
   int lanes_per_scene;    /* sweep lanes of a batched scene, about one per wavefront (default 4) */
-  int resident_contacts;  /* LDS capacity of a resident solve group, contacts (multiple of 64) */
-  int resident_bodies;    /* LDS capacity of a resident solve group, bodies (multiple of 64) */
+  int resident_contacts;  /* most contacts of a solve group kept in LDS (multiple of 64; at most 384, FP64 192) */
+  int resident_bodies;    /* most bodies of a solve group kept in LDS (multiple of 64; same bounds) */
   int lanes_per_group;    /* a single scene's cooperative grid: lanes per workgroup (default 4) */
 } rbh_tuning;
 

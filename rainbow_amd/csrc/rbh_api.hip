@@ -611,16 +611,14 @@ int first_step_index(rbh_ctx* c, int slot, int* out) {
   return 0;
 }
 
-// Dynamic LDS of the solve kernels: the resident-group capacities (FP32:
-// 384 contacts, 384 bodies, ~130 KB; FP64: half) or the velocity-only layout,
+// Dynamic LDS of the solve kernels: the resident layout (FP32: 384 contacts,
+// 384 bodies, ~130 KB; FP64: half) or the velocity-only layout,
 // whichever is larger; the kernels' limit is raised to it.
 template <typename R> int setup_solve_lds(rbh_ctx* c) {
-  const int scale = sizeof(R) == 4 ? 2 : 1;
-  c->res_cap_c = c->tune.resident_contacts > 0 ? c->tune.resident_contacts : 192 * scale;
-  c->res_cap_b = c->tune.resident_bodies > 0 ? c->tune.resident_bodies : 192 * scale;
-  c->res_cap_o = 4 * c->res_cap_c;
-  size_t bytes = std::max<size_t>(SOLVE_LDS_BYTES, res_lds_bytes<R>(c->res_cap_c, c->res_cap_b, c->res_cap_o,
-                                                                    RES_LANES));
+  c->res_cap_c = c->tune.resident_contacts > 0 ? std::min(c->tune.resident_contacts, ResCap<R>::C) : ResCap<R>::C;
+  c->res_cap_b = c->tune.resident_bodies > 0 ? std::min(c->tune.resident_bodies, ResCap<R>::B) : ResCap<R>::B;
+  c->res_cap_o = ResCap<R>::O;
+  size_t bytes = std::max<size_t>(SOLVE_LDS_BYTES, ResCap<R>::BYTES);
   const void* ks[2] = {reinterpret_cast<const void*>(k_solve<R>), reinterpret_cast<const void*>(k_solve_multi<R>)};
   for (const void* k : ks)
     if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
@@ -1261,12 +1259,9 @@ int rbh_set_tuning(rbh_ctx* c, const rbh_tuning* t) {
       t->resident_contacts < 0 || t->resident_bodies < 0 || t->lanes_per_group < 0 || t->lanes_per_group > 64 ||
       (t->resident_contacts % 64) || (t->resident_bodies % 64))
     return fail(c, RBH_E_ARG, "tuning values out of range (resident capacities are multiples of 64)");
-  const int cc = t->resident_contacts, cb = t->resident_bodies;
-  const size_t bytes = c->prec == 4 ? res_lds_bytes<float>(cc, cb, 4 * cc, RES_LANES)
-                                    : res_lds_bytes<double>(cc, cb, 4 * cc, RES_LANES);
-  if (bytes > (size_t)c->prop.sharedMemPerBlock)
-    return fail(c, RBH_E_ARG, "resident capacities need " + std::to_string(bytes) + " bytes of LDS, more than " +
-                                  std::to_string(c->prop.sharedMemPerBlock));
+  const int cap = c->prec == 4 ? ResCap<float>::C : ResCap<double>::C;
+  if (t->resident_contacts > cap || t->resident_bodies > cap)
+    return fail(c, RBH_E_ARG, "resident capacities are at most " + std::to_string(cap) + " contacts and bodies");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->tune = *t;

@@ -1027,25 +1027,38 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
 // quad exchanges values with DPP quad broadcasts and every lane repeats the
 // serial update's operations on the same operands in the same order, so the
 // results are bitwise those of update_contact.
+// Body arrays hold cb + 1 entries: slot cb stays zero and stands for every
+// fixed body, so a contact's loads never branch on fixed bodies.
 template <typename R> __host__ __device__ constexpr size_t res_lds_bytes(int cc, int cb, int co, int cl) {
-  return (size_t)cc * (12 * sizeof(V4<R>) + 8 * sizeof(R) + sizeof(int2)) + (size_t)cb * 6 * sizeof(V4<R>) +
+  return (size_t)cc * (12 * sizeof(V4<R>) + 8 * sizeof(R) + sizeof(int2)) + (size_t)(cb + 1) * 6 * sizeof(V4<R>) +
          (size_t)co * 4 + (size_t)cl * 16;
 }
 
 constexpr int RES_LANES = 64;   // lanes of a resident group
 
+// LDS capacities of the resident layout (compile-time, so array offsets fold
+// into the LDS instructions); Dev::res_cap_c/b may set lower fit limits
+template <typename R> struct ResCap {
+  static constexpr int C = sizeof(R) == 4 ? 384 : 192;   // contacts
+  static constexpr int B = C;                             // bodies
+  static constexpr int O = 4 * C;                         // colour offsets
+  static constexpr size_t BYTES = res_lds_bytes<R>(C, B, O, RES_LANES);
+};
+
 template <typename R> struct ResLds {
   V4<R> *JL, *JA, *JB;                     // per contact row (4 per contact)
   R *x0, *x1;                              // per contact row
   V4<R> *wl, *wa, *sl, *sa, *W0, *W1;      // per body (slot - b_lo)
-  int2* loc;                               // a contact's two bodies (slot - b_lo; -1: fixed)
+  int2* loc;                               // a contact's two bodies (slot - b_lo; zs: fixed)
   int4* lt;                                // per lane: first contact, K, colours, first colour offset
   int* coff;
-  __device__ ResLds(unsigned char* p, int cc, int cb) {
+  static constexpr int zs = ResCap<R>::B;  // the zero slot
+  __device__ explicit ResLds(unsigned char* p) {
+    constexpr int cc = ResCap<R>::C, nbs = ResCap<R>::B + 1;
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
     JL = v; JA = JL + 4 * cc; JB = JA + 4 * cc;
-    wl = JB + 4 * cc; wa = wl + cb; sl = wa + cb; sa = sl + cb; W0 = sa + cb; W1 = W0 + cb;
-    x0 = reinterpret_cast<R*>(W1 + cb); x1 = x0 + 4 * cc;
+    wl = JB + 4 * cc; wa = wl + nbs; sl = wa + nbs; sa = sl + nbs; W0 = sa + nbs; W1 = W0 + nbs;
+    x0 = reinterpret_cast<R*>(W1 + nbs); x1 = x0 + 4 * cc;
     loc = reinterpret_cast<int2*>(x1 + 4 * cc);
     lt = reinterpret_cast<int4*>(loc + cc);
     coff = reinterpret_cast<int*>(lt + RES_LANES);
@@ -1072,7 +1085,7 @@ template <typename R> __device__ GroupSize group_size(const Dev<R>& d, const Gro
 }
 
 template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d, const Group& gr, const GroupSize& g) {
-  return g.nc <= d.res_cap_c && gr.b_hi - gr.b_lo <= d.res_cap_b && g.no <= d.res_cap_o && g.nl <= RES_LANES;
+  return g.nc <= d.res_cap_c && gr.b_hi - gr.b_lo <= d.res_cap_b && g.no <= ResCap<R>::O && g.nl <= RES_LANES;
 }
 
 // value of lane K of each quad (DPP quad_perm(K, K, K, K))
@@ -1102,11 +1115,10 @@ __device__ __forceinline__ double res_update(const SceneParams& sp, ResLds<R>& L
   const V4<R>* src = ang ? (side_b ? L.JB : L.JA) : L.JL;
   const V4<R> v0 = src[4 * i], v1 = src[4 * i + 1], v2 = src[4 * i + 2], v3 = src[4 * i + 3];
   V4<R>* dst = ang ? L.wa : L.wl;
-  const V4<R> z4 = {R(0), R(0), R(0), R(0)};
-  const bool fa = lc.x >= 0, fb = lc.y >= 0, own = slot >= 0;
-  const V4<R> lA = fa ? rl[lc.x] : z4, aA = fa ? ra[lc.x] : z4;
-  const V4<R> lB = fb ? rl[lc.y] : z4, aB = fb ? ra[lc.y] : z4;
-  const V4<R> W0 = own ? L.W0[slot] : z4, W1 = own ? L.W1[slot] : z4, old = own ? dst[slot] : z4;
+  const bool own = slot != L.zs;
+  // fixed bodies read the zero slot (the serial update uses zeros for them)
+  const V4<R> lA = rl[lc.x], aA = ra[lc.x], lB = rl[lc.y], aB = ra[lc.y];
+  const V4<R> W0 = L.W0[slot], W1 = L.W1[slot], old = dst[slot];
   // row q of z = x - r (J w + b)  (contacts.py:77-92)
   const R Jw = -dot(xyz(jl), xyz(lA)) - dot(xyz(ja), xyz(aA)) + dot(xyz(jl), xyz(lB)) + dot(xyz(jb), xyz(aB));
   const R z = xo - (ja.w * rs) * (Jw + jl.w);
@@ -1178,7 +1190,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   const bool serial = sp.scheme == SCHEME_GS;
   const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
-  ResLds<R> L(smem, d.res_cap_c, d.res_cap_b);
+  ResLds<R> L(smem);
   const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
   // stage: lane table, colour offsets, contact rows, bodies
   if (threadIdx.x == 0) {
@@ -1218,12 +1230,16 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
       L.x1[4 * i + r] = x1v[r];
     }
     const int2 lc = d.s_loc[pos];
-    L.loc[i] = make_int2(lc.x >= 0 ? lc.x - b0 : -1, lc.y >= 0 ? lc.y - b0 : -1);
+    L.loc[i] = make_int2(lc.x >= 0 ? lc.x - b0 : L.zs, lc.y >= 0 ? lc.y - b0 : L.zs);
   }
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     const int b = d.u_blist[b0 + i];
     L.wl[i] = d.ww0[b]; L.wa[i] = d.ww1[b]; L.sl[i] = d.ws0[b]; L.sa[i] = d.ws1[b];
     L.W0[i] = d.b_W0[b]; L.W1[i] = d.b_W1[b];
+  }
+  if (threadIdx.x == 0) {
+    const V4<R> z = {R(0), R(0), R(0), R(0)};
+    L.wl[L.zs] = z; L.wa[L.zs] = z; L.sl[L.zs] = z; L.sa[L.zs] = z; L.W0[L.zs] = z; L.W1[L.zs] = z;
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
@@ -1262,7 +1278,6 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   // impulses to contact order; natural-map residual (FP64) with r0; accepted velocity changes out
   const R* xs = ctl.acc ? L.x1 : L.x0;
   double rloc = 0.0, xmax = 0.0;
-  const V4<R> z = {R(0), R(0), R(0), R(0)};
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
     const V4<R> x = {xs[4 * i], xs[4 * i + 1], xs[4 * i + 2], xs[4 * i + 3]};
     d.c_imp[d.perm[gs.p0 + i]] = x;
@@ -1277,8 +1292,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     }
     const V4<R> mu = {L.JB[4 * i + 1].w, L.JB[4 * i + 2].w, L.JB[4 * i + 3].w, R(0)};
     const int2 lc = L.loc[i];
-    const V4<R> lA = lc.x >= 0 ? L.sl[lc.x] : z, aA = lc.x >= 0 ? L.sa[lc.x] : z;
-    const V4<R> lB = lc.y >= 0 ? L.sl[lc.y] : z, aB = lc.y >= 0 ? L.sa[lc.y] : z;
+    const V4<R> lA = L.sl[lc.x], aA = L.sa[lc.x], lB = L.sl[lc.y], aB = L.sa[lc.y];
     const V4<R> y = contact_prox(sp, J, x, bb, r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
     rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
                               (double)x.w - (double)y.w));

@@ -87,7 +87,7 @@ def test_tuning_rejects_bad_values():
         with pytest.raises(Exception):
             ctx.set_tuning(resident_contacts=100)          # not a multiple of 64
         with pytest.raises(Exception):
-            ctx.set_tuning(resident_contacts=1 << 16, resident_bodies=1 << 16)   # beyond LDS
+            ctx.set_tuning(resident_contacts=1 << 16, resident_bodies=1 << 16)   # beyond the LDS layout
     finally:
         ctx.close()
 

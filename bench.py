@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=1024)
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--serving-rate", type=float, default=200.0)
+    ap.add_argument("--only", choices=("batched", "serving"), default=None,
+                    help="run one of the extra legs alone (for profiling) and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the step of one scene (or of one batch of scenes) is measured on "
@@ -226,6 +228,12 @@ def main():
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.only == "batched":
+        print(json.dumps(batched_leg(args.batched_scenes, 20, 3, 64, 2)), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps(serving_leg(args.serving_requests, args.serving_rate, 64, 3)), flush=True)
         return
 
     from rainbow_amd import native, scenes

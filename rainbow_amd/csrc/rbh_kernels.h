@@ -467,13 +467,20 @@ template <typename R> __global__ void k_cand_write(Dev<R> d) {
 
 // ========================================================== Frank-Wolfe
 // collision_detection.py:120-188 per candidate triangle (one lane each).
+// Each wavefront takes a contiguous run of candidates (pairs, and so scenes,
+// are contiguous), so a lane's scene changes only at scene boundaries and the
+// per-scene counters are merged once per wavefront.
 template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
   if (aborted(d.err)) return;
   const int nc = *d.n_cand;
   unsigned long long evals = 0;
   int ncand = 0;
   int cur = -1;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+  const long long nwaves = (long long)gridDim.x * (blockDim.x >> 6);
+  const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long long chunk = ((nc + nwaves - 1) / nwaves + 63) / 64 * 64;
+  const int c_lo = (int)min<long long>(wave * chunk, nc), c_hi = (int)min<long long>(wave * chunk + chunk, nc);
+  for (int c = c_lo + (int)(threadIdx.x & 63); c < c_hi; c += 64) {
     int p = d.cand_pair[c];
     int tw = d.cand_tri[c];
     int dir = (tw >> 31) & 1;
@@ -597,25 +604,49 @@ template <typename R> __global__ void k_contact_write(Dev<R> d) {
     if (nk > d.cap_contacts) { atomicOr(d.err, ERR_CAP_CONTACTS); atomicMax(d.err + 4, nk); }
   }
   if (nk > d.cap_contacts) return;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-    if (d.cand_keep[c] != 1) continue;
-    int k = d.keep_scan[c];
-    int p = d.cand_pair[c];
-    int dir = (d.cand_tri[c] >> 31) & 1;
-    int2 ab = d.pairs[p];
-    int own = dir ? ab.y : ab.x;
-    int sdf = dir ? ab.x : ab.y;
-    d.c_ia[k] = sdf;                   // ContactPoint(bodyB, bodyA, ...) at collision_detection.py:187
-    d.c_ib[k] = own;
-    d.c_tri[k] = d.cand_tri[c] & 0x7fffffff;
-    d.c_pair[k] = p;
-    d.c_p[k] = d.cand_p[c];
-    d.c_n[k] = d.cand_n[c];
-    int slot = d.b_slot[sdf];
-    atomicMin(&d.ss[slot].k_off, k);
-    atomicAdd(&d.ss[slot].k_cnt, 1);
-    R pen = -d.cand_p[c].w;
-    if (pen > R(0)) atomic_max_nonneg(&d.ss[slot].max_pen, (double)pen);
+  const int lane = threadIdx.x & 63;
+  // whole wavefronts per iteration, so the per-scene first index, count and
+  // largest penetration are merged across the wavefront before the atomics
+  for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < nc; base += gridDim.x * blockDim.x) {
+    const int c = base + lane;
+    int slot = -1, k = 0x7fffffff;
+    double pen = 0.0;
+    if (c < nc && d.cand_keep[c] == 1) {
+      k = d.keep_scan[c];
+      int p = d.cand_pair[c];
+      int dir = (d.cand_tri[c] >> 31) & 1;
+      int2 ab = d.pairs[p];
+      int own = dir ? ab.y : ab.x;
+      int sdf = dir ? ab.x : ab.y;
+      d.c_ia[k] = sdf;                   // ContactPoint(bodyB, bodyA, ...) at collision_detection.py:187
+      d.c_ib[k] = own;
+      d.c_tri[k] = d.cand_tri[c] & 0x7fffffff;
+      d.c_pair[k] = p;
+      d.c_p[k] = d.cand_p[c];
+      d.c_n[k] = d.cand_n[c];
+      slot = d.b_slot[sdf];
+      pen = fmax(0.0, -(double)d.cand_p[c].w);
+    }
+    unsigned long long live = __ballot(slot >= 0);
+    while (live) {
+      const int leader = __ffsll((long long)live) - 1;
+      const int s0 = __shfl(slot, leader, 64);
+      const bool mine = slot == s0;
+      int kmin = mine ? k : 0x7fffffff, cnt = mine ? 1 : 0;
+      double pm = mine ? pen : 0.0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        kmin = min(kmin, __shfl_xor(kmin, o, 64));
+        cnt += __shfl_xor(cnt, o, 64);
+        pm = fmax(pm, __shfl_xor(pm, o, 64));
+      }
+      if (lane == leader) {
+        atomicMin(&d.ss[s0].k_off, kmin);
+        atomicAdd(&d.ss[s0].k_cnt, cnt);
+        if (pm > 0.0) atomic_max_nonneg(&d.ss[s0].max_pen, pm);
+      }
+      live &= ~__ballot(mine);
+    }
   }
 }
 

@@ -121,6 +121,7 @@ struct rbh_ctx {
       u_bs, body_claim, body_loc, bc_cnt, bc_off, bc_fill, bc_list, pal, g_pc, g_done, col_off, merit, bar;
   DBuf u_T, isl_unit, u_nbody, u_boff, f_loc, f_unit, u_blist, s_loc, s_W;
   DBuf l_cnt, l_base, n_lanes, l_start, l_end, l_ncol, l_bs;
+  DBuf s_JL, s_JA, s_JB, xr0, xr1;
   DBuf perm, s_id, s_n, s_rA, s_rB, s_b, s_r0, x0, x1, errbuf, scan_part;
   int cap_pairs = 0, cap_slots = 0, cap_cand = 0, cap_contacts = 0, cap_units = 0;
   int unit_T = 64;          // contacts per colouring unit (a scene's units are windows of this size)
@@ -196,7 +197,8 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->g_pc, &c->g_done, &c->col_off, &c->merit, &c->bar, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
           &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part, &c->u_T, &c->isl_unit,
           &c->u_nbody, &c->u_boff, &c->f_loc, &c->f_unit, &c->u_blist, &c->s_loc, &c->s_W, &c->l_cnt,
-          &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs};
+          &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs, &c->s_JL, &c->s_JA, &c->s_JB,
+          &c->xr0, &c->xr1};
 }
 
 size_t total_bytes(rbh_ctx* c) {
@@ -374,6 +376,8 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.isl_unit = P(int*, isl_unit); d.u_nbody = P(int*, u_nbody); d.u_boff = P(int*, u_boff);
   d.f_loc = P(int*, f_loc); d.f_unit = P(int*, f_unit); d.u_blist = P(int*, u_blist);
   d.s_loc = P(int2*, s_loc); d.s_W = P(V4<R>*, s_W);
+  d.s_JL = P(V4<R>*, s_JL); d.s_JA = P(V4<R>*, s_JA); d.s_JB = P(V4<R>*, s_JB);
+  d.xr0 = P(R*, xr0); d.xr1 = P(R*, xr1);
   d.res_cap_c = c->res_cap_c; d.res_cap_b = c->res_cap_b; d.res_cap_o = c->res_cap_o;
   d.l_cnt = P(int*, l_cnt); d.l_base = P(int*, l_base); d.n_lanes = P(int*, n_lanes);
   d.l_start = P(int*, l_start); d.l_end = P(int*, l_end); d.l_ncol = P(int*, l_ncol); d.l_bs = P(int*, l_bs);
@@ -422,6 +426,8 @@ int size_workspace(rbh_ctx* c) {
       {&c->f_loc, nb * 4}, {&c->f_unit, nb * 4}, {&c->u_blist, nb * 4}, {&c->s_loc, ck * 8},
       {&c->s_W, 4 * ck * V}, {&c->l_cnt, ns * 4}, {&c->l_base, (ns + 1) * 4}, {&c->n_lanes, 16},
       {&c->l_start, nu * 4}, {&c->l_end, nu * 4}, {&c->l_ncol, nu * 4}, {&c->l_bs, nu * 4},
+      {&c->s_JL, 4 * ck * V}, {&c->s_JA, 4 * ck * V}, {&c->s_JB, 4 * ck * V}, {&c->xr0, 4 * ck * c->prec},
+      {&c->xr1, 4 * ck * c->prec},
       {&c->bar, 64},
       {&c->perm, ck * 4}, {&c->s_id, ck * 8}, {&c->s_n, ck * V}, {&c->s_rA, ck * V}, {&c->s_rB, ck * V},
       {&c->s_b, ck * V}, {&c->s_r0, ck * V}, {&c->x0, ck * V}, {&c->x1, ck * V}, {&c->errbuf, 64},

@@ -112,6 +112,8 @@ template <typename R> struct Dev {
   int2* s_loc;                         // body slots (u_blist index) of the two bodies (-1: fixed)
   int res_cap_c, res_cap_b, res_cap_o; // LDS capacities of a resident solve group (contacts, bodies, colour offsets)
   V4<R>* s_W;                          // 4 per position: W0, W1 of body A, then of body B (zero if fixed)
+  V4<R>* s_JL; V4<R>* s_JA; V4<R>* s_JB;   // rows 4 pos + r of J: (lin_r, b_r), (angA_r, r0_r), (angB_r, mu_r)
+  R* xr0; R* xr1;                      // iterates by row (4 pos + r), for the row-major sweeps
   int* err;                           // [0] flags, [1..] needed capacities
   double dt;                           // > 0: use for every scene, else the scene's time_step
   int flags;

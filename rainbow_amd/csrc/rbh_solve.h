@@ -536,6 +536,20 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color_uni
   color_unit<R>(d, u, s);
 }
 
+template <typename R> struct ContactRow {
+  V3<R> lin[4], angA[4], angB[4];
+};
+
+template <typename R>
+__device__ __forceinline__ void contact_rows(V4<R> n4, V4<R> rA4, V4<R> rB4, ContactRow<R>& J) {
+  V3<R> s, t, n;
+  orthonormal_frame(xyz(n4), s, t, n);
+  V3<R> rA = xyz(rA4), rB = xyz(rB4);
+  J.lin[0] = n; J.lin[1] = s; J.lin[2] = t; J.lin[3] = V3<R>{R(0), R(0), R(0)};
+  J.angA[0] = cross(rA, n); J.angA[1] = cross(rA, s); J.angA[2] = cross(rA, t); J.angA[3] = n;
+  J.angB[0] = cross(rB, n); J.angB[1] = cross(rB, s); J.angB[2] = cross(rB, t); J.angB[3] = n;
+}
+
 // ------------------------------------------------------------ sweep order
 // Gather one contact into the solver arrays at position `pos`; s_loc holds
 // the bodies' slots (index into u_blist; -1 for a fixed body).
@@ -563,6 +577,19 @@ template <typename R> __device__ __forceinline__ void gather_contact(const Dev<R
   W[3] = fb ? d.b_W1[B] : z;
   d.x0[pos] = z;
   d.x1[pos] = z;
+  // the rows of J once per step, for the row-major (quad) sweeps
+  ContactRow<R> J;
+  contact_rows(V4<R>{n.x, n.y, n.z, R(0)}, rA, rB, J);
+  const V4<R> bb = d.c_b[k], r0 = d.c_r0[k];
+  const R bv[4] = {bb.x, bb.y, bb.z, bb.w}, rv[4] = {r0.x, r0.y, r0.z, r0.w}, mv[4] = {R(0), mu.x, mu.y, mu.z};
+  for (int r = 0; r < 4; ++r) {
+    const size_t row = 4 * (size_t)pos + r;
+    d.s_JL[row] = V4<R>{J.lin[r].x, J.lin[r].y, J.lin[r].z, bv[r]};
+    d.s_JA[row] = V4<R>{J.angA[r].x, J.angA[r].y, J.angA[r].z, rv[r]};
+    d.s_JB[row] = V4<R>{J.angB[r].x, J.angB[r].y, J.angB[r].z, mv[r]};
+    d.xr0[row] = R(0);
+    d.xr1[row] = R(0);
+  }
 }
 
 // colour offsets of lane l: 2 K_l + 2 entries (an island's colours stay below
@@ -649,20 +676,6 @@ constexpr int SOLVE_LDS_BYTES = 65536 - 512;
 // bodies whose velocity changes fit the velocity-only layout
 template <typename R> __host__ __device__ constexpr int lds_body_cap() {
   return SOLVE_LDS_BYTES / (4 * (int)sizeof(V4<R>));
-}
-
-template <typename R> struct ContactRow {
-  V3<R> lin[4], angA[4], angB[4];
-};
-
-template <typename R>
-__device__ __forceinline__ void contact_rows(V4<R> n4, V4<R> rA4, V4<R> rB4, ContactRow<R>& J) {
-  V3<R> s, t, n;
-  orthonormal_frame(xyz(n4), s, t, n);
-  V3<R> rA = xyz(rA4), rB = xyz(rB4);
-  J.lin[0] = n; J.lin[1] = s; J.lin[2] = t; J.lin[3] = V3<R>{R(0), R(0), R(0)};
-  J.angA[0] = cross(rA, n); J.angA[1] = cross(rA, s); J.angA[2] = cross(rA, t); J.angA[3] = n;
-  J.angB[0] = cross(rB, n); J.angB[1] = cross(rB, s); J.angB[2] = cross(rB, t); J.angB[3] = n;
 }
 
 // contacts.py:77-95: z = x - r (J w + b); x_n = max(0, z_n); friction prox
@@ -1019,14 +1032,133 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
   }
 }
 
+// value of lane K of each quad (DPP quad_perm(K, K, K, K))
+template <int K> __device__ __forceinline__ float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xf, 0xf, false));
+}
+template <int K> __device__ __forceinline__ double quad_bcast(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), K * 0x55, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), K * 0x55, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// One lane's share of a contact for the quad update: row q of J (lin, angA,
+// angB with b, r0, mu in w), the rows 0..3 of the array this lane's velocity
+// block is built from, its iterate and the contact's two body slots.
+template <typename R> struct QuadRow {
+  V4<R> jl, ja, jb, v0, v1, v2, v3;
+  R xo;
+  int2 lc;
+};
+
+// Velocity changes and inverse masses of a group's bodies in LDS (slot
+// zs stays zero and stands for every fixed body).
+template <typename R> struct BodyLds {
+  V4<R> *wl, *wa, *sl, *sa, *W0, *W1;
+};
+
+// One contact by one quad; lane q holds row q.  Every lane repeats the serial
+// update's operations (update_contact) on the same operands in the same
+// order, so the results are bitwise the same.  Writes this lane's velocity
+// block; returns the new iterate in xq and |xq - xo| in FP64.
+template <typename R>
+__device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow<R>& it, int q, R rs, bool jacobi,
+                                            bool hybrid, const BodyLds<R>& B, int zs, R& xq) {
+  const V4<R>* rl = jacobi ? B.sl : B.wl;
+  const V4<R>* ra = jacobi ? B.sa : B.wa;
+  const bool side_b = q >= 2, ang = q & 1;   // lane 0: linear A, 1: angular A, 2: linear B, 3: angular B
+  const int slot = side_b ? it.lc.y : it.lc.x;
+  V4<R>* dst = ang ? B.wa : B.wl;
+  const bool own = slot != zs;
+  // fixed bodies read the zero slot (the serial update uses zeros for them)
+  const V4<R> lA = rl[it.lc.x], aA = ra[it.lc.x], lB = rl[it.lc.y], aB = ra[it.lc.y];
+  const V4<R> W0 = B.W0[slot], W1 = B.W1[slot], old = dst[slot];
+  const V4<R> jl = it.jl, ja = it.ja, jb = it.jb;
+  const R xo = it.xo;
+  // row q of z = x - r (J w + b)  (contacts.py:77-92)
+  const R Jw = -dot(xyz(jl), xyz(lA)) - dot(xyz(ja), xyz(aA)) + dot(xyz(jl), xyz(lB)) + dot(xyz(jb), xyz(aB));
+  const R z = xo - (ja.w * rs) * (Jw + jl.w);
+  const R z0 = quad_bcast<0>(z), z1 = quad_bcast<1>(z), z2 = quad_bcast<2>(z), z3 = quad_bcast<3>(z);
+  const R xo0 = quad_bcast<0>(xo);
+  const R mu0 = quad_bcast<1>(jb.w), mu1 = quad_bcast<2>(jb.w), mu2 = quad_bcast<3>(jb.w);
+  // projections (contacts.py:92-95), the same in every lane of the quad
+  const R xnrm = fmax(R(0), z0);
+  const R xr = (jacobi && !hybrid) ? xo0 : xnrm;
+  R os, ot, orr;
+  if (sp.friction == 1) {
+    R o[3];
+    prox_ellipsoid_call<R>(z1, z2, z3, mu0, mu1, mu2, xr, sp.ell_tol, sp.ell_expansion, sp.ell_max_iter, o);
+    os = o[0]; ot = o[1]; orr = o[2];
+  } else {
+    prox_sphere<R>(z1, z2, z3, mu0, xr, os, ot, orr);
+  }
+  xq = q == 0 ? xnrm : (q == 1 ? os : (q == 2 ? ot : orr));
+  const R dq = xq - xo;
+  const R d0 = quad_bcast<0>(dq), d1 = quad_bcast<1>(dq), d2 = quad_bcast<2>(dq), d3 = quad_bcast<3>(dq);
+  // w += W J^T dx (contacts.py:107), this lane's block
+  if (own) {
+    const R sign = side_b ? R(1) : R(-1);
+    const R s0 = sign * d0, s1 = sign * d1, s2 = sign * d2, s3 = sign * d3;
+    V3<R> acc = {R(0), R(0), R(0)};
+    acc.x = acc.x + it.v0.x * s0; acc.y = acc.y + it.v0.y * s0; acc.z = acc.z + it.v0.z * s0;
+    acc.x = acc.x + it.v1.x * s1; acc.y = acc.y + it.v1.y * s1; acc.z = acc.z + it.v1.z * s1;
+    acc.x = acc.x + it.v2.x * s2; acc.y = acc.y + it.v2.y * s2; acc.z = acc.z + it.v2.z * s2;
+    acc.x = acc.x + it.v3.x * s3; acc.y = acc.y + it.v3.y * s3; acc.z = acc.z + it.v3.z * s3;
+    V3<R> t;
+    if (ang) {
+      const R W[6] = {W0.x, W0.y, W0.z, W0.w, W1.x, W1.y};
+      t = sym_mul(W, acc);
+    } else {
+      t = V3<R>{W1.z * acc.x, W1.z * acc.y, W1.z * acc.z};
+    }
+    dst[slot] = V4<R>{old.x + t.x, old.y + t.y, old.z + t.z, R(0)};
+  }
+  return fabs((double)xq - (double)xo);
+}
+
+// lane q's share of contact i from row arrays (LDS or global); slots made local
+template <typename R>
+__device__ __forceinline__ void load_quad(const V4<R>* JL, const V4<R>* JA, const V4<R>* JB, const R* xa,
+                                          const int2* loc, int i, int q, int b0, int zs, QuadRow<R>& it) {
+  const size_t row = 4 * (size_t)i + q;
+  it.jl = JL[row]; it.ja = JA[row]; it.jb = JB[row];
+  it.xo = xa[row];
+  const int2 lc = loc[i];
+  it.lc = make_int2(lc.x >= 0 ? lc.x - b0 : zs, lc.y >= 0 ? lc.y - b0 : zs);
+  const V4<R>* src = (q & 1) ? (q >= 2 ? JB : JA) : JL;
+  it.v0 = src[4 * (size_t)i]; it.v1 = src[4 * (size_t)i + 1]; it.v2 = src[4 * (size_t)i + 2];
+  it.v3 = src[4 * (size_t)i + 3];
+}
+
+// natural-map residual parts of contact i (FP64) from its rows and the
+// accepted velocity changes; the impulses' final values
+template <typename R>
+__device__ __forceinline__ void quad_finish_contact(const SceneParams& sp, const V4<R>* JL, const V4<R>* JA,
+                                                    const V4<R>* JB, const R* xs, int i, const V4<R>& lA,
+                                                    const V4<R>& aA, const V4<R>& lB, const V4<R>& aB, V4<R>& x,
+                                                    double& rloc, double& xmax) {
+  x = V4<R>{xs[4 * (size_t)i], xs[4 * (size_t)i + 1], xs[4 * (size_t)i + 2], xs[4 * (size_t)i + 3]};
+  ContactRow<R> J;
+  V4<R> bb, r0;
+  R* bp = &bb.x;
+  R* rp = &r0.x;
+  for (int r = 0; r < 4; ++r) {
+    const V4<R> a = JL[4 * (size_t)i + r], e = JA[4 * (size_t)i + r], f = JB[4 * (size_t)i + r];
+    J.lin[r] = xyz(a); J.angA[r] = xyz(e); J.angB[r] = xyz(f);
+    bp[r] = a.w; rp[r] = e.w;
+  }
+  const V4<R> mu = {JB[4 * (size_t)i + 1].w, JB[4 * (size_t)i + 2].w, JB[4 * (size_t)i + 3].w, R(0)};
+  const V4<R> y = contact_prox(sp, J, x, bb, r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
+  rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
+                            (double)x.w - (double)y.w));
+  xmax = fmax(xmax, absmax4(x.x, x.y, x.z, x.w));
+}
+
 // ---- resident layout
-// Per contact i the four rows r of J are staged once (row 4i + r):
-//   JL = (lin_r, b_r), JA = (angA_r, r0_r), JB = (angB_r, mu_r) with mu_1..3 =
-//   the friction coefficients (mu_0 unused), and the iterates X0/X1[4i + r].
-// A contact is updated by a quad of four lanes, lane q holding row q; the
-// quad exchanges values with DPP quad broadcasts and every lane repeats the
-// serial update's operations on the same operands in the same order, so the
-// results are bitwise those of update_contact.
+// Per contact i the four rows r of J (row 4i + r: JL = (lin_r, b_r),
+// JA = (angA_r, r0_r), JB = (angB_r, mu_r), mu_1..3 the friction
+// coefficients) and the iterates X0/X1[4i + r] are staged in LDS once per
+// solve; so are the group's bodies.  A sweep issues no global memory access.
 // Body arrays hold cb + 1 entries: slot cb stays zero and stands for every
 // fixed body, so a contact's loads never branch on fixed bodies.
 template <typename R> __host__ __device__ constexpr size_t res_lds_bytes(int cc, int cb, int co, int cl) {
@@ -1043,25 +1175,38 @@ template <typename R> struct ResCap {
   static constexpr int B = C;                             // bodies
   static constexpr int O = 4 * C;                         // colour offsets
   static constexpr size_t BYTES = res_lds_bytes<R>(C, B, O, RES_LANES);
+  // the streaming layout keeps only bodies in the same LDS
+  static constexpr int VB = (int)(BYTES / (6 * sizeof(V4<R>))) - 1;
 };
 
 template <typename R> struct ResLds {
   V4<R> *JL, *JA, *JB;                     // per contact row (4 per contact)
   R *x0, *x1;                              // per contact row
-  V4<R> *wl, *wa, *sl, *sa, *W0, *W1;      // per body (slot - b_lo)
+  BodyLds<R> B;                            // per body (slot - b_lo), zero slot zs
   int2* loc;                               // a contact's two bodies (slot - b_lo; zs: fixed)
   int4* lt;                                // per lane: first contact, K, colours, first colour offset
   int* coff;
-  static constexpr int zs = ResCap<R>::B;  // the zero slot
+  static constexpr int zs = ResCap<R>::B;
   __device__ explicit ResLds(unsigned char* p) {
     constexpr int cc = ResCap<R>::C, nbs = ResCap<R>::B + 1;
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
     JL = v; JA = JL + 4 * cc; JB = JA + 4 * cc;
-    wl = JB + 4 * cc; wa = wl + nbs; sl = wa + nbs; sa = sl + nbs; W0 = sa + nbs; W1 = W0 + nbs;
-    x0 = reinterpret_cast<R*>(W1 + nbs); x1 = x0 + 4 * cc;
+    B.wl = JB + 4 * cc; B.wa = B.wl + nbs; B.sl = B.wa + nbs; B.sa = B.sl + nbs; B.W0 = B.sa + nbs; B.W1 = B.W0 + nbs;
+    x0 = reinterpret_cast<R*>(B.W1 + nbs); x1 = x0 + 4 * cc;
     loc = reinterpret_cast<int2*>(x1 + 4 * cc);
     lt = reinterpret_cast<int4*>(loc + cc);
     coff = reinterpret_cast<int*>(lt + RES_LANES);
+  }
+};
+
+// ---- streaming layout: bodies in LDS, contact rows and iterates in global memory
+template <typename R> struct StreamLds {
+  BodyLds<R> B;
+  static constexpr int zs = ResCap<R>::VB;
+  __device__ explicit StreamLds(unsigned char* p) {
+    constexpr int nbs = ResCap<R>::VB + 1;
+    V4<R>* v = reinterpret_cast<V4<R>*>(p);
+    B.wl = v; B.wa = B.wl + nbs; B.sl = B.wa + nbs; B.sa = B.sl + nbs; B.W0 = B.sa + nbs; B.W1 = B.W0 + nbs;
   }
 };
 
@@ -1088,86 +1233,11 @@ template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d
   return g.nc <= d.res_cap_c && gr.b_hi - gr.b_lo <= d.res_cap_b && g.no <= ResCap<R>::O && g.nl <= RES_LANES;
 }
 
-// value of lane K of each quad (DPP quad_perm(K, K, K, K))
-template <int K> __device__ __forceinline__ float quad_bcast(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xf, 0xf, false));
-}
-template <int K> __device__ __forceinline__ double quad_bcast(double v) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), K * 0x55, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), K * 0x55, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
-}
-
-// One contact by one quad; lane q holds row q.  rl/ra: velocity changes the
-// rows read (working for Gauss-Seidel, accepted for Jacobi).  Returns
-// |x_q - xo_q| in FP64.
-template <typename R>
-__device__ __forceinline__ double res_update(const SceneParams& sp, ResLds<R>& L, const V4<R>* rl, const V4<R>* ra,
-                                             const R* xa, R* xn, int i, int q, R rs, bool jacobi, bool hybrid) {
-  // every load first (none depends on this contact's results; the only
-  // writes come at the end, to this contact's own iterate and bodies)
-  const int row = 4 * i + q;
-  const int2 lc = L.loc[i];
-  const V4<R> jl = L.JL[row], ja = L.JA[row], jb = L.JB[row];
-  const R xo = xa[row];
-  const bool side_b = q >= 2, ang = q & 1;   // lane 0: linear A, 1: angular A, 2: linear B, 3: angular B
-  const int slot = side_b ? lc.y : lc.x;
-  const V4<R>* src = ang ? (side_b ? L.JB : L.JA) : L.JL;
-  const V4<R> v0 = src[4 * i], v1 = src[4 * i + 1], v2 = src[4 * i + 2], v3 = src[4 * i + 3];
-  V4<R>* dst = ang ? L.wa : L.wl;
-  const bool own = slot != L.zs;
-  // fixed bodies read the zero slot (the serial update uses zeros for them)
-  const V4<R> lA = rl[lc.x], aA = ra[lc.x], lB = rl[lc.y], aB = ra[lc.y];
-  const V4<R> W0 = L.W0[slot], W1 = L.W1[slot], old = dst[slot];
-  // row q of z = x - r (J w + b)  (contacts.py:77-92)
-  const R Jw = -dot(xyz(jl), xyz(lA)) - dot(xyz(ja), xyz(aA)) + dot(xyz(jl), xyz(lB)) + dot(xyz(jb), xyz(aB));
-  const R z = xo - (ja.w * rs) * (Jw + jl.w);
-  const R z0 = quad_bcast<0>(z), z1 = quad_bcast<1>(z), z2 = quad_bcast<2>(z), z3 = quad_bcast<3>(z);
-  const R xo0 = quad_bcast<0>(xo);
-  const R mu0 = quad_bcast<1>(jb.w), mu1 = quad_bcast<2>(jb.w), mu2 = quad_bcast<3>(jb.w);
-  // projections (contacts.py:92-95), the same in every lane of the quad
-  const R xnrm = fmax(R(0), z0);
-  const R xr = (jacobi && !hybrid) ? xo0 : xnrm;
-  R os, ot, orr;
-  if (sp.friction == 1) {
-    R o[3];
-    prox_ellipsoid_call<R>(z1, z2, z3, mu0, mu1, mu2, xr, sp.ell_tol, sp.ell_expansion, sp.ell_max_iter, o);
-    os = o[0]; ot = o[1]; orr = o[2];
-  } else {
-    prox_sphere<R>(z1, z2, z3, mu0, xr, os, ot, orr);
-  }
-  const R xq = q == 0 ? xnrm : (q == 1 ? os : (q == 2 ? ot : orr));
-  xn[row] = xq;
-  const R dq = xq - xo;
-  const R d0 = quad_bcast<0>(dq), d1 = quad_bcast<1>(dq), d2 = quad_bcast<2>(dq), d3 = quad_bcast<3>(dq);
-  // w += W J^T dx (contacts.py:107), this lane's block
-  if (own) {
-    const R sign = side_b ? R(1) : R(-1);
-    const R s0 = sign * d0, s1 = sign * d1, s2 = sign * d2, s3 = sign * d3;
-    V3<R> acc = {R(0), R(0), R(0)};
-    acc.x = acc.x + v0.x * s0; acc.y = acc.y + v0.y * s0; acc.z = acc.z + v0.z * s0;
-    acc.x = acc.x + v1.x * s1; acc.y = acc.y + v1.y * s1; acc.z = acc.z + v1.z * s1;
-    acc.x = acc.x + v2.x * s2; acc.y = acc.y + v2.y * s2; acc.z = acc.z + v2.z * s2;
-    acc.x = acc.x + v3.x * s3; acc.y = acc.y + v3.y * s3; acc.z = acc.z + v3.z * s3;
-    V3<R> t;
-    if (ang) {
-      const R W[6] = {W0.x, W0.y, W0.z, W0.w, W1.x, W1.y};
-      t = sym_mul(W, acc);
-    } else {
-      t = V3<R>{W1.z * acc.x, W1.z * acc.y, W1.z * acc.z};
-    }
-    dst[slot] = V4<R>{old.x + t.x, old.y + t.y, old.z + t.z, R(0)};
-  }
-  return fabs((double)xq - (double)xo);
-}
-
 // one sweep of one resident lane by STRIDE threads (a wavefront or the
 // workgroup): STRIDE / 4 contacts per pass
 template <typename R, int STRIDE>
 __device__ double res_sweep(const SceneParams& sp, ResLds<R>& L, int4 lt, bool serial, bool jacobi, bool hybrid,
                             const R* xa, R* xn, R rs, int tid) {
-  const V4<R>* rl = jacobi ? L.sl : L.wl;
-  const V4<R>* ra = jacobi ? L.sa : L.wa;
   const int c0 = lt.x, nc = serial ? lt.y : lt.z;
   const int* co = L.coff + lt.w;
   const int q = tid & 3, j = tid >> 2;
@@ -1175,11 +1245,75 @@ __device__ double res_sweep(const SceneParams& sp, ResLds<R>& L, int4 lt, bool s
   double m = 0.0;
   for (int c = 0; c < nc; ++c) {
     const int lo = serial ? c : co[c], hi = serial ? c + 1 : co[c + 1];
-    for (int pos = lo + j; pos < hi; pos += PER)
-      m = fmax(m, res_update(sp, L, rl, ra, xa, xn, c0 + pos, q, rs, jacobi, hybrid));
+    for (int pos = lo + j; pos < hi; pos += PER) {
+      const int i = c0 + pos;
+      QuadRow<R> it;
+      it.jl = L.JL[4 * i + q]; it.ja = L.JA[4 * i + q]; it.jb = L.JB[4 * i + q];
+      it.xo = xa[4 * i + q];
+      it.lc = L.loc[i];
+      const V4<R>* src = (q & 1) ? (q >= 2 ? L.JB : L.JA) : L.JL;
+      it.v0 = src[4 * i]; it.v1 = src[4 * i + 1]; it.v2 = src[4 * i + 2]; it.v3 = src[4 * i + 3];
+      R xq;
+      m = fmax(m, quad_core(sp, it, q, rs, jacobi, hybrid, L.B, L.zs, xq));
+      xn[4 * i + q] = xq;
+    }
     phase_sync<STRIDE>();
   }
   return m;
+}
+
+// merit meets, accept/reject decided, velocity changes settled (all groups of the scene)
+template <typename R, bool COOP>
+__device__ __forceinline__ int sweep_decide(const Dev<R>& d, const SceneParams& sp, SolveCtl& ctl, double m,
+                                            int it, bool lead, double* sh_red, int* s_ok, const BodyLds<R>& B, int nb,
+                                            bool& abort) {
+  double merit = block_max(m, sh_red);
+  if (COOP) {
+    if (threadIdx.x == 0 && merit > 0.0) atomic_max_nonneg(&d.merit[it], merit);
+    if (!grid_barrier(d.bar, gridDim.x, d.err, s_ok)) { abort = true; return 0; }
+    merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  double* res = d.residuals + sp.res_off;
+  const int dec = ctl.decide(sp, merit, (lead && threadIdx.x == 0) ? res : nullptr, it);
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    if (dec & 1) { B.sl[i] = B.wl[i]; B.sa[i] = B.wa[i]; }
+    else { B.wl[i] = B.sl[i]; B.wa[i] = B.sa[i]; }
+  }
+  __syncthreads();
+  return dec;
+}
+
+template <typename R> __device__ void stage_bodies(const Dev<R>& d, const BodyLds<R>& B, int b0, int nb, int zs) {
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const int b = d.u_blist[b0 + i];
+    B.wl[i] = d.ww0[b]; B.wa[i] = d.ww1[b]; B.sl[i] = d.ws0[b]; B.sa[i] = d.ws1[b];
+    B.W0[i] = d.b_W0[b]; B.W1[i] = d.b_W1[b];
+  }
+  if (threadIdx.x == 0) {
+    const V4<R> z = {R(0), R(0), R(0), R(0)};
+    B.wl[zs] = z; B.wa[zs] = z; B.sl[zs] = z; B.sa[zs] = z; B.W0[zs] = z; B.W1[zs] = z;
+  }
+}
+
+template <typename R, bool COOP>
+__device__ void finish_group(const Dev<R>& d, SceneState& st, const SolveCtl& ctl, bool lead, const BodyLds<R>& B,
+                             int b0, int nb, double rloc, double xmax, double* sh_red) {
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const int b = d.u_blist[b0 + i];
+    d.ws0[b] = B.sl[i];
+    d.ws1[b] = B.sa[i];
+  }
+  const double rm = block_max(rloc, sh_red), xm = block_max(xmax, sh_red);
+  if (threadIdx.x == 0) {
+    if (COOP) {
+      if (rm > 0.0) atomic_max_nonneg(&st.nat_r, rm);
+      if (xm > 0.0) atomic_max_nonneg(&st.nat_x, xm);
+    } else {
+      st.nat_r = rm;
+      st.nat_x = xm;
+    }
+    if (lead) write_solve_state(d, st, ctl);
+  }
 }
 
 template <typename R, bool COOP>
@@ -1192,7 +1326,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
   ResLds<R> L(smem);
   const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
-  // stage: lane table, colour offsets, contact rows, bodies
+  // stage: lane table, colour offsets, contact rows and iterates, bodies
   if (threadIdx.x == 0) {
     int t = 0, o = 0;
     for (int l = gr.l_lo; l < gr.l_hi; ++l) {
@@ -1213,39 +1347,21 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
       for (int c = threadIdx.x; c <= lt.z; c += blockDim.x) L.coff[lt.w + c] = src[c];
     }
   }
+  for (int r = threadIdx.x; r < 4 * gs.nc; r += blockDim.x) {
+    const size_t g = 4 * (size_t)gs.p0 + r;
+    L.JL[r] = d.s_JL[g]; L.JA[r] = d.s_JA[g]; L.JB[r] = d.s_JB[g];
+    L.x0[r] = d.xr0[g]; L.x1[r] = d.xr1[g];
+  }
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
-    const int pos = gs.p0 + i;
-    const V4<R> n4 = d.s_n[pos], rA4 = d.s_rA[pos], rB4 = d.s_rB[pos], bb = d.s_b[pos], r0 = d.s_r0[pos];
-    ContactRow<R> J;
-    contact_rows(n4, rA4, rB4, J);
-    const R b[4] = {bb.x, bb.y, bb.z, bb.w}, rr[4] = {r0.x, r0.y, r0.z, r0.w};
-    const R mu[4] = {R(0), n4.w, rA4.w, rB4.w};
-    const V4<R> xa0 = d.x0[pos], xa1 = d.x1[pos];
-    const R x0v[4] = {xa0.x, xa0.y, xa0.z, xa0.w}, x1v[4] = {xa1.x, xa1.y, xa1.z, xa1.w};
-    for (int r = 0; r < 4; ++r) {
-      L.JL[4 * i + r] = V4<R>{J.lin[r].x, J.lin[r].y, J.lin[r].z, b[r]};
-      L.JA[4 * i + r] = V4<R>{J.angA[r].x, J.angA[r].y, J.angA[r].z, rr[r]};
-      L.JB[4 * i + r] = V4<R>{J.angB[r].x, J.angB[r].y, J.angB[r].z, mu[r]};
-      L.x0[4 * i + r] = x0v[r];
-      L.x1[4 * i + r] = x1v[r];
-    }
-    const int2 lc = d.s_loc[pos];
+    const int2 lc = d.s_loc[gs.p0 + i];
     L.loc[i] = make_int2(lc.x >= 0 ? lc.x - b0 : L.zs, lc.y >= 0 ? lc.y - b0 : L.zs);
   }
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    const int b = d.u_blist[b0 + i];
-    L.wl[i] = d.ww0[b]; L.wa[i] = d.ww1[b]; L.sl[i] = d.ws0[b]; L.sa[i] = d.ws1[b];
-    L.W0[i] = d.b_W0[b]; L.W1[i] = d.b_W1[b];
-  }
-  if (threadIdx.x == 0) {
-    const V4<R> z = {R(0), R(0), R(0), R(0)};
-    L.wl[L.zs] = z; L.wa[L.zs] = z; L.sl[L.zs] = z; L.sa[L.zs] = z; L.W0[L.zs] = z; L.W1[L.zs] = z;
-  }
+  stage_bodies(d, L.B, b0, nb, L.zs);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
-  double* res = d.residuals + sp.res_off;
   SolveCtl ctl;
   ctl.iters = sp.max_iter;
+  bool abort = false;
   for (int it = 0; it < sp.max_iter; ++it) {
     const R* xa = ctl.acc ? L.x1 : L.x0;
     R* xn = ctl.acc ? L.x0 : L.x1;
@@ -1261,59 +1377,122 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
       if (lt.y <= WAVE_LANE_MAX || serial) continue;
       m = fmax(m, res_sweep<R, SOLVE_BLOCK>(sp, L, lt, serial, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
-    double merit = block_max(m, sh_red);
-    if (COOP) {
-      if (threadIdx.x == 0 && merit > 0.0) atomic_max_nonneg(&d.merit[it], merit);
-      if (!grid_barrier(d.bar, gridDim.x, d.err, s_ok)) return;
-      merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0) ? res : nullptr, it);
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      if (dec & 1) { L.sl[i] = L.wl[i]; L.sa[i] = L.wa[i]; }
-      else { L.wl[i] = L.sl[i]; L.wa[i] = L.sa[i]; }
-    }
-    __syncthreads();
+    const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
+    if (abort) return;
     if (dec & 2) { ctl.iters = it; break; }
   }
   // impulses to contact order; natural-map residual (FP64) with r0; accepted velocity changes out
   const R* xs = ctl.acc ? L.x1 : L.x0;
   double rloc = 0.0, xmax = 0.0;
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
-    const V4<R> x = {xs[4 * i], xs[4 * i + 1], xs[4 * i + 2], xs[4 * i + 3]};
-    d.c_imp[d.perm[gs.p0 + i]] = x;
-    ContactRow<R> J;
-    V4<R> bb, r0;
-    R* bp = &bb.x;
-    R* rp = &r0.x;
-    for (int r = 0; r < 4; ++r) {
-      const V4<R> a = L.JL[4 * i + r], e = L.JA[4 * i + r], f = L.JB[4 * i + r];
-      J.lin[r] = xyz(a); J.angA[r] = xyz(e); J.angB[r] = xyz(f);
-      bp[r] = a.w; rp[r] = e.w;
-    }
-    const V4<R> mu = {L.JB[4 * i + 1].w, L.JB[4 * i + 2].w, L.JB[4 * i + 3].w, R(0)};
     const int2 lc = L.loc[i];
-    const V4<R> lA = L.sl[lc.x], aA = L.sa[lc.x], lB = L.sl[lc.y], aB = L.sa[lc.y];
-    const V4<R> y = contact_prox(sp, J, x, bb, r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
-    rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
-                              (double)x.w - (double)y.w));
-    xmax = fmax(xmax, absmax4(x.x, x.y, x.z, x.w));
+    V4<R> x;
+    quad_finish_contact(sp, L.JL, L.JA, L.JB, xs, i, L.B.sl[lc.x], L.B.sa[lc.x], L.B.sl[lc.y], L.B.sa[lc.y], x,
+                        rloc, xmax);
+    d.c_imp[d.perm[gs.p0 + i]] = x;
   }
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    const int b = d.u_blist[b0 + i];
-    d.ws0[b] = L.sl[i];
-    d.ws1[b] = L.sa[i];
-  }
-  const double rm = block_max(rloc, sh_red), xm = block_max(xmax, sh_red);
-  if (threadIdx.x == 0) {
-    if (COOP) {
-      if (rm > 0.0) atomic_max_nonneg(&st.nat_r, rm);
-      if (xm > 0.0) atomic_max_nonneg(&st.nat_x, xm);
+  finish_group<R, COOP>(d, st, ctl, gr.lead, L.B, b0, nb, rloc, xmax, sh_red);
+}
+
+// One sweep of one lane in the streaming layout: rows and iterates from
+// global memory, the next pass prefetched into the other of two buffers
+// while the current one is updated.
+template <typename R, int STRIDE>
+__device__ double stream_sweep(const Dev<R>& d, const SceneParams& sp, const BodyLds<R>& B, int zs, int b0, int p0,
+                               int nc, const int* coff, bool serial, bool jacobi, bool hybrid, const R* xa, R* xn,
+                               R rs, int tid) {
+  constexpr int PER = STRIDE / 4;
+  const int q = tid & 3, j = tid >> 2;
+  double m = 0.0;
+  if (nc <= 0) return m;
+  int c = 0;
+  int pb = serial ? 0 : coff[0];                        // first position of the current pass
+  int hi = serial ? 1 : coff[1];                        // end of the current colour
+  int hn = nc > 1 ? (serial ? 2 : coff[2]) : hi;        // end of the next colour
+  QuadRow<R> ia, ib;
+  bool ha = pb + j < hi;
+  if (ha) load_quad(d.s_JL, d.s_JA, d.s_JB, xa, d.s_loc, p0 + pb + j, q, b0, zs, ia);
+  auto step = [&](QuadRow<R>& use, bool have, QuadRow<R>& fill, bool& fhave) {
+    const bool same = pb + PER < hi;
+    int c1, pb1, hi1, hn1;
+    if (same) {
+      c1 = c; pb1 = pb + PER; hi1 = hi; hn1 = hn;
     } else {
-      st.nat_r = rm;
-      st.nat_x = xm;
+      c1 = c + 1; pb1 = hi; hi1 = hn;
+      hn1 = c1 + 1 < nc ? (serial ? c1 + 2 : coff[c1 + 2]) : hi1;
     }
-    if (gr.lead) write_solve_state(d, st, ctl);
+    fhave = c1 < nc && pb1 + j < hi1;
+    if (fhave) load_quad(d.s_JL, d.s_JA, d.s_JB, xa, d.s_loc, p0 + pb1 + j, q, b0, zs, fill);
+    if (have) {
+      R xq;
+      m = fmax(m, quad_core(sp, use, q, rs, jacobi, hybrid, B, zs, xq));
+      xn[4 * (size_t)(p0 + pb + j) + q] = xq;
+    }
+    if (!same) phase_sync<STRIDE>();
+    c = c1; pb = pb1; hi = hi1; hn = hn1;
+  };
+  bool hb = false;
+  while (true) {
+    step(ia, ha, ib, hb);
+    if (c >= nc) break;
+    step(ib, hb, ia, ha);
+    if (c >= nc) break;
   }
+  return m;
+}
+
+template <typename R, bool COOP>
+__device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group& gr, unsigned char* smem,
+                                   double* sh_red, int* s_ok) {
+  const SceneParams& sp = d.sp[slot];
+  SceneState& st = d.ss[slot];
+  const bool serial = sp.scheme == SCHEME_GS;
+  const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
+  const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
+  StreamLds<R> L(smem);
+  const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
+  stage_bodies(d, L.B, b0, nb, L.zs);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+  SolveCtl ctl;
+  ctl.iters = sp.max_iter;
+  bool abort = false;
+  for (int it = 0; it < sp.max_iter; ++it) {
+    const R* xa = ctl.acc ? d.xr1 : d.xr0;
+    R* xn = ctl.acc ? d.xr0 : d.xr1;
+    const R rs = R(ctl.rscale);
+    double m = 0.0;
+    for (int l = gr.l_lo + wave; l < gr.l_hi; l += nwave) {
+      const int p0 = d.l_start[l], K = d.l_end[l] - p0;
+      if (K <= 0 || (K > WAVE_LANE_MAX && !serial)) continue;
+      m = fmax(m, stream_sweep<R, 64>(d, sp, L.B, L.zs, b0, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
+                                      jacobi, hybrid, xa, xn, rs, lane));
+    }
+    for (int l = gr.l_lo; l < gr.l_hi; ++l) {
+      const int p0 = d.l_start[l], K = d.l_end[l] - p0;
+      if (K <= WAVE_LANE_MAX || serial) continue;
+      m = fmax(m, stream_sweep<R, SOLVE_BLOCK>(d, sp, L.B, L.zs, b0, p0, d.l_ncol[l], lane_coff(d, l), serial,
+                                               jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
+    }
+    const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
+    if (abort) return;
+    if (dec & 2) { ctl.iters = it; break; }
+  }
+  const R* xs = ctl.acc ? d.xr1 : d.xr0;
+  double rloc = 0.0, xmax = 0.0;
+  for (int l = gr.l_lo; l < gr.l_hi; ++l) {
+    const int p0 = d.l_start[l], K = d.l_end[l] - p0;
+    for (int pos = threadIdx.x; pos < K; pos += blockDim.x) {
+      const int i = p0 + pos;
+      const int2 lc = d.s_loc[i];
+      const int sa = lc.x >= 0 ? lc.x - b0 : L.zs, sb = lc.y >= 0 ? lc.y - b0 : L.zs;
+      V4<R> x;
+      quad_finish_contact(sp, d.s_JL, d.s_JA, d.s_JB, xs, i, L.B.sl[sa], L.B.sa[sa], L.B.sl[sb], L.B.sa[sb], x,
+                          rloc, xmax);
+      d.c_imp[d.perm[i]] = x;
+    }
+  }
+  finish_group<R, COOP>(d, st, ctl, gr.lead, L.B, b0, nb, rloc, xmax, sh_red);
 }
 
 template <typename R> __device__ __forceinline__ void no_contacts(SceneState& st) {
@@ -1326,7 +1505,7 @@ __device__ __forceinline__ void solve_dispatch(const Dev<R>& d, int slot, const 
   const GroupSize gs = group_size(d, gr, d.sp[slot].scheme == SCHEME_GS);
   V4<R>* vs = reinterpret_cast<V4<R>*>(smem);
   if (group_fits(d, gr, gs)) solve_group_resident<R, COOP>(d, slot, gr, gs, smem, sh_red, s_ok);
-  else if (gr.b_hi - gr.b_lo <= lds_body_cap<R>()) solve_group<R, true, COOP>(d, slot, gr, vs, sh_red, s_ok);
+  else if (gr.b_hi - gr.b_lo <= ResCap<R>::VB) solve_group_stream<R, COOP>(d, slot, gr, smem, sh_red, s_ok);
   else solve_group<R, false, COOP>(d, slot, gr, vs, sh_red, s_ok);
 }
 

@@ -105,7 +105,7 @@ def cpu_baseline(eng, stepper, steps_done):
 def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
     """1024 mixed box/sphere scenes stepped as one batch vs one scene per launch."""
     import numpy as np
-    from rainbow_amd import native, scenes
+    from rainbow_amd import autotune, native, scenes
     from rainbow_amd.stepper import engine_desc, gather_state, shared_context
     t0 = time.perf_counter()
     ctx = shared_context(0, "fp32")
@@ -126,6 +126,8 @@ def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
     ctx.step(sids, warmup, 0.0, native.STEP_GRAPH)
     ctx.sync()
     ref = {i: ctx.get_state(sids[i], init[i][3]) for i in sample}
+    # launch shapes picked on this batch (the trials restore every scene's state)
+    tuned = autotune.autotune(ctx, sids, [x[3] for x in init], steps=3, log=log)
     for sid in sids:
         ctx.scene_totals(sid, clear=True)
     t1 = time.perf_counter()
@@ -167,6 +169,7 @@ def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
         ctx.scene_destroy(sid)
     for sid in sids:
         ctx.scene_destroy(sid)
+    ctx.set_tuning(**autotune.DEFAULT)
     batched = n_scenes * steps / tb
     plain = len(dup) * plain_steps / tp
     return {"workload": "mixed_drop_%d_scenes" % n_scenes, "bodies": n_bodies, "steps": steps,
@@ -176,7 +179,14 @@ def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
             "plain": {"scene_steps_per_s": plain, "sample_scenes": len(dup), "steps": plain_steps,
                       "ms_per_scene_step": 1e3 * tp / (len(dup) * plain_steps)},
             "speedup_vs_plain": batched / plain if plain > 0 else None,
-            "max_abs_state_diff_vs_plain": diff}
+            "max_abs_state_diff_vs_plain": diff,
+            "tuning": tuning_report(tuned)}
+
+
+def tuning_report(res):
+    keep = lambda t: {k: v for k, v in t.items() if v}
+    return {"chosen": keep(res["best"]), "ms_per_step": round(res["ms_per_step"], 4),
+            "trials": [[keep(t), round(ms, 4)] for t, ms in res["trials"]]}
 
 
 def serving_leg(n_req, rate, budget_gib, alone_checks):

@@ -116,6 +116,7 @@ typedef struct rbh_tuning {
   int resident_contacts;  /* most contacts of a solve group kept in LDS (multiple of 64; at most 384, FP64 192) */
   int resident_bodies;    /* most bodies of a solve group kept in LDS (multiple of 64; same bounds) */
   int lanes_per_group;    /* a single scene's cooperative grid: lanes per workgroup (default 4) */
+  int waves_per_group;    /* wavefronts per workgroup of the batched solve: 4 (default) or 8 */
 } rbh_tuning;
 
 typedef struct rbh_device_info {

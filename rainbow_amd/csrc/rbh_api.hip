@@ -334,6 +334,9 @@ int check_params(rbh_ctx* c, const rbh_params& p) {
 }
 
 // ------------------------------------------------------------ batch workspace
+// wavefronts per workgroup of the batched solve (4 or 8; rbh_tuning.waves_per_group)
+int batch_waves(const rbh_ctx* c) { return c->tune.waves_per_group == 8 ? 8 : 4; }
+
 // multi: a single scene solved by a cooperative grid (its lanes are its units)
 template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool multi) {
   Dev<R> d{};
@@ -371,7 +374,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.isl_fill = P(int*, isl_fill); d.isl_max = P(int*, isl_max); d.c_lab = P(int*, c_lab); d.uperm = P(int*, uperm);
   d.u_cnt = P(int*, u_cnt); d.u_base = P(int*, u_base); d.n_units = P(int*, n_units); d.cap_units = c->cap_units;
   d.unit_T = c->unit_T;
-  d.unit_split = multi ? 0 : (c->tune.lanes_per_scene > 0 ? c->tune.lanes_per_scene : SOLVE_BLOCK / 64);
+  d.unit_split = multi ? 0 : (c->tune.lanes_per_scene > 0 ? c->tune.lanes_per_scene : batch_waves(c));
   d.u_T = P(int*, u_T);
   d.isl_unit = P(int*, isl_unit); d.u_nbody = P(int*, u_nbody); d.u_boff = P(int*, u_boff);
   d.f_loc = P(int*, f_loc); d.f_unit = P(int*, f_unit); d.u_blist = P(int*, u_blist);
@@ -593,7 +596,10 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
     if (e != hipSuccess) return e;
   } else {
     mark(ST_COLOR);
-    hipLaunchKernelGGL(k_solve<R>, dim3(ns), dim3(SOLVE_BLOCK), c->solve_lds, s, d);
+    if (batch_waves(c) == 8)
+      hipLaunchKernelGGL((k_solve<R, 512>), dim3(ns), dim3(512), c->solve_lds, s, d);
+    else
+      hipLaunchKernelGGL((k_solve<R, SOLVE_BLOCK>), dim3(ns), dim3(SOLVE_BLOCK), c->solve_lds, s, d);
   }
   mark(ST_SOLVE);
   hipLaunchKernelGGL(k_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
@@ -625,7 +631,9 @@ template <typename R> int setup_solve_lds(rbh_ctx* c) {
   c->res_cap_b = c->tune.resident_bodies > 0 ? std::min(c->tune.resident_bodies, ResCap<R>::B) : ResCap<R>::B;
   c->res_cap_o = ResCap<R>::O;
   size_t bytes = std::max<size_t>(SOLVE_LDS_BYTES, ResCap<R>::BYTES);
-  const void* ks[2] = {reinterpret_cast<const void*>(k_solve<R>), reinterpret_cast<const void*>(k_solve_multi<R>)};
+  const void* ks[3] = {reinterpret_cast<const void*>(k_solve<R, SOLVE_BLOCK>),
+                       reinterpret_cast<const void*>(k_solve<R, 512>),
+                       reinterpret_cast<const void*>(k_solve_multi<R>)};
   for (const void* k : ks)
     if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
       (void)hipGetLastError();
@@ -1263,6 +1271,7 @@ int rbh_set_tuning(rbh_ctx* c, const rbh_tuning* t) {
   if (!c || !t) return RBH_E_ARG;
   if (t->unit_size < 0 || t->unit_size > 4096 || t->lanes_per_scene < 0 || t->lanes_per_scene > 64 ||
       t->resident_contacts < 0 || t->resident_bodies < 0 || t->lanes_per_group < 0 || t->lanes_per_group > 64 ||
+      (t->waves_per_group != 0 && t->waves_per_group != 4 && t->waves_per_group != 8) ||
       (t->resident_contacts % 64) || (t->resident_bodies % 64))
     return fail(c, RBH_E_ARG, "tuning values out of range (resident capacities are multiples of 64)");
   const int cap = c->prec == 4 ? ResCap<float>::C : ResCap<double>::C;

@@ -927,7 +927,7 @@ struct Group {
   bool lead;          // writes the scene's solver state
 };
 
-template <typename R, bool LDS, bool COOP>
+template <typename R, bool LDS, bool COOP, int NT>
 __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4<R>* smem, double* sh_red,
                             int* s_ok) {
   const SceneParams& sp = d.sp[slot];
@@ -968,7 +968,7 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
     for (int l = gr.l_lo; l < gr.l_hi; ++l) {
       const int p0 = d.l_start[l], K = d.l_end[l] - p0;
       if (K <= 0 || (LDS && (K <= WAVE_LANE_MAX || serial))) continue;
-      m = fmax(m, sweep_lane<R, LDS, SOLVE_BLOCK>(d, sp, w, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
+      m = fmax(m, sweep_lane<R, LDS, NT>(d, sp, w, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
                                                   jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     double merit = block_max(m, sh_red);
@@ -1043,10 +1043,9 @@ template <int K> __device__ __forceinline__ double quad_bcast(double v) {
 }
 
 // One lane's share of a contact for the quad update: row q of J (lin, angA,
-// angB with b, r0, mu in w), the rows 0..3 of the array this lane's velocity
-// block is built from, its iterate and the contact's two body slots.
+// angB with b, r0, mu in w), its iterate and the contact's two body slots.
 template <typename R> struct QuadRow {
-  V4<R> jl, ja, jb, v0, v1, v2, v3;
+  V4<R> jl, ja, jb;
   R xo;
   int2 lc;
 };
@@ -1061,9 +1060,14 @@ template <typename R> struct BodyLds {
 // update's operations (update_contact) on the same operands in the same
 // order, so the results are bitwise the same.  Writes this lane's velocity
 // block; returns the new iterate in xq and |xq - xo| in FP64.
+// JL/JA/JB + 4 i: the contact's rows (LDS or global); this lane's velocity
+// block is built from rows 0..3 of one of them, loaded first.
 template <typename R>
 __device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow<R>& it, int q, R rs, bool jacobi,
-                                            bool hybrid, const BodyLds<R>& B, int zs, R& xq) {
+                                            bool hybrid, const BodyLds<R>& B, int zs, const V4<R>* JL,
+                                            const V4<R>* JA, const V4<R>* JB, size_t i4, R& xq) {
+  const V4<R>* srcv = (q & 1) ? (q >= 2 ? JB : JA) : JL;
+  const V4<R> v0 = srcv[i4], v1 = srcv[i4 + 1], v2 = srcv[i4 + 2], v3 = srcv[i4 + 3];
   const V4<R>* rl = jacobi ? B.sl : B.wl;
   const V4<R>* ra = jacobi ? B.sa : B.wa;
   const bool side_b = q >= 2, ang = q & 1;   // lane 0: linear A, 1: angular A, 2: linear B, 3: angular B
@@ -1100,10 +1104,10 @@ __device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow
     const R sign = side_b ? R(1) : R(-1);
     const R s0 = sign * d0, s1 = sign * d1, s2 = sign * d2, s3 = sign * d3;
     V3<R> acc = {R(0), R(0), R(0)};
-    acc.x = acc.x + it.v0.x * s0; acc.y = acc.y + it.v0.y * s0; acc.z = acc.z + it.v0.z * s0;
-    acc.x = acc.x + it.v1.x * s1; acc.y = acc.y + it.v1.y * s1; acc.z = acc.z + it.v1.z * s1;
-    acc.x = acc.x + it.v2.x * s2; acc.y = acc.y + it.v2.y * s2; acc.z = acc.z + it.v2.z * s2;
-    acc.x = acc.x + it.v3.x * s3; acc.y = acc.y + it.v3.y * s3; acc.z = acc.z + it.v3.z * s3;
+    acc.x = acc.x + v0.x * s0; acc.y = acc.y + v0.y * s0; acc.z = acc.z + v0.z * s0;
+    acc.x = acc.x + v1.x * s1; acc.y = acc.y + v1.y * s1; acc.z = acc.z + v1.z * s1;
+    acc.x = acc.x + v2.x * s2; acc.y = acc.y + v2.y * s2; acc.z = acc.z + v2.z * s2;
+    acc.x = acc.x + v3.x * s3; acc.y = acc.y + v3.y * s3; acc.z = acc.z + v3.z * s3;
     V3<R> t;
     if (ang) {
       const R W[6] = {W0.x, W0.y, W0.z, W0.w, W1.x, W1.y};
@@ -1125,9 +1129,6 @@ __device__ __forceinline__ void load_quad(const V4<R>* JL, const V4<R>* JA, cons
   it.xo = xa[row];
   const int2 lc = loc[i];
   it.lc = make_int2(lc.x >= 0 ? lc.x - b0 : zs, lc.y >= 0 ? lc.y - b0 : zs);
-  const V4<R>* src = (q & 1) ? (q >= 2 ? JB : JA) : JL;
-  it.v0 = src[4 * (size_t)i]; it.v1 = src[4 * (size_t)i + 1]; it.v2 = src[4 * (size_t)i + 2];
-  it.v3 = src[4 * (size_t)i + 3];
 }
 
 // natural-map residual parts of contact i (FP64) from its rows and the
@@ -1251,10 +1252,8 @@ __device__ double res_sweep(const SceneParams& sp, ResLds<R>& L, int4 lt, bool s
       it.jl = L.JL[4 * i + q]; it.ja = L.JA[4 * i + q]; it.jb = L.JB[4 * i + q];
       it.xo = xa[4 * i + q];
       it.lc = L.loc[i];
-      const V4<R>* src = (q & 1) ? (q >= 2 ? L.JB : L.JA) : L.JL;
-      it.v0 = src[4 * i]; it.v1 = src[4 * i + 1]; it.v2 = src[4 * i + 2]; it.v3 = src[4 * i + 3];
       R xq;
-      m = fmax(m, quad_core(sp, it, q, rs, jacobi, hybrid, L.B, L.zs, xq));
+      m = fmax(m, quad_core(sp, it, q, rs, jacobi, hybrid, L.B, L.zs, L.JL, L.JA, L.JB, 4 * (size_t)i, xq));
       xn[4 * i + q] = xq;
     }
     phase_sync<STRIDE>();
@@ -1316,7 +1315,7 @@ __device__ void finish_group(const Dev<R>& d, SceneState& st, const SolveCtl& ct
   }
 }
 
-template <typename R, bool COOP>
+template <typename R, bool COOP, int NT>
 __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Group& gr, const GroupSize& gs,
                                      unsigned char* smem, double* sh_red, int* s_ok) {
   const SceneParams& sp = d.sp[slot];
@@ -1375,7 +1374,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     for (int t = 0; t < gs.nl; ++t) {
       const int4 lt = L.lt[t];
       if (lt.y <= WAVE_LANE_MAX || serial) continue;
-      m = fmax(m, res_sweep<R, SOLVE_BLOCK>(sp, L, lt, serial, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
+      m = fmax(m, res_sweep<R, NT>(sp, L, lt, serial, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
     if (abort) return;
@@ -1425,7 +1424,8 @@ __device__ double stream_sweep(const Dev<R>& d, const SceneParams& sp, const Bod
     if (fhave) load_quad(d.s_JL, d.s_JA, d.s_JB, xa, d.s_loc, p0 + pb1 + j, q, b0, zs, fill);
     if (have) {
       R xq;
-      m = fmax(m, quad_core(sp, use, q, rs, jacobi, hybrid, B, zs, xq));
+      m = fmax(m, quad_core(sp, use, q, rs, jacobi, hybrid, B, zs, d.s_JL, d.s_JA, d.s_JB,
+                            4 * (size_t)(p0 + pb + j), xq));
       xn[4 * (size_t)(p0 + pb + j) + q] = xq;
     }
     if (!same) phase_sync<STRIDE>();
@@ -1441,7 +1441,7 @@ __device__ double stream_sweep(const Dev<R>& d, const SceneParams& sp, const Bod
   return m;
 }
 
-template <typename R, bool COOP>
+template <typename R, bool COOP, int NT>
 __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group& gr, unsigned char* smem,
                                    double* sh_red, int* s_ok) {
   const SceneParams& sp = d.sp[slot];
@@ -1471,7 +1471,7 @@ __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group&
     for (int l = gr.l_lo; l < gr.l_hi; ++l) {
       const int p0 = d.l_start[l], K = d.l_end[l] - p0;
       if (K <= WAVE_LANE_MAX || serial) continue;
-      m = fmax(m, stream_sweep<R, SOLVE_BLOCK>(d, sp, L.B, L.zs, b0, p0, d.l_ncol[l], lane_coff(d, l), serial,
+      m = fmax(m, stream_sweep<R, NT>(d, sp, L.B, L.zs, b0, p0, d.l_ncol[l], lane_coff(d, l), serial,
                                                jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
@@ -1499,20 +1499,20 @@ template <typename R> __device__ __forceinline__ void no_contacts(SceneState& st
   st.solved = 0; st.iterations = 0; st.exitcode = 0; st.sweeps = 0; st.rejects = 0; st.final_merit = 0.0;
 }
 
-template <typename R, bool COOP>
+template <typename R, bool COOP, int NT>
 __device__ __forceinline__ void solve_dispatch(const Dev<R>& d, int slot, const Group& gr, unsigned char* smem,
                                                double* sh_red, int* s_ok) {
   const GroupSize gs = group_size(d, gr, d.sp[slot].scheme == SCHEME_GS);
   V4<R>* vs = reinterpret_cast<V4<R>*>(smem);
-  if (group_fits(d, gr, gs)) solve_group_resident<R, COOP>(d, slot, gr, gs, smem, sh_red, s_ok);
-  else if (gr.b_hi - gr.b_lo <= ResCap<R>::VB) solve_group_stream<R, COOP>(d, slot, gr, smem, sh_red, s_ok);
-  else solve_group<R, false, COOP>(d, slot, gr, vs, sh_red, s_ok);
+  if (group_fits(d, gr, gs)) solve_group_resident<R, COOP, NT>(d, slot, gr, gs, smem, sh_red, s_ok);
+  else if (gr.b_hi - gr.b_lo <= ResCap<R>::VB) solve_group_stream<R, COOP, NT>(d, slot, gr, smem, sh_red, s_ok);
+  else solve_group<R, false, COOP, NT>(d, slot, gr, vs, sh_red, s_ok);
 }
 
-// One workgroup per batch scene.
-template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve(Dev<R> d) {
+// One workgroup of NT threads (NT / 64 wavefronts, one per sweep lane) per batch scene.
+template <typename R, int NT> __global__ void __launch_bounds__(NT) k_solve(Dev<R> d) {
   extern __shared__ __align__(32) unsigned char solve_smem[];
-  __shared__ double sh_red[SOLVE_BLOCK / 64];
+  __shared__ double sh_red[NT / 64];
   __shared__ int s_ok;
   if (aborted(d.err)) return;
   const int bs = blockIdx.x;
@@ -1520,14 +1520,14 @@ template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve(Dev
   const SceneParams& sp = d.sp[slot];
   SceneState& st = d.ss[slot];
   double* res = d.residuals + sp.res_off;
-  for (int i = threadIdx.x; i < sp.max_iter; i += SOLVE_BLOCK) res[i] = INFINITY;
+  for (int i = threadIdx.x; i < sp.max_iter; i += NT) res[i] = INFINITY;
   if (st.k_cnt == 0) {
     if (threadIdx.x == 0) no_contacts<R>(st);
     return;
   }
   const int u0 = d.u_base[bs], u1 = u0 + d.u_cnt[bs];
   const Group gr{d.l_base[bs], d.l_base[bs] + d.l_cnt[bs], d.u_boff[u0], d.u_boff[u1], true};
-  solve_dispatch<R, false>(d, slot, gr, solve_smem, sh_red, &s_ok);
+  solve_dispatch<R, false, NT>(d, slot, gr, solve_smem, sh_red, &s_ok);
 }
 
 // A single scene (its lanes are its colouring units): the lanes split into
@@ -1550,5 +1550,5 @@ template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve_mul
   const long long nl = *d.n_lanes, G = gridDim.x;
   const int l_lo = (int)(blockIdx.x * nl / G), l_hi = (int)((blockIdx.x + 1) * nl / G);
   const Group gr{l_lo, l_hi, d.u_boff[l_lo], d.u_boff[l_hi], blockIdx.x == 0};
-  solve_dispatch<R, true>(d, slot, gr, solve_smem, sh_red, &s_ok);
+  solve_dispatch<R, true, SOLVE_BLOCK>(d, slot, gr, solve_smem, sh_red, &s_ok);
 }

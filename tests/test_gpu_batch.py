@@ -3,13 +3,15 @@
 * Every tuning of the step (unit size, sweep lanes per scene, resident LDS
   capacities small enough to force the velocity-only and global layouts)
   gives bitwise the same contacts, impulses and states.
+* The autotuner, which steps the batch under each candidate, leaves every
+  scene where it found it: the steps after it are bitwise the untuned ones.
 * A request served by continuous batching, among other requests arriving
   over time, ends in bitwise the same state as the same request run alone.
 """
 import numpy as np
 import pytest
 
-from rainbow_amd import native, scenes, serving
+from rainbow_amd import autotune, native, scenes, serving
 from rainbow_amd.stepper import engine_desc, gather_state
 
 pytestmark = pytest.mark.gpu
@@ -18,6 +20,9 @@ TUNINGS = [
     dict(),                                                   # defaults: resident groups
     dict(unit_size=32, lanes_per_scene=2),
     dict(unit_size=128, lanes_per_scene=8, lanes_per_group=2),
+    dict(waves_per_group=8),                                  # 512-thread solve groups
+    dict(waves_per_group=8, lanes_per_scene=3),
+    dict(unit_size=256, lanes_per_scene=1),
     dict(resident_contacts=64, resident_bodies=64),           # groups too big: velocity-only layout
 ]
 
@@ -27,7 +32,7 @@ def bank():
     return scenes.ShapeBank(resolution=32)
 
 
-def run_batch(bank, tuning, seeds, steps, single=False):
+def run_batch(bank, tuning, seeds, steps, single=False, tune_first=False):
     ctx = native.Context(0, "fp32")
     try:
         if tuning:
@@ -43,6 +48,11 @@ def run_batch(bank, tuning, seeds, steps, single=False):
             sid = ctx.scene_create(desc)
             ctx.set_state(sid, *gather_state(list(e.bodies.values())))
             ids.append(sid)
+        if tune_first:
+            cands = autotune.batch_candidates()[::3] if len(ids) > 1 else autotune.single_candidates()[::4]
+            res = autotune.autotune(ctx, ids, [len(e.bodies) for e in engines], dt=0.01, steps=2,
+                                    candidates=cands, graph=False)
+            assert ctx.get_tuning() == res["best"]
         out = []
         for _ in range(steps):
             if single:
@@ -81,6 +91,17 @@ def test_single_scene_schedules_bitwise(bank):
             assert np.array_equal(x, y)
 
 
+def test_autotune_leaves_results_unchanged(bank):
+    """Trials restore every scene: stepping after the tuner equals stepping without it."""
+    for seeds in ((3205, 3206, 3207), (3208,)):
+        ref, _ = run_batch(bank, {}, seeds, 3, single=len(seeds) == 1)
+        got, _ = run_batch(bank, {}, seeds, 3, single=len(seeds) == 1, tune_first=True)
+        for a, b in zip(ref, got):
+            for sa, sb in zip(a, b):
+                for x, y in zip(sa, sb):
+                    assert np.array_equal(x, y), seeds
+
+
 def test_tuning_rejects_bad_values():
     ctx = native.Context(0, "fp32")
     try:
@@ -88,6 +109,8 @@ def test_tuning_rejects_bad_values():
             ctx.set_tuning(resident_contacts=100)          # not a multiple of 64
         with pytest.raises(Exception):
             ctx.set_tuning(resident_contacts=1 << 16, resident_bodies=1 << 16)   # beyond the LDS layout
+        with pytest.raises(Exception):
+            ctx.set_tuning(waves_per_group=16)             # only 4 or 8
     finally:
         ctx.close()
 

@@ -208,8 +208,56 @@ def create_torus_knot(p=2, q=3, R=1.0, r=0.4, tube=0.15, segments=200, sides=25)
         for j in range(sides):
             j2 = (j + 1) % sides
             a0, a1, b0, b1 = i * sides + j, i * sides + j2, i2 * sides + j, i2 * sides + j2
-            T.append((a0, b0, b1))
-            T.append((a0, b1, a1))
+            T.append((a0, b1, b0))          # outward normals
+            T.append((a0, a1, b1))
+    return V, np.array(T, dtype=np.int64)
+
+
+def create_noisy_icosphere(rng, frequency, amplitude=0.2):
+    """Unit icosphere whose vertices are pushed radially by a factor
+    1 + amplitude * U(-1, 1), one draw per vertex from ``rng``."""
+    V, T = create_icosphere(1.0, frequency)
+    V = V * (1.0 + amplitude * rng.uniform(-1.0, 1.0, size=len(V)))[:, None]
+    return V, T
+
+
+def create_funnel(wide=0.5, narrow=0.17, wall=0.05, height=1.0 / 3.0, rings=10, sides=32):
+    """Closed funnel: a conical shell around the y axis, open at both ends.
+
+    The outer surface narrows linearly from radius ``wide`` at y = -height/2
+    to ``narrow`` at y = +height/2; the inner surface lies ``wall`` closer to
+    the axis at every ring; annular rims close the shell at both ends.
+    Faces are wound with outward normals (the inner surface faces the axis),
+    so the solid is the shell.  The wide end is at -y, as in the mesh that
+    scenes/create_funnel.py:84-110 turns upside down with Rx(pi).
+    ``rings`` x ``sides`` vertices per surface.
+    """
+    k = np.arange(rings) / (rings - 1.0)
+    y = -0.5 * height + height * k
+    ro = wide + (narrow - wide) * k
+    ri = ro - wall
+    th = np.arange(sides) * 2.0 * np.pi / sides
+    c, s_ = np.cos(th), np.sin(th)
+
+    def ring(rad):
+        return np.stack([rad[:, None] * c[None, :], np.broadcast_to(y[:, None], (rings, sides)),
+                         rad[:, None] * s_[None, :]], axis=-1).reshape(-1, 3)
+    V = np.concatenate([ring(ro), ring(ri)])
+    o = lambda i, j: i * sides + j % sides
+    n = lambda i, j: rings * sides + i * sides + j % sides
+    T = []
+    for i in range(rings - 1):
+        for j in range(sides):
+            T.append((o(i, j), o(i + 1, j), o(i, j + 1)))
+            T.append((o(i + 1, j), o(i + 1, j + 1), o(i, j + 1)))
+            T.append((n(i, j), n(i, j + 1), n(i + 1, j)))
+            T.append((n(i + 1, j), n(i, j + 1), n(i + 1, j + 1)))
+    b, t = 0, rings - 1
+    for j in range(sides):
+        T.append((o(b, j), o(b, j + 1), n(b, j)))          # wide rim, facing -y
+        T.append((n(b, j), o(b, j + 1), n(b, j + 1)))
+        T.append((o(t, j), n(t, j), o(t, j + 1)))          # narrow rim, facing +y
+        T.append((n(t, j), n(t, j + 1), o(t, j + 1)))
     return V, np.array(T, dtype=np.int64)
 
 

@@ -18,6 +18,14 @@ Workloads (seeds from numpy.random.default_rng):
                      [8, 512], half boxes half spheres, dropped from heights
                      U[1, 10]; mu ~ U[0.1, 0.8]; dt 0.01, 30 iterations.
 * ``serving_request`` a sphere-rain style scene with N log-uniform in [8, 1024].
+* ``funnel_stress``  n bodies (half noisy icospheres of geodesic frequency 22,
+                     10 f^2 + 2 = 4,842 vertices, radial noise 1 + 0.2 U(-1, 1)
+                     per vertex; half (2, 3) torus knots of 200 x 25 = 5,000
+                     vertices), each scaled to a unit bounding box, dropped in
+                     a lattice with random orientations (seed 5) into a fixed
+                     funnel placed as in scenes/create_funnel.py:83-110
+                     (scaled to 2R x H x 2R, at y = 1.5 H, turned by Rx(pi));
+                     64^3 SDFs, mu 0.4, dt 0.005, 50 iterations.
 
 ``sdf_builder`` (optional) replaces the GPU SDF construction, e.g. for
 CPU-only tests.
@@ -199,3 +207,82 @@ def serving_request(seed, bank=None, backend="hip", params=None, nmin=8, nmax=10
     extent = min(extent, 9.5)
     return sphere_rain(N, bank=bank, seed=seed, backend=backend, params=params, extent=extent,
                        ylo=1.0, yhi=1.0 + max(4.0, N * 0.52 / (0.15 * (2 * extent) ** 2)))
+
+
+def _unit_shape(bank, name, make_VT):
+    """A shape whose mesh is scaled to a unit bounding box (surface_mesh.scale_to_unit)."""
+    def make():
+        V, T = make_VT()
+        m = MESH.Mesh(V.copy(), T)
+        MESH.scale_to_unit(m)
+        return m.V, m.T
+    return bank.get(name, make)
+
+
+def funnel_shape(bank, height, radius):
+    """The funnel mesh scaled to 2 radius x height x 2 radius (create_funnel.py:84-95)."""
+    def make():
+        m = MESH.Mesh(*MESH.create_funnel())
+        MESH.scale_to_unit(m)
+        lo, hi = MESH.aabb(m)
+        ext = hi - lo
+        MESH.scale(m, 2.0 * radius / ext[0], height / ext[1], 2.0 * radius / ext[2])
+        return m.V, m.T
+    return bank.get("funnel_%g_%g" % (height, radius), make)
+
+
+def funnel_inner_radius(y, height, radius):
+    """Inner radius of the placed funnel at world height y (height <= y <= 2 height)."""
+    s = (2.0 * height - y) / height              # 0 at the wide top rim, 1 at the spout
+    wide, narrow, wall = 0.5, 0.17, 0.05
+    return 2.0 * radius * ((wide - wall) + (narrow - wide) * s)
+
+
+def funnel_stress(seed=5, n=256, bank=None, backend="hip", params=None, frequency=22, height=8.0, radius=8.0,
+                  spacing=1.5, clearance=1.0):
+    """Noisy icospheres and torus knots poured into a funnel (see the module docstring).
+
+    Bodies fill the funnel from the spout up: lattice sites ``spacing`` apart
+    (more than twice the largest body radius, so no two bodies touch at the
+    start), layer by layer from 0.9 above the spout, each layer's sites in
+    (x, z) order, keeping the sites whose axis distance is at most the
+    funnel's inner radius at that height (its top-rim radius above the rim)
+    minus ``clearance``.  Even bodies are icospheres, odd ones knots.
+    Random draws from default_rng(seed): the vertex noise of each icosphere
+    (in body order), then one orientation quaternion per body.
+    """
+    bank = bank or ShapeBank(resolution=64)
+    prm = dict(time_step=0.005, max_iterations=50, absolute_tolerance=0.0, relative_tolerance=0.0,
+               coloring_seed=seed)
+    prm.update(params or {})
+    eng = new_engine(bank, prm, backend)
+    set_friction(eng, 0.4)
+    API.create_gravity_force(eng, "gravity", 9.81, np.array([0.0, 1.0, 0.0]))
+    add_ground(eng, bank)
+    rng = np.random.default_rng(seed)
+    shapes = []
+    for i in range(n):
+        if i % 2 == 0:
+            V, T = MESH.create_noisy_icosphere(rng, frequency)
+            shapes.append(_unit_shape(bank, "noisy_ico_f%d_s%d_%d" % (frequency, seed, i), lambda V=V, T=T: (V, T)))
+        else:
+            shapes.append(_unit_shape(bank, "torus_knot_23", MESH.create_torus_knot))
+    sites = []
+    half = int(np.ceil(radius / spacing))
+    k = 0
+    while len(sites) < n:
+        y = height + 0.9 + k * spacing
+        rho = funnel_inner_radius(min(y, 2.0 * height), height, radius) - clearance
+        for i in range(-half, half + 1):
+            for j in range(-half, half + 1):
+                x, z = i * spacing, j * spacing
+                if np.hypot(x, z) <= rho and len(sites) < n:
+                    sites.append((x, y, z))
+        k += 1
+    for b, r in enumerate(sites):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        add_body(eng, "body_%d" % b, shapes[b], np.array(r), q=q)
+    add_body(eng, "funnel", funnel_shape(bank, height, radius), np.array([0.0, 1.5 * height, 0.0]),
+             q=Q.Rx(np.pi), fixed=True, forces=())
+    return eng

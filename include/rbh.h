@@ -148,7 +148,8 @@ int rbh_memory_usage(rbh_ctx* ctx, unsigned long long* bytes);
 int rbh_sdf_build(rbh_ctx* ctx, const double* V, int nv, const int* T, int nt, const double* min_coord,
                   const double* max_coord, int I, int J, int K, double* values);
 
-/* Upload one shape (types.py:82-108): body-frame mesh and its SDF grid. */
+/* Upload one shape (types.py:82-108): body-frame mesh (at least 2 triangles)
+ * and its SDF grid. */
 int rbh_shape_upload(rbh_ctx* ctx, const double* V, int nv, const int* T, int nt, const double* sdf_values, int I,
                      int J, int K, const double* min_coord, const double* max_coord, int* shape_id);
 

@@ -114,7 +114,7 @@ struct rbh_ctx {
   long long batch_arena_gen = -1;
   int nb = 0, ns = 0;
   DBuf bb_body, bs_slot, bs_foff, b2f, f_end, d_nb;
-  DBuf row_cnt, row_off, pairs, pair_sz, slot_off, n_pairs, slot_flag, slot_scan, n_slots;
+  DBuf row_cnt, row_off, pairs, pair_sz, slot_off, n_pairs, slot_flag, slot_scan, n_slots, pair_items, item_off;
   DBuf cand_pair, cand_tri, pair_cstart, n_cand, cand_p, cand_n, cand_keep, keep_scan;
   DBuf c_ia, c_ib, c_tri, c_pair, c_p, c_n, n_contacts, c_rA, c_rB, c_b, c_r0, c_mu, c_imp, c_color;
   DBuf f_bs, uf, isl_cnt, isl_off, isl_fill, isl_max, c_lab, uperm, u_cnt, u_base, n_units, u_start, u_end, u_ncol,
@@ -187,7 +187,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
           &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi, &c->clo, &c->chi,
           &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
-          &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs,
+          &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs, &c->pair_items, &c->item_off,
           &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
           &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->c_ia, &c->c_ib, &c->c_tri,
           &c->c_pair, &c->c_p, &c->c_n, &c->n_contacts, &c->c_rA, &c->c_rB, &c->c_b, &c->c_r0, &c->c_mu,
@@ -358,6 +358,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.b2f = P(const int*, b2f); d.f_end = P(const int*, f_end); d.d_nb = P(const int*, d_nb);
   d.row_cnt = P(int*, row_cnt); d.row_off = P(int*, row_off);
   d.pairs = P(int2*, pairs); d.pair_sz = P(int*, pair_sz); d.slot_off = P(int*, slot_off);
+  d.pair_items = P(int*, pair_items); d.item_off = P(int*, item_off);
   d.n_pairs = P(int*, n_pairs); d.cap_pairs = c->cap_pairs;
   d.slot_flag = P(unsigned char*, slot_flag); d.slot_scan = P(int*, slot_scan); d.n_slots = P(int*, n_slots);
   d.cap_slots = c->cap_slots;
@@ -412,6 +413,7 @@ int size_workspace(rbh_ctx* c) {
   struct E { DBuf* b; size_t bytes; } el[] = {
       {&c->row_cnt, nb * 4}, {&c->row_off, (nb + 1) * 4},
       {&c->pairs, cp * 8}, {&c->pair_sz, cp * 4}, {&c->slot_off, (cp + 1) * 4}, {&c->n_pairs, 16},
+      {&c->pair_items, cp * 4}, {&c->item_off, (cp + 1) * 4},
       {&c->slot_flag, cs + 16}, {&c->slot_scan, (cs + 1) * 4}, {&c->n_slots, 16},
       {&c->cand_pair, cc * 4}, {&c->cand_tri, cc * 4}, {&c->pair_cstart, (cp + 1) * 4}, {&c->n_cand, 16},
       {&c->cand_p, cc * V}, {&c->cand_n, cc * V}, {&c->cand_keep, cc + 16}, {&c->keep_scan, (cc + 1) * 4},
@@ -550,8 +552,9 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_pair_count<R>, dim3(nb), dim3(64), 0, s, d);
   scan<int>(c, d.row_cnt, d.row_off, d.d_nb, nb, d.n_pairs);
   hipLaunchKernelGGL(k_pair_write<R>, dim3(nb), dim3(64), 0, s, d);
-  // candidate slots: (nt_a + nt_b) per pair
+  // candidate slots: (nt_a + nt_b) per pair; work items: triangle blocks of both sides
   scan<int>(c, d.pair_sz, d.slot_off, d.n_pairs, c->cap_pairs, d.n_slots);
+  scan<int>(c, d.pair_items, d.item_off, d.n_pairs, c->cap_pairs, nullptr);
   hipLaunchKernelGGL(k_cand_flags<R>, dim3(grid_for(c->cap_pairs, 1, 16384)), dim3(CAND_BLOCK), 0, s, d);
   scan<unsigned char>(c, d.slot_flag, d.slot_scan, d.n_slots, c->cap_slots, nullptr);
   hipLaunchKernelGGL(k_cand_write<R>, dim3(grid_for(c->cap_slots, 256)), dim3(256), 0, s, d);
@@ -986,6 +989,9 @@ int rbh_sdf_build(rbh_ctx* c, const double* V, int nv, const int* T, int nt, con
 int rbh_shape_upload(rbh_ctx* c, const double* V, int nv, const int* T, int nt, const double* sdf, int I, int J,
                      int K, const double* mn, const double* mx, int* shape_id) {
   if (!c || !V || !T || !sdf || !mn || !mx || !shape_id || nv <= 0 || nt <= 0) return fail(c, RBH_E_ARG, "bad shape");
+  // a closed surface has at least 4 triangles (mass.py rejects a mesh without volume); the broad phase keeps
+  // its cluster and block boxes inside the body's own triangle range, which needs nt >= 2
+  if (nt < 2) return fail(c, RBH_E_ARG, "a shape needs at least 2 triangles");
   if (I < 2 || J < 2 || K < 2) return fail(c, RBH_E_ARG, "grid needs at least 2 nodes per axis");
   for (int i = 0; i < 3 * nt; ++i) if (T[i] < 0 || T[i] >= nv) return fail(c, RBH_E_ARG, "triangle index out of range");
   ShapeInfo s{};

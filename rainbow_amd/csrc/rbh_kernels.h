@@ -66,7 +66,8 @@ template <typename R> struct Dev {
   V4<R>* ws0; V4<R>* ws1;              // accepted W J^T sol
   V4<R>* ww0; V4<R>* ww1;              // working copy during a sweep
   V4<R>* tlo; V4<R>* thi;              // per body-instance triangle AABBs
-  V4<R>* clo; V4<R>* chi;              // per body-instance cluster AABBs (CLUSTER triangles each) at b_tri + c
+  V4<R>* clo; V4<R>* chi;              // per body-instance cluster AABBs (CLUSTER triangles each) at b_tri + c,
+                                       // then block AABBs (CAND_BLK positions each) at blk_box(b_tri, nt) + k
   // batch
   int nb, ns;
   const int* bb_body;                  // flat batch body -> arena body
@@ -77,6 +78,7 @@ template <typename R> struct Dev {
   const int* d_nb;                     // device copy of nb (scan length)
   int* row_cnt; int* row_off;          // [nb], [nb+1]
   int2* pairs; int* pair_sz; int* slot_off; int* n_pairs; int cap_pairs;
+  int* pair_items; int* item_off;      // candidate work items (triangle blocks of both sides) per pair, scan
   unsigned char* slot_flag; int* slot_scan; int* n_slots; int cap_slots;
   int* cand_pair; int* cand_tri; int* pair_cstart; int* n_cand; int cap_cand;
   V4<R>* cand_p; V4<R>* cand_n; unsigned char* cand_keep; int* keep_scan;
@@ -196,6 +198,13 @@ template <typename R> __global__ void k_body_prep(Dev<R> d) {
 // triangles' boxes, so a cluster that misses a volume holds no triangle that
 // meets it.
 constexpr int CLUSTER = 32;
+// Blocks of CAND_BLK consecutive positions (CAND_BLK / CLUSTER clusters) are
+// the broad phase's unit of work; their boxes follow the cluster boxes.
+constexpr int CAND_BLK = 256;
+__host__ __device__ constexpr int n_blocks(int nt) { return (nt + CAND_BLK - 1) / CAND_BLK; }
+// first block box of a body whose triangles start at tb; ceil(nt / CLUSTER) +
+// ceil(nt / CAND_BLK) <= nt for nt >= 2, so the boxes stay inside [tb, tb + nt)
+__device__ __forceinline__ int blk_box(int tb, int nt) { return tb + (nt + CLUSTER - 1) / CLUSTER; }
 
 template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
   int f = blockIdx.x;
@@ -208,7 +217,9 @@ template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
   const R inf = R(INFINITY);
   R lx = inf, ly = inf, lz = inf, hx = -inf, hy = -inf, hz = -inf;
   const int tb = d.b_tri[b];
+  const int kb = blk_box(tb, sh.nt);
   const int lane = threadIdx.x;
+  R bx = inf, by = inf, bz = inf, ex = -inf, ey = -inf, ez = -inf;   // box of the current block
   for (int base = 0; base < sh.nt; base += 64) {
     const int t = base + lane;
     R cx = inf, cy = inf, cz = inf, dx = -inf, dy = -inf, dz = -inf;
@@ -239,6 +250,18 @@ template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
     }
     lx = fmin(lx, cx); ly = fmin(ly, cy); lz = fmin(lz, cz);
     hx = fmax(hx, dx); hy = fmax(hy, dy); hz = fmax(hz, dz);
+    // the two clusters of this chunk into the block box
+    bx = fmin(bx, fmin(cx, __shfl_xor(cx, 32, 64))); by = fmin(by, fmin(cy, __shfl_xor(cy, 32, 64)));
+    bz = fmin(bz, fmin(cz, __shfl_xor(cz, 32, 64)));
+    ex = fmax(ex, fmax(dx, __shfl_xor(dx, 32, 64))); ey = fmax(ey, fmax(dy, __shfl_xor(dy, 32, 64)));
+    ez = fmax(ez, fmax(dz, __shfl_xor(dz, 32, 64)));
+    if ((base + 64) % CAND_BLK == 0 || base + 64 >= sh.nt) {
+      if (lane == 0) {
+        d.clo[kb + base / CAND_BLK] = V4<R>{bx, by, bz, R(0)};
+        d.chi[kb + base / CAND_BLK] = V4<R>{ex, ey, ez, R(0)};
+      }
+      bx = by = bz = inf; ex = ey = ez = -inf;
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -286,8 +309,10 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
     unsigned long long m = __ballot(hit);
     if (hit) {
       int pos = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));
+      const int ntB = d.shapes[d.b_shape[j]].nt;
       d.pairs[pos] = make_int2(b, j);
-      d.pair_sz[pos] = ntA + d.shapes[d.b_shape[j]].nt;
+      d.pair_sz[pos] = ntA + ntB;
+      d.pair_items[pos] = n_blocks(ntA) + n_blocks(ntB);
     }
     base += __popcll(m);
   }
@@ -296,16 +321,19 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
 // ========================================================== candidate triangles
 // Candidate set of X w.r.t. Y: triangles of X whose AABB overlaps the AABB of
 // some triangle of Y (the BVH traversal result, collision_detection.py:71,
-// kdop_bvh.py:328-345, reduced by np.unique at :234-246).  One wavefront per
-// (body pair, side): Y's triangles that overlap X's root volume are staged in
-// LDS, then every triangle of X that overlaps Y's root volume is tested
-// against them.  Clusters whose box misses the other root volume are skipped
-// whole (none of their triangles can meet it).
+// kdop_bvh.py:328-345, reduced by np.unique at :234-246).  The unit of work is
+// a block of CAND_BLK positions of X's cluster order, one wavefront each: the
+// triangles of Y that overlap the block's box are staged in LDS (whole Y
+// blocks, then Y clusters, whose box misses it are skipped), then every
+// triangle of the block that overlaps Y's root volume is tested against them.
+// A block whose box misses Y's root volume only clears its flags.  A large
+// body pair is spread over many wavefronts, and each stages only the part of
+// Y near its own block.
 constexpr int CAND_BLOCK = 256;
 constexpr int CAND_WAVES = CAND_BLOCK / 64;
 
 template <typename R> struct CandLds {
-  static constexpr int CAP = (sizeof(R) == 4) ? 512 : 256;   // staged triangles per wavefront (16 KB)
+  static constexpr int CAP = (sizeof(R) == 4) ? 256 : 128;   // staged triangles per wavefront (8 KB)
 };
 
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
@@ -314,28 +342,47 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
 
 constexpr int CAND_CL = 64;   // staged clusters per wavefront
 
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Flags of X's positions [x0, x1) (one block) w.r.t. Y.
 template <typename R>
-__device__ void cand_side(const Dev<R>& d, int X, int Y, int slot_base, bool count_all, V4<R>* lo_s, V4<R>* hi_s,
-                          V4<R>* clo_s, V4<R>* chi_s, int2* crange_s, unsigned long long& overlaps) {
+__device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, int x1, bool count_all,
+                           V4<R>* lo_s, V4<R>* hi_s, V4<R>* clo_s, V4<R>* chi_s, int2* crange_s,
+                           unsigned long long& overlaps) {
   const int cap = CandLds<R>::CAP;
   const int lane = threadIdx.x & 63;
   const ShapeInfo& sX = d.shapes[d.b_shape[X]];
   const ShapeInfo& sY = d.shapes[d.b_shape[Y]];
   const int tX = d.b_tri[X], tY = d.b_tri[Y];
-  const V4<R> rXlo = d.b_lo[X], rXhi = d.b_hi[X], rYlo = d.b_lo[Y], rYhi = d.b_hi[Y];
+  const int kY = blk_box(tY, sY.nt);
+  const V4<R> xlo = d.clo[blk_box(tX, sX.nt) + x0 / CAND_BLK], xhi = d.chi[blk_box(tX, sX.nt) + x0 / CAND_BLK];
+  const V4<R> rYlo = d.b_lo[Y], rYhi = d.b_hi[Y];
+  if (!box_overlap(xlo, xhi, rYlo, rYhi)) {
+    for (int p = x0 + lane; p < x1; p += 64) d.slot_flag[slot_base + d.sT[sX.t_off + p].w] = 0;
+    return;
+  }
   int ybase = 0;
   bool first = true;
   while (true) {
-    // stage the next run of Y triangles that overlap root(X), two clusters
-    // (64 positions) at a time, with the boxes of their clusters
+    // stage the next run of Y triangles that overlap the block box, two
+    // clusters (64 positions) at a time, with the boxes of their clusters
     int nfill = 0, ncl = 0;
     while (ybase < sY.nt) {
+      if (ybase % CAND_BLK == 0 &&
+          !box_overlap(d.clo[kY + ybase / CAND_BLK], d.chi[kY + ybase / CAND_BLK], xlo, xhi)) {
+        ybase += CAND_BLK;
+        continue;
+      }
       const int c0 = ybase / CLUSTER;
       const bool has1 = ybase + CLUSTER < sY.nt;
       const V4<R> b0lo = d.clo[tY + c0], b0hi = d.chi[tY + c0];
       const V4<R> b1lo = has1 ? d.clo[tY + c0 + 1] : b0lo, b1hi = has1 ? d.chi[tY + c0 + 1] : b0hi;
-      const bool o0 = box_overlap(b0lo, b0hi, rXlo, rXhi);
-      const bool o1 = has1 && box_overlap(b1lo, b1hi, rXlo, rXhi);
+      const bool o0 = box_overlap(b0lo, b0hi, xlo, xhi);
+      const bool o1 = has1 && box_overlap(b1lo, b1hi, xlo, xhi);
       if (!o0 && !o1) { ybase += 64; continue; }
       const int p = ybase + lane;
       V4<R> lo, hi;
@@ -344,7 +391,7 @@ __device__ void cand_side(const Dev<R>& d, int X, int Y, int slot_base, bool cou
         const int t = d.sT[sY.t_off + p].w;
         lo = d.tlo[tY + t];
         hi = d.thi[tY + t];
-        in = box_overlap(lo, hi, rXlo, rXhi);
+        in = box_overlap(lo, hi, xlo, xhi);
       }
       const unsigned long long m = __ballot(in);
       const int n0 = __popcll(m & 0xffffffffull), n1 = __popcll(m >> 32);
@@ -359,15 +406,13 @@ __device__ void cand_side(const Dev<R>& d, int X, int Y, int slot_base, bool cou
       nfill += n0 + n1;
       ybase += 64;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int xbase = 0; xbase < sX.nt; xbase += 64) {
+    wave_sync_lds();
+    for (int xbase = x0; xbase < x1; xbase += 64) {
       const int c0 = xbase / CLUSTER;
       const bool o0 = box_overlap(d.clo[tX + c0], d.chi[tX + c0], rYlo, rYhi);
-      const bool o1 = xbase + CLUSTER < sX.nt && box_overlap(d.clo[tX + c0 + 1], d.chi[tX + c0 + 1], rYlo, rYhi);
+      const bool o1 = xbase + CLUSTER < x1 && box_overlap(d.clo[tX + c0 + 1], d.chi[tX + c0 + 1], rYlo, rYhi);
       const int p = xbase + lane;
-      if (p >= sX.nt) continue;
+      if (p >= x1) continue;
       const int t = d.sT[sX.t_off + p].w;
       if (!(lane < CLUSTER ? o0 : o1)) {
         if (first) d.slot_flag[slot_base + t] = 0;
@@ -392,15 +437,28 @@ __device__ void cand_side(const Dev<R>& d, int X, int Y, int slot_base, bool cou
       overlaps += cnt;
       d.slot_flag[slot_base + t] = flag;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync_lds();
     first = false;
     if (ybase >= sY.nt) break;
   }
 }
 
-// one wavefront per (pair, side): item 2p is A's candidates w.r.t. B, 2p + 1 B's w.r.t. A
+// the pair p with item_off[p] <= item < item_off[p + 1] (item_off strictly
+// increasing: every pair has at least one block per side); 64-ary search
+__device__ __forceinline__ int item_pair(const int* __restrict__ item_off, int np, int item) {
+  const int lane = threadIdx.x & 63;
+  int lo = 0, hi = np;
+  while (hi - lo > 1) {
+    const int step = (hi - lo + 63) / 64;
+    const int idx = lo + lane * step;
+    const unsigned long long m = __ballot(idx < hi && item_off[idx] <= item);
+    lo += (63 - __builtin_clzll(m)) * step;      // lane 0 (idx = lo) always qualifies
+    hi = min(hi, lo + step);
+  }
+  return lo;
+}
+
+// one wavefront per item: the items of pair p are A's blocks, then B's
 template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags(Dev<R> d) {
   __shared__ V4<R> lo_s[CAND_WAVES][CandLds<R>::CAP];
   __shared__ V4<R> hi_s[CAND_WAVES][CandLds<R>::CAP];
@@ -415,21 +473,24 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
     if (blockIdx.x == 0 && threadIdx.x == 0) { atomicOr(d.err, ERR_CAP_SLOTS); atomicMax(d.err + 2, nslots); }
     return;
   }
+  const int nitems = d.item_off[np];
   const bool count_all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   const int wave = threadIdx.x >> 6;
-  for (int item = blockIdx.x * CAND_WAVES + wave; item < 2 * np; item += gridDim.x * CAND_WAVES) {
-    const int p = item >> 1, side = item & 1;
+  for (int item = blockIdx.x * CAND_WAVES + wave; item < nitems; item += gridDim.x * CAND_WAVES) {
+    const int p = item_pair(d.item_off, np, item);
     const int2 ab = d.pairs[p];
-    const int base = d.slot_off[p];
+    const int ntA = d.shapes[d.b_shape[ab.x]].nt, ntB = d.shapes[d.b_shape[ab.y]].nt;
+    const int k = item - d.item_off[p];
+    const bool side0 = k < n_blocks(ntA);
+    const int X = side0 ? ab.x : ab.y, Y = side0 ? ab.y : ab.x;
+    const int blk = side0 ? k : k - n_blocks(ntA);
+    const int x0 = blk * CAND_BLK, x1 = min(x0 + CAND_BLK, side0 ? ntA : ntB);
     unsigned long long ov = 0;
-    if (side == 0) {
-      cand_side(d, ab.x, ab.y, base, count_all, lo_s[wave], hi_s[wave], clo_s[wave], chi_s[wave], crange_s[wave], ov);
+    cand_block(d, X, Y, d.slot_off[p] + (side0 ? 0 : ntA), x0, x1, count_all && side0, lo_s[wave], hi_s[wave],
+               clo_s[wave], chi_s[wave], crange_s[wave], ov);
+    if (count_all && side0) {
       const unsigned long long tot = wave_sum_u64(ov);
-      if (count_all && (threadIdx.x & 63) == 0 && tot) atomicAdd(&d.ss[d.b_slot[ab.x]].n_overlaps, tot);
-    } else {
-      const int ntA = d.shapes[d.b_shape[ab.x]].nt;
-      cand_side(d, ab.y, ab.x, base + ntA, false, lo_s[wave], hi_s[wave], clo_s[wave], chi_s[wave], crange_s[wave],
-                ov);
+      if ((threadIdx.x & 63) == 0 && tot) atomicAdd(&d.ss[d.b_slot[ab.x]].n_overlaps, tot);
     }
   }
 }

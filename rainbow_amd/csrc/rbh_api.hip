@@ -116,6 +116,7 @@ struct rbh_ctx {
   DBuf bb_body, bs_slot, bs_foff, b2f, f_end, d_nb;
   DBuf row_cnt, row_off, pairs, pair_sz, slot_off, n_pairs, slot_flag, slot_scan, n_slots, pair_items, item_off;
   DBuf cand_pair, cand_tri, pair_cstart, n_cand, cand_p, cand_n, cand_keep, keep_scan;
+  DBuf kidx, kpos, n_kept, kept_flag, kept_scan;
   DBuf c_ia, c_ib, c_tri, c_pair, c_p, c_n, n_contacts, c_rA, c_rB, c_b, c_r0, c_mu, c_imp, c_color;
   DBuf f_bs, uf, isl_cnt, isl_off, isl_fill, isl_max, c_lab, uperm, u_cnt, u_base, n_units, u_start, u_end, u_ncol,
       u_bs, body_claim, body_loc, bc_cnt, bc_off, bc_fill, bc_list, pal, g_pc, g_done, col_off, merit, bar;
@@ -189,7 +190,8 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
           &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs, &c->pair_items, &c->item_off,
           &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
-          &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->c_ia, &c->c_ib, &c->c_tri,
+          &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->kidx, &c->kpos,
+          &c->n_kept, &c->kept_flag, &c->kept_scan, &c->c_ia, &c->c_ib, &c->c_tri,
           &c->c_pair, &c->c_p, &c->c_n, &c->n_contacts, &c->c_rA, &c->c_rB, &c->c_b, &c->c_r0, &c->c_mu,
           &c->c_imp, &c->c_color, &c->f_bs, &c->uf, &c->isl_cnt, &c->isl_off, &c->isl_fill, &c->isl_max,
           &c->c_lab, &c->uperm, &c->u_cnt, &c->u_base, &c->n_units, &c->u_start, &c->u_end, &c->u_ncol,
@@ -366,6 +368,8 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.n_cand = P(int*, n_cand); d.cap_cand = c->cap_cand;
   d.cand_p = P(V4<R>*, cand_p); d.cand_n = P(V4<R>*, cand_n); d.cand_keep = P(unsigned char*, cand_keep);
   d.keep_scan = P(int*, keep_scan);
+  d.kidx = P(int*, kidx); d.kpos = P(V4<R>*, kpos); d.n_kept = P(int*, n_kept);
+  d.kept_flag = P(unsigned char*, kept_flag); d.kept_scan = P(int*, kept_scan);
   d.c_ia = P(int*, c_ia); d.c_ib = P(int*, c_ib); d.c_tri = P(int*, c_tri); d.c_pair = P(int*, c_pair);
   d.c_p = P(V4<R>*, c_p); d.c_n = P(V4<R>*, c_n); d.n_contacts = P(int*, n_contacts);
   d.cap_contacts = c->cap_contacts;
@@ -417,6 +421,7 @@ int size_workspace(rbh_ctx* c) {
       {&c->slot_flag, cs + 16}, {&c->slot_scan, (cs + 1) * 4}, {&c->n_slots, 16},
       {&c->cand_pair, cc * 4}, {&c->cand_tri, cc * 4}, {&c->pair_cstart, (cp + 1) * 4}, {&c->n_cand, 16},
       {&c->cand_p, cc * V}, {&c->cand_n, cc * V}, {&c->cand_keep, cc + 16}, {&c->keep_scan, (cc + 1) * 4},
+      {&c->kidx, cc * 4}, {&c->kpos, cc * V}, {&c->n_kept, 16}, {&c->kept_flag, cc + 16}, {&c->kept_scan, (cc + 1) * 4},
       {&c->c_ia, ck * 4}, {&c->c_ib, ck * 4}, {&c->c_tri, ck * 4}, {&c->c_pair, ck * 4},
       {&c->c_p, ck * V}, {&c->c_n, ck * V}, {&c->n_contacts, 16}, {&c->c_rA, ck * V}, {&c->c_rB, ck * V},
       {&c->c_b, ck * V}, {&c->c_r0, ck * V}, {&c->c_mu, ck * V}, {&c->c_imp, ck * V}, {&c->c_color, ck * 4},
@@ -560,9 +565,10 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_cand_write<R>, dim3(grid_for(c->cap_slots, 256)), dim3(256), 0, s, d);
   mark(ST_BROAD);
   hipLaunchKernelGGL(k_frank_wolfe<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(k_dedup<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(k_keep01<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
   scan<unsigned char>(c, d.cand_keep, d.keep_scan, d.n_cand, c->cap_cand, nullptr);
+  hipLaunchKernelGGL(k_kept_list<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_dedup<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
+  scan<unsigned char>(c, d.kept_flag, d.kept_scan, d.n_kept, c->cap_cand, nullptr);
   hipLaunchKernelGGL(k_contact_write<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
   mark(ST_FW);
   hipLaunchKernelGGL(k_assemble<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);

@@ -82,6 +82,8 @@ template <typename R> struct Dev {
   unsigned char* slot_flag; int* slot_scan; int* n_slots; int cap_slots;
   int* cand_pair; int* cand_tri; int* pair_cstart; int* n_cand; int cap_cand;
   V4<R>* cand_p; V4<R>* cand_n; unsigned char* cand_keep; int* keep_scan;
+  int* kidx; V4<R>* kpos; int* n_kept;  // candidates kept by Frank-Wolfe, compacted (candidate order)
+  unsigned char* kept_flag; int* kept_scan;   // 1 if not a duplicate; its scan (contact index)
   // contacts (canonical order)
   int* c_ia; int* c_ib; int* c_tri; int* c_pair; V4<R>* c_p; V4<R>* c_n; int* n_contacts; int cap_contacts;
   V4<R>* c_rA; V4<R>* c_rB; V4<R>* c_b; V4<R>* c_r0; V4<R>* c_mu;
@@ -635,33 +637,44 @@ template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
   }
 }
 
-// collision_detection.py:255-290: keep the first of contacts with equal
-// position within a body pair (candidates of one pair are contiguous).
-template <typename R> __global__ void k_dedup(Dev<R> d) {
+// Candidates kept by Frank-Wolfe in candidate order: kidx[keep_scan[c]] = c,
+// with their points copied alongside for the duplicate test.
+template <typename R> __global__ void k_kept_list(Dev<R> d) {
   if (aborted(d.err)) return;
   const int nc = *d.n_cand;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.n_kept = d.keep_scan[nc];
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
     if (!d.cand_keep[c]) continue;
-    int p = d.cand_pair[c];
-    int c0 = d.pair_cstart[p];
-    V4<R> pc = d.cand_p[c];
+    const int k = d.keep_scan[c];
+    d.kidx[k] = c;
+    d.kpos[k] = d.cand_p[c];
+  }
+}
+
+// collision_detection.py:255-290: keep the first of contacts with equal
+// position within a body pair.  A pair's candidates are contiguous, so its
+// kept ones are contiguous in the compacted list too: each is compared with
+// the kept contacts of its pair before it.
+template <typename R> __global__ void k_dedup(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nk = *d.n_kept;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
+    const int c = d.kidx[k];
+    const int k0 = d.keep_scan[d.pair_cstart[d.cand_pair[c]]];
+    const V4<R> pc = d.kpos[k];
     bool dup = false;
-    for (int e = c0; e < c; ++e) {
-      if (!d.cand_keep[e]) continue;
-      V4<R> pe = d.cand_p[e];
-      if (pe.x == pc.x && pe.y == pc.y && pe.z == pc.z) { dup = true; break; }
+    for (int e = k0; e < k && !dup; ++e) {
+      const V4<R> pe = d.kpos[e];
+      dup = pe.x == pc.x && pe.y == pc.y && pe.z == pc.z;
     }
-    // a later lane reads this flag only for e < its own c and only after the
-    // decision above, so clearing it here cannot change another lane's test
-    // (duplicates of an earlier contact equal that earlier contact as well)
-    if (dup) d.cand_keep[c] = 2;
+    d.kept_flag[k] = dup ? 0 : 1;
   }
 }
 
 template <typename R> __global__ void k_contact_write(Dev<R> d) {
   if (aborted(d.err)) return;
-  const int nc = *d.n_cand;
-  const int nk = d.keep_scan[nc];
+  const int nkept = *d.n_kept;
+  const int nk = d.kept_scan[nkept];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *d.n_contacts = nk;
     if (nk > d.cap_contacts) { atomicOr(d.err, ERR_CAP_CONTACTS); atomicMax(d.err + 4, nk); }
@@ -670,12 +683,13 @@ template <typename R> __global__ void k_contact_write(Dev<R> d) {
   const int lane = threadIdx.x & 63;
   // whole wavefronts per iteration, so the per-scene first index, count and
   // largest penetration are merged across the wavefront before the atomics
-  for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < nc; base += gridDim.x * blockDim.x) {
-    const int c = base + lane;
+  for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < nkept; base += gridDim.x * blockDim.x) {
+    const int j = base + lane;
     int slot = -1, k = 0x7fffffff;
     double pen = 0.0;
-    if (c < nc && d.cand_keep[c] == 1) {
-      k = d.keep_scan[c];
+    if (j < nkept && d.kept_flag[j]) {
+      const int c = d.kidx[j];
+      k = d.kept_scan[j];
       int p = d.cand_pair[c];
       int dir = (d.cand_tri[c] >> 31) & 1;
       int2 ab = d.pairs[p];
@@ -711,14 +725,6 @@ template <typename R> __global__ void k_contact_write(Dev<R> d) {
       live &= ~__ballot(mine);
     }
   }
-}
-
-// after dedup: duplicates (flag 2) become 0 so the scan counts kept contacts
-template <typename R> __global__ void k_keep01(Dev<R> d) {
-  if (aborted(d.err)) return;
-  const int nc = *d.n_cand;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
-    if (d.cand_keep[c] == 2) d.cand_keep[c] = 0;
 }
 
 // ========================================================== assembly

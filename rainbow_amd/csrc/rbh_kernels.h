@@ -65,7 +65,7 @@ template <typename R> struct Dev {
   V4<R>* b_lo; V4<R>* b_hi;            // world AABB grown by the envelope
   V4<R>* ws0; V4<R>* ws1;              // accepted W J^T sol
   V4<R>* ww0; V4<R>* ww1;              // working copy during a sweep
-  V4<R>* tlo; V4<R>* thi;              // per body-instance triangle AABBs
+  V4<R>* tlo; V4<R>* thi;              // per body-instance triangle AABBs, in cluster order (b_tri + position)
   V4<R>* clo; V4<R>* chi;              // per body-instance cluster AABBs (CLUSTER triangles each) at b_tri + c,
                                        // then block AABBs (CAND_BLK positions each) at blk_box(b_tri, nt) + k
   // batch
@@ -235,8 +235,8 @@ template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
                   fmin(fmin(p0.z, p1.z), p2.z) - env, R(0)};
       V4<R> hi = {fmax(fmax(p0.x, p1.x), p2.x) + env, fmax(fmax(p0.y, p1.y), p2.y) + env,
                   fmax(fmax(p0.z, p1.z), p2.z) + env, R(0)};
-      d.tlo[tb + tt] = lo;
-      d.thi[tb + tt] = hi;
+      d.tlo[tb + t] = lo;
+      d.thi[tb + t] = hi;
       cx = lo.x; cy = lo.y; cz = lo.z; dx = hi.x; dy = hi.y; dz = hi.z;
     }
     // cluster boxes: the 32 lanes of each half-wave hold one cluster
@@ -335,7 +335,7 @@ constexpr int CAND_BLOCK = 256;
 constexpr int CAND_WAVES = CAND_BLOCK / 64;
 
 template <typename R> struct CandLds {
-  static constexpr int CAP = (sizeof(R) == 4) ? 256 : 128;   // staged triangles per wavefront (8 KB)
+  static constexpr int CAP = (sizeof(R) == 4) ? 128 : 64;    // staged triangles per wavefront (4 KB)
 };
 
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
@@ -343,6 +343,11 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
 }
 
 constexpr int CAND_CL = 64;   // staged clusters per wavefront
+
+// lane src's box corner (x, y, z; w is unused and read as 0)
+template <typename R> __device__ __forceinline__ V4<R> shfl_xyz(const V4<R>& v, int src) {
+  return V4<R>{__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), R(0)};
+}
 
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -374,27 +379,32 @@ __device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0,
     // clusters (64 positions) at a time, with the boxes of their clusters
     int nfill = 0, ncl = 0;
     while (ybase < sY.nt) {
-      if (ybase % CAND_BLK == 0 &&
-          !box_overlap(d.clo[kY + ybase / CAND_BLK], d.chi[kY + ybase / CAND_BLK], xlo, xhi)) {
-        ybase += CAND_BLK;
-        continue;
+      if (ybase % CAND_BLK == 0) {
+        // the next Y block whose box meets the block box, 64 block boxes per load
+        int yb = ybase / CAND_BLK;
+        const int nbY = n_blocks(sY.nt);
+        while (yb < nbY) {
+          const int j = yb + lane;
+          const unsigned long long m = __ballot(j < nbY && box_overlap(d.clo[kY + j], d.chi[kY + j], xlo, xhi));
+          if (m) { yb += __ffsll((long long)m) - 1; break; }
+          yb += 64;
+        }
+        if (yb >= nbY) { ybase = sY.nt; break; }
+        ybase = yb * CAND_BLK;
       }
-      const int c0 = ybase / CLUSTER;
-      const bool has1 = ybase + CLUSTER < sY.nt;
-      const V4<R> b0lo = d.clo[tY + c0], b0hi = d.chi[tY + c0];
-      const V4<R> b1lo = has1 ? d.clo[tY + c0 + 1] : b0lo, b1hi = has1 ? d.chi[tY + c0 + 1] : b0hi;
-      const bool o0 = box_overlap(b0lo, b0hi, xlo, xhi);
-      const bool o1 = has1 && box_overlap(b1lo, b1hi, xlo, xhi);
-      if (!o0 && !o1) { ybase += 64; continue; }
+      // this lane's cluster box and triangle box, loaded together
       const int p = ybase + lane;
-      V4<R> lo, hi;
-      bool in = false;
-      if (p < sY.nt && (lane < CLUSTER ? o0 : o1)) {
-        const int t = d.sT[sY.t_off + p].w;
-        lo = d.tlo[tY + t];
-        hi = d.thi[tY + t];
-        in = box_overlap(lo, hi, xlo, xhi);
-      }
+      const bool live = p < sY.nt;
+      const int cl = min(p, sY.nt - 1) / CLUSTER;
+      const V4<R> cbl = d.clo[tY + cl], cbh = d.chi[tY + cl];
+      V4<R> lo = cbl, hi = cbh;
+      if (live) { lo = d.tlo[tY + p]; hi = d.thi[tY + p]; }
+      const bool oc = live && box_overlap(cbl, cbh, xlo, xhi);
+      const bool in = oc && box_overlap(lo, hi, xlo, xhi);
+      const unsigned long long mc = __ballot(oc);
+      if (!mc) { ybase += 64; continue; }
+      const V4<R> b0lo = shfl_xyz(cbl, 0), b0hi = shfl_xyz(cbh, 0);
+      const V4<R> b1lo = shfl_xyz(cbl, 32), b1hi = shfl_xyz(cbh, 32);
       const unsigned long long m = __ballot(in);
       const int n0 = __popcll(m & 0xffffffffull), n1 = __popcll(m >> 32);
       if (nfill + n0 + n1 > cap || ncl + 2 > CAND_CL) break;   // uniform: stage full, test what is staged
@@ -410,19 +420,19 @@ __device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0,
     }
     wave_sync_lds();
     for (int xbase = x0; xbase < x1; xbase += 64) {
-      const int c0 = xbase / CLUSTER;
-      const bool o0 = box_overlap(d.clo[tX + c0], d.chi[tX + c0], rYlo, rYhi);
-      const bool o1 = xbase + CLUSTER < x1 && box_overlap(d.clo[tX + c0 + 1], d.chi[tX + c0 + 1], rYlo, rYhi);
       const int p = xbase + lane;
       if (p >= x1) continue;
+      // cluster box, triangle box and flag of this lane's position, loaded together
+      const int cl = p / CLUSTER;
       const int t = d.sT[sX.t_off + p].w;
-      if (!(lane < CLUSTER ? o0 : o1)) {
+      const V4<R> cbl = d.clo[tX + cl], cbh = d.chi[tX + cl];
+      const V4<R> lo = d.tlo[tX + p], hi = d.thi[tX + p];
+      unsigned char flag = first ? 0 : d.slot_flag[slot_base + t];
+      if (!box_overlap(cbl, cbh, rYlo, rYhi)) {
         if (first) d.slot_flag[slot_base + t] = 0;
         continue;
       }
-      unsigned char flag = first ? 0 : d.slot_flag[slot_base + t];
       if (flag && !count_all) continue;
-      V4<R> lo = d.tlo[tX + t], hi = d.thi[tX + t];
       if (!box_overlap(lo, hi, rYlo, rYhi)) { if (first) d.slot_flag[slot_base + t] = 0; continue; }
       unsigned cnt = 0;
       for (int k = 0; k < ncl && !(flag && !count_all); ++k) {

@@ -214,8 +214,19 @@ template <typename R> __device__ __forceinline__ void prox_sphere(R zs, R zt, R 
   R radius = mu * xn;
   R sq = zs * zs + zt * zt + zr * zr;
   if (sq <= radius * radius) { os = zs; ot = zt; orr = zr; return; }
-  R scale = radius / sqrt(sq);
+  // FP32: one hardware reciprocal square root (1 ulp) in place of a correctly
+  // rounded sqrt and divide; FP64 keeps radius / sqrt(sq) as the reference has it
+  R scale;
+  if constexpr (sizeof(R) == 4) scale = radius * __builtin_amdgcn_rsqf(sq);
+  else scale = radius / sqrt(sq);
   os = zs * scale; ot = zt * scale; orr = zr * scale;
+}
+
+// First term of a velocity-change sum: FP64 starts from 0 as the reference's
+// sparse products do (0 + p only differs from p for p = -0); FP32 starts from p.
+template <typename R> __device__ __forceinline__ R sum_first(R p) {
+  if constexpr (sizeof(R) == 8) return R(0) + p;
+  else return p;
 }
 
 // proximal_contact/proximal_operators.py:42-139, evaluated in FP64.

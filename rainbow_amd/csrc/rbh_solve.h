@@ -710,10 +710,11 @@ __device__ __forceinline__ void dw_body(V4<R> W0, V4<R> W1, R sign, const Contac
                                         V4<R> dx, V4<R>& l, V4<R>& g) {
   const R im = W1.z;
   const R W[6] = {W0.x, W0.y, W0.z, W0.w, W1.x, W1.y};
-  R dl[3] = {R(0), R(0), R(0)};
-  V3<R> a = {R(0), R(0), R(0)};
   const R dd[4] = {dx.x, dx.y, dx.z, dx.w};
-  for (int i = 0; i < 4; ++i) {
+  const R s0 = sign * dd[0];
+  R dl[3] = {sum_first(J.lin[0].x * s0), sum_first(J.lin[0].y * s0), sum_first(J.lin[0].z * s0)};
+  V3<R> a = {sum_first(s0 * ang[0].x), sum_first(s0 * ang[0].y), sum_first(s0 * ang[0].z)};
+  for (int i = 1; i < 4; ++i) {
     const R s = sign * dd[i];
     dl[0] += J.lin[i].x * s; dl[1] += J.lin[i].y * s; dl[2] += J.lin[i].z * s;
     a = a + s * ang[i];
@@ -1076,7 +1077,17 @@ __device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow
   const bool own = slot != zs;
   // fixed bodies read the zero slot (the serial update uses zeros for them)
   const V4<R> lA = rl[it.lc.x], aA = ra[it.lc.x], lB = rl[it.lc.y], aB = ra[it.lc.y];
-  const V4<R> W0 = B.W0[slot], W1 = B.W1[slot], old = dst[slot];
+  const V4<R> W0 = B.W0[slot], W1 = B.W1[slot];
+  // the block this lane updates: already loaded in Gauss-Seidel (w is read in place)
+  // (component-wise selects: a select of whole structs by a per-lane flag goes through scratch)
+  V4<R> old;
+  if (jacobi) {
+    old = dst[slot];
+  } else {
+    const R ox0 = ang ? aA.x : lA.x, oy0 = ang ? aA.y : lA.y, oz0 = ang ? aA.z : lA.z;
+    const R ox1 = ang ? aB.x : lB.x, oy1 = ang ? aB.y : lB.y, oz1 = ang ? aB.z : lB.z;
+    old = V4<R>{side_b ? ox1 : ox0, side_b ? oy1 : oy0, side_b ? oz1 : oz0, R(0)};
+  }
   const V4<R> jl = it.jl, ja = it.ja, jb = it.jb;
   const R xo = it.xo;
   // row q of z = x - r (J w + b)  (contacts.py:77-92)
@@ -1103,8 +1114,7 @@ __device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow
   if (own) {
     const R sign = side_b ? R(1) : R(-1);
     const R s0 = sign * d0, s1 = sign * d1, s2 = sign * d2, s3 = sign * d3;
-    V3<R> acc = {R(0), R(0), R(0)};
-    acc.x = acc.x + v0.x * s0; acc.y = acc.y + v0.y * s0; acc.z = acc.z + v0.z * s0;
+    V3<R> acc = {sum_first(v0.x * s0), sum_first(v0.y * s0), sum_first(v0.z * s0)};
     acc.x = acc.x + v1.x * s1; acc.y = acc.y + v1.y * s1; acc.z = acc.z + v1.z * s1;
     acc.x = acc.x + v2.x * s2; acc.y = acc.y + v2.y * s2; acc.z = acc.z + v2.z * s2;
     acc.x = acc.x + v3.x * s3; acc.y = acc.y + v3.y * s3; acc.z = acc.z + v3.z * s3;

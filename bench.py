@@ -29,9 +29,15 @@ of the same scenes, whose end states must equal the batched ones.
 
 serving (on by default, --no-serving to skip): Poisson arrivals of
 sphere-drop requests (8..1024 bodies, 50..500 steps) admitted against an HBM
-budget and advanced by continuous batching; p50/p99 per-step latency, completed
-scene-steps/s, admission wait, and the alone-vs-mixed difference of a few
-requests.
+budget and advanced by continuous batching, at 10, 50 and 200 requests/s and
+at saturation; p50/p99 per-step latency, completed scene-steps/s, admission
+wait, the plain path (one request per launch) on a sample of the requests,
+and the alone-vs-mixed difference of a few requests.
+
+stress (on by default, --no-stress to skip): 128 noisy icospheres and 128
+torus knots (about 10^4 triangles each, 64^3 SDFs) poured into a fixed funnel,
+300 steps from rest; contact-iters/s over the run, and per-stage times of the
+last 50 steps.
 
 --copy-calibration BYTES runs only a plain device-to-device copy of BYTES
 bytes (the reference kernel for scaling the profiler's byte counters).
@@ -189,29 +195,100 @@ def tuning_report(res):
             "trials": [[keep(t), round(ms, 4)] for t, ms in res["trials"]]}
 
 
-def serving_leg(n_req, rate, budget_gib, alone_checks):
-    """Poisson arrivals served by continuous batching."""
+def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, plain_steps=10):
+    """Poisson arrivals served by continuous batching, one run per arrival rate
+    (0 = saturation: every request waiting at t = 0), on the same requests;
+    the plain path (one request per launch, no graph) on a sample of them."""
     import numpy as np
     from rainbow_amd import scenes, serving
     t0 = time.perf_counter()
     bank = scenes.ShapeBank(resolution=32)
     rng = np.random.default_rng(4)
-    arrivals = serving.poisson_arrivals(rate, n_req, seed=4)
     srv = serving.Server(budget=budget_gib << 30)
     reqs = []
     for i in range(n_req):
         eng = scenes.serving_request(4000 + i, bank)
-        reqs.append(srv.prepare(serving.Request(i, eng, int(rng.integers(50, 501)), float(arrivals[i]))))
-    log("serving: %d requests (%d bodies) prepared in %.1f s" % (
-        n_req, sum(len(r.engine.bodies) for r in reqs), time.perf_counter() - t0))
-    rep = srv.run(reqs)
+        reqs.append(srv.prepare(serving.Request(i, eng, int(rng.integers(50, 501)), 0.0)))
+    log("serving: %d requests (%d bodies, %d scene-steps) prepared in %.1f s" % (
+        n_req, sum(len(r.engine.bodies) for r in reqs), sum(r.n_steps for r in reqs), time.perf_counter() - t0))
+    runs = []
+    for rate in rates:
+        arr = serving.poisson_arrivals(rate, n_req, seed=4) if rate > 0 else np.zeros(n_req)
+        for r, a in zip(reqs, arr):
+            r.arrival = float(a)
+        rep = srv.run(reqs)
+        rep["rate_per_s"] = rate if rate > 0 else "saturation"
+        runs.append(rep)
+        log("serving at %s req/s: %.0f scene-steps/s, p50 %.2f ms, p99 %.2f ms, wait p99 %.1f ms" % (
+            rep["rate_per_s"], rep["scene_steps_per_s"], rep["p50_step_ms"], rep["p99_step_ms"], rep["p99_wait_ms"]))
     diffs = []
     for r in reqs[:alone_checks]:
         alone = srv.run_alone(r)
         diffs.append(max(float(np.abs(a - b).max()) for a, b in zip(alone, r.state)))
-    rep.update(rate_per_s=rate, max_abs_diff_alone_vs_mixed=max(diffs) if diffs else None,
-               alone_checked=len(diffs))
-    return rep
+    # plain path: a sample of the requests, each stepped alone (one scene per launch, no graph)
+    sample = reqs[::max(1, n_req // plain_sample)][:plain_sample]
+    ctx = srv.ctx
+    sids = []
+    for r in sample:
+        sid = ctx.scene_create(r._desc)
+        ctx.set_state(sid, *r._state0)
+        ctx.step([sid], 1, 0.0, 0)               # workspace sizing, untimed
+        sids.append(sid)
+    ctx.sync()
+    t1 = time.perf_counter()
+    for _ in range(plain_steps):
+        for sid in sids:
+            ctx.step([sid], 1, 0.0, 0)
+            ctx.sync()
+    tp = time.perf_counter() - t1
+    for sid in sids:
+        ctx.scene_destroy(sid)
+    plain = len(sids) * plain_steps / tp
+    sat = [r for r in runs if r["rate_per_s"] == "saturation"]
+    best = sat[0] if sat else max(runs, key=lambda r: r["scene_steps_per_s"])
+    return {"workload": "sphere_drop_requests_%d" % n_req, "requests": n_req,
+            "scene_steps": sum(r.n_steps for r in reqs), "budget_bytes": budget_gib << 30, "runs": runs,
+            "plain": {"scene_steps_per_s": plain, "sample_requests": len(sids), "steps": plain_steps,
+                      "ms_per_scene_step": 1e3 * tp / (len(sids) * plain_steps)},
+            "speedup_vs_plain_at_saturation": best["scene_steps_per_s"] / plain if plain > 0 else None,
+            "max_abs_diff_alone_vs_mixed": max(diffs) if diffs else None, "alone_checked": len(diffs)}
+
+
+def stress_leg(steps):
+    """Noisy icospheres and torus knots poured into a funnel (64^3 SDFs), from rest."""
+    from rainbow_amd import native, scenes
+    t0 = time.perf_counter()
+    eng = scenes.funnel_stress()
+    st = eng.stepper
+    st._prepare(eng)
+    ctx, sid = st.ctx, st.scene
+    dt = eng.params.time_step
+    tris = sum(len(b.shape.mesh.T) for b in eng.bodies.values())
+    log("stress: %d bodies, %d triangles, built in %.1f s" % (len(eng.bodies), tris, time.perf_counter() - t0))
+    ctx.scene_totals(sid, clear=True)
+    t1 = time.perf_counter()
+    ctx.step([sid], steps - 50, dt, 0)
+    ctx.sync()
+    part = ctx.scene_totals(sid, clear=True)
+    t2 = time.perf_counter()
+    ctx.step([sid], 50, dt, native.STEP_TIMING)      # the last 50 steps with stage timings
+    ctx.sync()
+    t3 = time.perf_counter()
+    last = ctx.scene_totals(sid)
+    tm = ctx.get_timings()
+    lg = ctx.get_log(sid, eng.params.max_iterations)
+    tot = {k: part[k] + last[k] for k in part}
+    ctx.scene_destroy(sid)
+    st.scene = None
+    return {"workload": "funnel_stress_256", "bodies": len(eng.bodies), "triangles": tris, "sdf_resolution": 64,
+            "steps": steps, "contact_iters_per_s": tot["contact_iters"] / (t3 - t1),
+            "ms_per_step": 1e3 * (t3 - t1) / steps,
+            "last_50": {"ms_per_step": 1e3 * (t3 - t2) / 50, "contacts_per_step": last["contacts"] / 50,
+                        "candidates_per_step": last["candidates"] / 50,
+                        "contact_iters_per_s": last["contact_iters"] / (t3 - t2),
+                        "stages_ms": {k: round(1e3 * v, 4) for k, v in tm.items()}},
+            "last_step": {"contacts": lg["contact_points"], "colors": lg["colors"], "sweeps": lg["sweeps"],
+                          "natural_residual": lg["natural_residual"]}}
 
 
 def main():
@@ -226,8 +303,11 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--batched-scenes", type=int, default=1024)
     ap.add_argument("--serving-requests", type=int, default=64)
-    ap.add_argument("--serving-rate", type=float, default=200.0)
-    ap.add_argument("--only", choices=("batched", "serving"), default=None,
+    ap.add_argument("--serving-rates", default="10,50,200,0",
+                    help="arrival rates in requests/s, 0 = saturation")
+    ap.add_argument("--no-stress", action="store_true", help="skip the funnel stress leg")
+    ap.add_argument("--stress-steps", type=int, default=300)
+    ap.add_argument("--only", choices=("batched", "serving", "stress"), default=None,
                     help="run one of the extra legs alone (for profiling) and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -242,8 +322,12 @@ def main():
     if args.only == "batched":
         print(json.dumps(batched_leg(args.batched_scenes, 20, 3, 64, 2)), flush=True)
         return
+    rates = [float(x) for x in args.serving_rates.split(",") if x]
     if args.only == "serving":
-        print(json.dumps(serving_leg(args.serving_requests, args.serving_rate, 64, 3)), flush=True)
+        print(json.dumps(serving_leg(args.serving_requests, rates, 64, 3)), flush=True)
+        return
+    if args.only == "stress":
+        print(json.dumps(stress_leg(args.stress_steps)), flush=True)
         return
 
     from rainbow_amd import native, scenes
@@ -322,10 +406,12 @@ def main():
                                                           result["batched"]["plain"]["scene_steps_per_s"]))
     if not args.no_serving:
         log("serving leg ...")
-        result["serving"] = serving_leg(args.serving_requests, args.serving_rate, 64, 3)
-        log("serving: %.0f scene-steps/s, p50 %.2f ms, p99 %.2f ms" % (
-            result["serving"]["scene_steps_per_s"], result["serving"]["p50_step_ms"],
-            result["serving"]["p99_step_ms"]))
+        result["serving"] = serving_leg(args.serving_requests, rates, 64, 3)
+    if not args.no_stress:
+        log("stress leg ...")
+        result["stress"] = stress_leg(args.stress_steps)
+        log("stress: %.3g contact-iters/s, %.2f ms/step" % (result["stress"]["contact_iters_per_s"],
+                                                            result["stress"]["ms_per_step"]))
     print(json.dumps(result), flush=True)
 
 

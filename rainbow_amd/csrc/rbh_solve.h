@@ -297,11 +297,18 @@ template <typename R> struct GlbAcc {
 };
 
 // idx-th set bit of a palette of pw words, or -1
+// (palettes of large units live in global memory and are changed by other
+// wavefronts' atomics in L2: read them past the vector L1)
+__device__ __forceinline__ unsigned pal_word(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ int pal_select(const unsigned* p, int pw, int idx) {
   for (int w = 0; w < pw; ++w) {
-    int c = __popc(p[w]);
+    const unsigned pwv = pal_word(p + w);
+    int c = __popc(pwv);
     if (idx < c) {
-      unsigned m = p[w];
+      unsigned m = pwv;
       for (int i = 0; i < idx; ++i) m &= m - 1;   // drop the lowest set bits
       return w * 32 + __ffs(m) - 1;
     }
@@ -363,17 +370,27 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
       if (ok) a.set_done(i, 2);
     }
     __syncthreads();
-    // winners leave their colour out of the palettes of neighbours uncoloured at round start
+    // winners leave their colour out of the palettes of neighbours uncoloured
+    // at round start (no-return atomics: nothing waits on them) ...
     for (int i = threadIdx.x; i < K; i += NT) {
       if (a.done(i) != 2) continue;
       int ci = a.col(i);
       unsigned bit = 1u << (ci & 31);
       a.nb(i, [&](int j) {
-        if (a.done(j) != 1) {
-          unsigned old = atomicAnd(&pal[(size_t)j * pw + (ci >> 5)], ~bit);
-          if (old & bit) a.dec_pc(j);
-        }
+        if (a.done(j) != 1)
+          (void)__hip_atomic_fetch_and(&pal[(size_t)j * pw + (ci >> 5)], ~bit, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
       });
+    }
+    __syncthreads();
+    // ... and the palette sizes of contacts still uncoloured are recounted
+    // (a size always equals its palette's popcount)
+    for (int i = threadIdx.x; i < K; i += NT) {
+      if (a.done(i)) continue;
+      const unsigned* p = pal + (size_t)i * pw;
+      int n = 0;
+      for (int w = 0; w < pw; ++w) n += __popc(pal_word(p + w));
+      a.set_pc(i, n);
     }
     __syncthreads();
     // commit winners; island maxima over kept and tentative colours
@@ -402,7 +419,8 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
           if (j != i && !a.done(j) && a.pc(j) == 0 && a.isl(j) == il && a.key(j) < ki) ++rank;
         int c = d.isl_max[il] + 1 + rank;
         if (c >= pcap) { atomicOr(d.err, ERR_COLORS); continue; }
-        pal[(size_t)i * pw + (c >> 5)] |= 1u << (c & 31);
+        (void)__hip_atomic_fetch_or(&pal[(size_t)i * pw + (c >> 5)], 1u << (c & 31), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
       }
       __syncthreads();
       for (int i = threadIdx.x; i < K; i += NT)

@@ -137,6 +137,36 @@ def test_mixed_scene_fp64(bank, seed):
     assert total > 0 and exact >= 3
 
 
+def sphere_row(bank, n=7, gap=0.98, sink=0.03, params=None):
+    """n spheres in a touching row resting on the ground: one island of a few
+    hundred contacts (ground contacts of every sphere plus sphere pairs)."""
+    eng = scenes.new_engine(bank, params)
+    scenes.set_friction(eng, 0.4)
+    API.create_gravity_force(eng, "gravity", 9.81, np.array([0.0, 1.0, 0.0]))
+    scenes.add_ground(eng, bank)
+    sph = bank.get("sphere_shape", lambda: MESH.create_sphere(0.5, 16, 16))
+    for i in range(n):
+        scenes.add_body(eng, "s%d" % i, sph, np.array([(i - (n - 1) / 2.0) * gap, 0.5 - sink, 0.0]),
+                        q=np.array([1.0, 0.0, 0.0, 0.0]))
+    return eng
+
+
+def test_big_island_fp64(bank):
+    """An island larger than a one-wavefront colouring unit (palettes in global
+    memory) is coloured and solved exactly as by the oracle."""
+    eng = sphere_row(bank, params=dict(precision="fp64", max_iterations=30, proximal_solver="parallel_gauss_seidel"))
+    sc = OS.from_engine(eng)
+    exact = 0
+    for _ in range(2):
+        m = {}
+        out = OST.step(sc, 0.01, m)
+        c, state, lg = gpu_step(eng, 0.01)
+        assert len(c["ia"]) > 300, len(c["ia"])
+        exact += assert_fp64_parity(c, state, lg, out, sc, m)
+        resync(eng, sc)
+    assert exact >= 1
+
+
 @pytest.mark.parametrize("scheme,friction", [("gauss_seidel", "isotropic"), ("parallel_jacobi", "isotropic"),
                                              ("parallel_jacobi_hybrid", "isotropic"),
                                              ("parallel_gauss_seidel", "anisotropic")])

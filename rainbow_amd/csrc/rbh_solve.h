@@ -246,9 +246,12 @@ template <typename S> struct LdsAcc {
   __device__ int pc(int i) const { return s->pc[i]; }
   __device__ void set_pc(int i, int v) { s->pc[i] = (unsigned short)v; }
   __device__ void dec_pc(int i) { atomicSub(reinterpret_cast<unsigned*>(&s->pc[i & ~1]), (i & 1) ? 0x10000u : 1u); }
+  // (unrolled so that the list reads of several neighbours are in flight together)
   template <typename F> __device__ void nb(int i, F&& fn) const {
     int fa = s->fa[i];
+#pragma unroll 8
     for (int e = s->ao[i], n = s->ao[i] + s->ac[i]; e < n; ++e) { int j = s->list[e]; if (j != i) fn(j); }
+#pragma unroll 8
     for (int e = s->bo[i], n = s->bo[i] + s->bcn[i]; e < n; ++e) {
       int j = s->list[e];
       if (j == i) continue;

@@ -333,10 +333,15 @@ template <int NT, typename R, typename A>
 __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw, unsigned long long seed,
                              unsigned long long stepk, int* sh, int* s_flag, int* s_mx) {
   const int pcap = 32 * pw;
+  // words of the palettes that can hold a colour: up to the largest degree,
+  // raised when an empty palette grows (kept in *s_mx until the rounds end)
+  if (threadIdx.x == 0) *s_mx = 0;
+  __syncthreads();
   // palettes {0, ..., deg-1}
   for (int i = threadIdx.x; i < K; i += NT) {
     int deg = 0;
     a.nb(i, [&](int) { ++deg; });
+    if (deg > 0) atomicMax(s_mx, min(deg, pcap));
     unsigned* p = pal + (size_t)i * pw;
     for (int w = 0; w < pw; ++w) {
       int lo = w * 32;
@@ -351,6 +356,7 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
     a.set_done(i, 0);
   }
   __syncthreads();
+  int pwe = (*s_mx + 31) / 32;   // palette words in use
   int rnd = 0;
   for (; rnd < MAX_ROUNDS; ++rnd) {
     int left = 0;
@@ -359,7 +365,7 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
       ++left;
       int n = a.pc(i);
       if (n > 0)
-        a.set_col(i, pal_select(pal + (size_t)i * pw, pw,
+        a.set_col(i, pal_select(pal + (size_t)i * pw, pwe,
                                 (int)color_draw(seed, stepk, (unsigned long long)a.key(i), (unsigned long long)rnd, (unsigned)n)));
     }
     if (block_sum<NT>(left, sh) == 0) break;
@@ -392,7 +398,7 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
       if (a.done(i)) continue;
       const unsigned* p = pal + (size_t)i * pw;
       int n = 0;
-      for (int w = 0; w < pw; ++w) n += __popc(pal_word(p + w));
+      for (int w = 0; w < pwe; ++w) n += __popc(pal_word(p + w));
       a.set_pc(i, n);
     }
     __syncthreads();
@@ -424,8 +430,10 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
         if (c >= pcap) { atomicOr(d.err, ERR_COLORS); continue; }
         (void)__hip_atomic_fetch_or(&pal[(size_t)i * pw + (c >> 5)], 1u << (c & 31), __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
+        atomicMax(s_mx, c + 1);
       }
       __syncthreads();
+      pwe = (*s_mx + 31) / 32;
       for (int i = threadIdx.x; i < K; i += NT)
         if (!a.done(i) && a.pc(i) == 0) a.set_pc(i, 1);
       __syncthreads();

@@ -221,6 +221,8 @@ constexpr int SMALL_LIST = 1024;
 constexpr int SMALL_PAL = SMALL_UNIT * ((2 * SMALL_UNIT + 2 + 31) / 32);
 
 template <int NT, int CAP, int LCAP, int PAL> struct ColorLds {
+  // colours a palette can hold (at most PALMAX in either palette layout)
+  static constexpr int CC = PALMAX;
   int key[CAP];                       // contact index within its scene (random draws, ranks)
   int isl[CAP];
   int fa[CAP], fb[CAP];               // free flat bodies (-1 if fixed)
@@ -229,6 +231,11 @@ template <int NT, int CAP, int LCAP, int PAL> struct ColorLds {
   unsigned short pc[CAP];             // palette sizes
   unsigned char done[CAP];            // 0 uncoloured, 1 coloured, 2 won this round
   unsigned short list[LCAP];
+  unsigned short bl_off[LCAP], bl_cnt[LCAP];   // the unit's free bodies: their contact lists in `list`
+  unsigned char lose[CAP];            // drew a colour that another contact of one of its bodies drew
+  unsigned seen[NT / 64][CC / 32];    // per wavefront, at the body in hand: colours drawn ...
+  unsigned dup[NT / 64][CC / 32];     // ... drawn twice (conflict check), or won (palette removal)
+  int nbody;
   unsigned pal[PAL > 0 ? PAL : 1];    // palettes (PAL > 0)
   int nlist, flag, cnt, mx;
   int sh[NT / 64];
@@ -236,6 +243,7 @@ template <int NT, int CAP, int LCAP, int PAL> struct ColorLds {
 
 // colouring state in LDS (unit fits) ...
 template <typename S> struct LdsAcc {
+  static constexpr bool PER_BODY = true;
   S* s;
   __device__ int key(int i) const { return s->key[i]; }
   __device__ int isl(int i) const { return s->isl[i]; }
@@ -263,6 +271,7 @@ template <typename S> struct LdsAcc {
 
 // ... or in global memory (unit larger than the LDS capacity)
 template <typename R> struct GlbAcc {
+  static constexpr bool PER_BODY = false;
   const Dev<R>* d;
   int p0;      // first unit-order position of the unit
   int k_off;   // first contact of the scene
@@ -336,12 +345,18 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
   // words of the palettes that can hold a colour: up to the largest degree,
   // raised when an empty palette grows (kept in *s_mx until the rounds end)
   if (threadIdx.x == 0) *s_mx = 0;
+  if constexpr (A::PER_BODY) {
+    auto* S = a.s;
+    constexpr int CW = sizeof(S->seen[0]) / sizeof(unsigned);
+    for (int k = threadIdx.x & 63; k < CW; k += 64) { S->seen[threadIdx.x >> 6][k] = 0; S->dup[threadIdx.x >> 6][k] = 0; }
+  }
   __syncthreads();
   // palettes {0, ..., deg-1}
   for (int i = threadIdx.x; i < K; i += NT) {
     int deg = 0;
     a.nb(i, [&](int) { ++deg; });
     if (deg > 0) atomicMax(s_mx, min(deg, pcap));
+    if constexpr (A::PER_BODY) a.s->lose[i] = 0;
     unsigned* p = pal + (size_t)i * pw;
     for (int w = 0; w < pw; ++w) {
       int lo = w * 32;
@@ -371,25 +386,93 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
     if (block_sum<NT>(left, sh) == 0) break;
     __syncthreads();
     // keep the draw if no uncoloured neighbour drew the same colour
-    for (int i = threadIdx.x; i < K; i += NT) {
-      if (a.done(i)) continue;
-      int ci = a.col(i);
-      bool ok = true;
-      a.nb(i, [&](int j) { if (a.done(j) != 1 && a.col(j) == ci) ok = false; });
-      if (ok) a.set_done(i, 2);
+    if constexpr (A::PER_BODY) {
+      // neighbours are the contacts sharing a free body: a draw is lost when
+      // another uncoloured contact of one of its bodies drew the same colour.
+      // Wavefronts take whole bodies and sweep their contact lists 64 at a time.
+      auto* S = a.s;
+      const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, pwr = max(pwe, 1);
+      unsigned* seen = S->seen[w];
+      unsigned* dup = S->dup[w];
+      for (int b = w; b < S->nbody; b += NT / 64) {
+        const int off = S->bl_off[b], n = S->bl_cnt[b];
+        for (int e = lane; e < n; e += 64) {
+          const int j = S->list[off + e];
+          if (a.done(j)) continue;
+          const int c = a.col(j);
+          const unsigned bit = 1u << (c & 31);
+          if (atomicOr(&seen[c >> 5], bit) & bit) atomicOr(&dup[c >> 5], bit);
+        }
+        wave_sync_lds();
+        for (int e = lane; e < n; e += 64) {
+          const int j = S->list[off + e];
+          if (a.done(j)) continue;
+          const int c = a.col(j);
+          if (dup[c >> 5] & (1u << (c & 31))) S->lose[j] = 1;
+        }
+        wave_sync_lds();
+        for (int k = lane; k < pwr; k += 64) { seen[k] = 0; dup[k] = 0; }
+        wave_sync_lds();
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < K; i += NT) {
+        if (a.done(i)) continue;
+        if (!S->lose[i]) a.set_done(i, 2);
+        S->lose[i] = 0;
+      }
+    } else {
+      for (int i = threadIdx.x; i < K; i += NT) {
+        if (a.done(i)) continue;
+        int ci = a.col(i);
+        bool ok = true;
+        a.nb(i, [&](int j) { if (a.done(j) != 1 && a.col(j) == ci) ok = false; });
+        if (ok) a.set_done(i, 2);
+      }
     }
     __syncthreads();
     // winners leave their colour out of the palettes of neighbours uncoloured
     // at round start (no-return atomics: nothing waits on them) ...
-    for (int i = threadIdx.x; i < K; i += NT) {
-      if (a.done(i) != 2) continue;
-      int ci = a.col(i);
-      unsigned bit = 1u << (ci & 31);
-      a.nb(i, [&](int j) {
-        if (a.done(j) != 1)
-          (void)__hip_atomic_fetch_and(&pal[(size_t)j * pw + (ci >> 5)], ~bit, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-      });
+    if constexpr (A::PER_BODY) {
+      // per body: the colours won by its contacts leave the palettes of its
+      // uncoloured contacts (a winner's own palette is no longer read)
+      auto* S = a.s;
+      const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, pwr = max(pwe, 1);
+      unsigned* won = S->dup[w];
+      for (int b = w; b < S->nbody; b += NT / 64) {
+        const int off = S->bl_off[b], n = S->bl_cnt[b];
+        bool any = false;
+        for (int e = lane; e < n; e += 64) {
+          const int j = S->list[off + e];
+          if (a.done(j) != 2) continue;
+          const int c = a.col(j);
+          atomicOr(&won[c >> 5], 1u << (c & 31));
+          any = true;
+        }
+        if (!__ballot(any)) continue;
+        wave_sync_lds();
+        for (int e = lane; e < n; e += 64) {
+          const int j = S->list[off + e];
+          if (a.done(j)) continue;
+          for (int k = 0; k < pwr; ++k) {
+            const unsigned m = won[k];
+            if (m) (void)__hip_atomic_fetch_and(&pal[(size_t)j * pw + k], ~m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        wave_sync_lds();
+        for (int k = lane; k < pwr; k += 64) won[k] = 0;
+        wave_sync_lds();
+      }
+    } else {
+      for (int i = threadIdx.x; i < K; i += NT) {
+        if (a.done(i) != 2) continue;
+        int ci = a.col(i);
+        unsigned bit = 1u << (ci & 31);
+        a.nb(i, [&](int j) {
+          if (a.done(j) != 1)
+            (void)__hip_atomic_fetch_and(&pal[(size_t)j * pw + (ci >> 5)], ~bit, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        });
+      }
     }
     __syncthreads();
     // ... and the palette sizes of contacts still uncoloured are recounted
@@ -486,7 +569,7 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
   bool fits = K <= CAP;
   if (fits) {
     // stage contacts and the bodies' contact lists (claimed once per body)
-    if (threadIdx.x == 0) { s.nlist = 0; s.flag = 0; }
+    if (threadIdx.x == 0) { s.nlist = 0; s.flag = 0; s.nbody = 0; }
     __syncthreads();
     for (int i = threadIdx.x; i < K; i += NT) {
       int k = d.uperm[p0 + i];
@@ -510,6 +593,9 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
         if (off + n > LCAP) { atomicOr(&s.flag, 1); continue; }
         for (int e = 0; e < n; ++e) s.list[off + e] = (unsigned short)(d.bc_list[d.bc_off[f] + e] - p0);
         d.body_loc[f] = make_int2(off, n);
+        const int b = atomicAdd(&s.nbody, 1);   // b < LCAP: every body holds at least one list entry
+        s.bl_off[b] = (unsigned short)off;
+        s.bl_cnt[b] = (unsigned short)n;
       }
     }
     __threadfence_block();

@@ -485,14 +485,12 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
       a.set_pc(i, n);
     }
     __syncthreads();
-    // commit winners; island maxima over kept and tentative colours
-    for (int i = threadIdx.x; i < K; i += NT) d.isl_max[a.isl(i)] = -1;
+    // commit winners
     if (threadIdx.x == 0) *s_flag = 0;
     __syncthreads();
     int moving = 0;   // a winner, or an uncoloured node whose next draw is not forced
     for (int i = threadIdx.x; i < K; i += NT) {
       if (a.done(i) == 2) { a.set_done(i, 1); moving = 1; }
-      atomicMax(&d.isl_max[a.isl(i)], a.col(i));
       if (!a.done(i) && a.pc(i) == 0) atomicOr(s_flag, 1);
       if (!a.done(i) && a.pc(i) != 1) moving = 1;
     }
@@ -503,7 +501,12 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
     if (block_sum<NT>(moving, sh) == 0) { rnd = MAX_ROUNDS; break; }
     if (*s_flag) {
       // rare: empty palettes get max+1, max+2, ... in increasing contact order
-      // per island; ranks read only done/pc/isl/key, which stay fixed here
+      // per island (max over the island's kept and tentative colours); ranks
+      // read only done/pc/isl/key, which stay fixed here
+      for (int i = threadIdx.x; i < K; i += NT) d.isl_max[a.isl(i)] = -1;
+      __syncthreads();
+      for (int i = threadIdx.x; i < K; i += NT) atomicMax(&d.isl_max[a.isl(i)], a.col(i));
+      __syncthreads();
       for (int i = threadIdx.x; i < K; i += NT) {
         if (a.done(i) || a.pc(i) != 0) continue;
         int il = a.isl(i), ki = a.key(i), rank = 0;

@@ -217,6 +217,7 @@ constexpr int COLOR_BLOCK = 256;
 constexpr int UNIT_CAP = 2048;        // contacts of a large unit staged in LDS
 constexpr int LIST_CAP = 4096;        // body-list entries staged in LDS
 constexpr int SMALL_UNIT = 256;
+constexpr int SMALL_BODY = 32;        // bodies with at most this many contacts are checked entry by entry
 constexpr int SMALL_LIST = 1024;
 constexpr int SMALL_PAL = SMALL_UNIT * ((2 * SMALL_UNIT + 2 + 31) / 32);
 
@@ -231,7 +232,8 @@ template <int NT, int CAP, int LCAP, int PAL> struct ColorLds {
   unsigned short pc[CAP];             // palette sizes
   unsigned char done[CAP];            // 0 uncoloured, 1 coloured, 2 won this round
   unsigned short list[LCAP];
-  unsigned short bl_off[LCAP], bl_cnt[LCAP];   // the unit's free bodies: their contact lists in `list`
+  unsigned short bl_off[LCAP], bl_cnt[LCAP];   // bodies with more than SMALL_BODY contacts: their lists
+  unsigned short ent_off[LCAP], ent_cnt[LCAP]; // per list entry of a smaller body: that body's list (cnt 0: larger body)
   unsigned char lose[CAP];            // drew a colour that another contact of one of its bodies drew
   unsigned seen[NT / 64][CC / 32];    // per wavefront, at the body in hand: colours drawn ...
   unsigned dup[NT / 64][CC / 32];     // ... drawn twice (conflict check), or won (palette removal)
@@ -394,6 +396,19 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
       const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, pwr = max(pwe, 1);
       unsigned* seen = S->seen[w];
       unsigned* dup = S->dup[w];
+      // smaller bodies: every list entry compares its draw with its body's others
+      for (int e = threadIdx.x; e < S->nlist; e += NT) {
+        const int cnt = S->ent_cnt[e];
+        if (!cnt) continue;
+        const int j = S->list[e];
+        if (a.done(j)) continue;
+        const int c = a.col(j), o = S->ent_off[e];
+        for (int e2 = o; e2 < o + cnt; ++e2) {
+          const int j2 = S->list[e2];
+          if (j2 != j && !a.done(j2) && a.col(j2) == c) { S->lose[j] = 1; break; }
+        }
+      }
+      // larger bodies: a wavefront per body
       for (int b = w; b < S->nbody; b += NT / 64) {
         const int off = S->bl_off[b], n = S->bl_cnt[b];
         for (int e = lane; e < n; e += 64) {
@@ -438,6 +453,22 @@ __device__ void color_rounds(const Dev<R>& d, A& a, int K, unsigned* pal, int pw
       auto* S = a.s;
       const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, pwr = max(pwe, 1);
       unsigned* won = S->dup[w];
+      // smaller bodies: every uncoloured list entry clears the colours won by its body's others
+      for (int e = threadIdx.x; e < S->nlist; e += NT) {
+        const int cnt = S->ent_cnt[e];
+        if (!cnt) continue;
+        const int j = S->list[e];
+        if (a.done(j)) continue;
+        const int o = S->ent_off[e];
+        for (int e2 = o; e2 < o + cnt; ++e2) {
+          const int j2 = S->list[e2];
+          if (a.done(j2) != 2) continue;
+          const int c = a.col(j2);
+          (void)__hip_atomic_fetch_and(&pal[(size_t)j * pw + (c >> 5)], ~(1u << (c & 31)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      // larger bodies: a wavefront per body
       for (int b = w; b < S->nbody; b += NT / 64) {
         const int off = S->bl_off[b], n = S->bl_cnt[b];
         bool any = false;
@@ -596,9 +627,15 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
         if (off + n > LCAP) { atomicOr(&s.flag, 1); continue; }
         for (int e = 0; e < n; ++e) s.list[off + e] = (unsigned short)(d.bc_list[d.bc_off[f] + e] - p0);
         d.body_loc[f] = make_int2(off, n);
-        const int b = atomicAdd(&s.nbody, 1);   // b < LCAP: every body holds at least one list entry
-        s.bl_off[b] = (unsigned short)off;
-        s.bl_cnt[b] = (unsigned short)n;
+        if (n > SMALL_BODY) {
+          const int b = atomicAdd(&s.nbody, 1);   // b < LCAP: every body holds at least one list entry
+          s.bl_off[b] = (unsigned short)off;
+          s.bl_cnt[b] = (unsigned short)n;
+        }
+        for (int e = 0; e < n; ++e) {
+          s.ent_off[off + e] = (unsigned short)off;
+          s.ent_cnt[off + e] = (unsigned short)(n > SMALL_BODY ? 0 : n);
+        }
       }
     }
     __threadfence_block();

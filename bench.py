@@ -29,10 +29,11 @@ of the same scenes, whose end states must equal the batched ones.
 
 serving (on by default, --no-serving to skip): Poisson arrivals of
 sphere-drop requests (8..1024 bodies, 50..500 steps) admitted against an HBM
-budget and advanced by continuous batching, at 10, 50 and 200 requests/s and
-at saturation; p50/p99 per-step latency, completed scene-steps/s, admission
-wait, the plain path (one request per launch) on a sample of the requests,
-and the alone-vs-mixed difference of a few requests.
+budget and advanced by continuous batching, 64 requests at 10, 50 and 200
+requests/s, and 256 requests all waiting at t = 0 (saturation); launch shapes
+from the autotuner; p50/p99 per-step latency, completed scene-steps/s,
+admission wait, the plain path (one request per launch) on a sample of the
+requests, and the alone-vs-mixed difference of a few requests.
 
 stress (on by default, --no-stress to skip): 128 noisy icospheres and 128
 torus knots (about 10^4 triangles each, 64^3 SDFs) poured into a fixed funnel,
@@ -195,39 +196,55 @@ def tuning_report(res):
             "trials": [[keep(t), round(ms, 4)] for t, ms in res["trials"]]}
 
 
-def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, plain_steps=10):
+def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, plain_steps=10, sat_requests=256):
     """Poisson arrivals served by continuous batching, one run per arrival rate
-    (0 = saturation: every request waiting at t = 0), on the same requests;
-    the plain path (one request per launch, no graph) on a sample of them."""
+    on the same n_req requests, and a saturation run (every one of
+    sat_requests requests waiting at t = 0; the first n_req are the same
+    requests); the plain path (one request per launch, no graph) on a sample.
+    Launch shapes come from the autotuner, run on the 32 largest requests
+    after 300 steps (when their bodies have piled up)."""
     import numpy as np
-    from rainbow_amd import scenes, serving
+    from rainbow_amd import autotune, native, scenes, serving
     t0 = time.perf_counter()
     bank = scenes.ShapeBank(resolution=32)
     rng = np.random.default_rng(4)
     srv = serving.Server(budget=budget_gib << 30)
     reqs = []
-    for i in range(n_req):
+    for i in range(max(n_req, sat_requests)):
         eng = scenes.serving_request(4000 + i, bank)
         reqs.append(srv.prepare(serving.Request(i, eng, int(rng.integers(50, 501)), 0.0)))
     log("serving: %d requests (%d bodies, %d scene-steps) prepared in %.1f s" % (
-        n_req, sum(len(r.engine.bodies) for r in reqs), sum(r.n_steps for r in reqs), time.perf_counter() - t0))
+        len(reqs), sum(len(r.engine.bodies) for r in reqs), sum(r.n_steps for r in reqs), time.perf_counter() - t0))
+    ctx = srv.ctx
+    # launch shapes for the served batches
+    tsids = []
+    big = sorted(reqs, key=lambda r: -len(r.engine.bodies))[:32]
+    for r in big:
+        sid = ctx.scene_create(r._desc)
+        ctx.set_state(sid, *r._state0)
+        tsids.append(sid)
+    ctx.step(tsids, 300, 0.0, native.STEP_GRAPH)
+    ctx.sync()
+    tuned = autotune.autotune(ctx, tsids, [len(r.engine.bodies) for r in big], steps=3, log=log)
+    for sid in tsids:
+        ctx.scene_destroy(sid)
     runs = []
-    for rate in rates:
-        arr = serving.poisson_arrivals(rate, n_req, seed=4) if rate > 0 else np.zeros(n_req)
-        for r, a in zip(reqs, arr):
+    for rate in list(rates) + ["saturation"]:
+        sub = reqs[:n_req] if rate != "saturation" else reqs[:sat_requests]
+        arr = serving.poisson_arrivals(rate, len(sub), seed=4) if rate != "saturation" else np.zeros(len(sub))
+        for r, a in zip(sub, arr):
             r.arrival = float(a)
-        rep = srv.run(reqs)
-        rep["rate_per_s"] = rate if rate > 0 else "saturation"
+        rep = srv.run(sub)
+        rep["rate_per_s"] = rate
         runs.append(rep)
-        log("serving at %s req/s: %.0f scene-steps/s, p50 %.2f ms, p99 %.2f ms, wait p99 %.1f ms" % (
-            rep["rate_per_s"], rep["scene_steps_per_s"], rep["p50_step_ms"], rep["p99_step_ms"], rep["p99_wait_ms"]))
+        log("serving at %s req/s (%d requests): %.0f scene-steps/s, p50 %.2f ms, p99 %.2f ms, wait p99 %.1f ms" % (
+            rate, len(sub), rep["scene_steps_per_s"], rep["p50_step_ms"], rep["p99_step_ms"], rep["p99_wait_ms"]))
     diffs = []
     for r in reqs[:alone_checks]:
         alone = srv.run_alone(r)
         diffs.append(max(float(np.abs(a - b).max()) for a, b in zip(alone, r.state)))
     # plain path: a sample of the requests, each stepped alone (one scene per launch, no graph)
-    sample = reqs[::max(1, n_req // plain_sample)][:plain_sample]
-    ctx = srv.ctx
+    sample = reqs[:n_req][::max(1, n_req // plain_sample)][:plain_sample]
     sids = []
     for r in sample:
         sid = ctx.scene_create(r._desc)
@@ -243,14 +260,14 @@ def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, plain_st
     tp = time.perf_counter() - t1
     for sid in sids:
         ctx.scene_destroy(sid)
+    ctx.set_tuning(**autotune.DEFAULT)
     plain = len(sids) * plain_steps / tp
-    sat = [r for r in runs if r["rate_per_s"] == "saturation"]
-    best = sat[0] if sat else max(runs, key=lambda r: r["scene_steps_per_s"])
-    return {"workload": "sphere_drop_requests_%d" % n_req, "requests": n_req,
-            "scene_steps": sum(r.n_steps for r in reqs), "budget_bytes": budget_gib << 30, "runs": runs,
+    sat = runs[-1]
+    return {"workload": "sphere_drop_requests", "requests": n_req, "saturation_requests": sat_requests,
+            "budget_bytes": budget_gib << 30, "tuning": tuning_report(tuned), "runs": runs,
             "plain": {"scene_steps_per_s": plain, "sample_requests": len(sids), "steps": plain_steps,
                       "ms_per_scene_step": 1e3 * tp / (len(sids) * plain_steps)},
-            "speedup_vs_plain_at_saturation": best["scene_steps_per_s"] / plain if plain > 0 else None,
+            "speedup_vs_plain_at_saturation": sat["scene_steps_per_s"] / plain if plain > 0 else None,
             "max_abs_diff_alone_vs_mixed": max(diffs) if diffs else None, "alone_checked": len(diffs)}
 
 
@@ -303,8 +320,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--batched-scenes", type=int, default=1024)
     ap.add_argument("--serving-requests", type=int, default=64)
-    ap.add_argument("--serving-rates", default="10,50,200,0",
-                    help="arrival rates in requests/s, 0 = saturation")
+    ap.add_argument("--serving-rates", default="10,50,200",
+                    help="arrival rates in requests/s (a saturation run follows)")
     ap.add_argument("--no-stress", action="store_true", help="skip the funnel stress leg")
     ap.add_argument("--stress-steps", type=int, default=300)
     ap.add_argument("--only", choices=("batched", "serving", "stress"), default=None,

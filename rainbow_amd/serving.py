@@ -58,14 +58,18 @@ class Server:
     ctx: a native.Context (FP32 by default); budget: bytes of device memory
     the admitted scenes may occupy (rbh_scene_footprint per scene);
     graph: replay the step as a hipGraph while the batch composition is
-    unchanged; max_batch: cap on concurrently admitted scenes.
+    unchanged; max_batch: cap on concurrently admitted scenes; tuning: launch
+    shapes (rbh_tuning fields) set on the context for the served batches.
     """
 
-    def __init__(self, ctx=None, budget=64 << 30, graph=True, max_batch=None, device=0, precision="fp32"):
+    def __init__(self, ctx=None, budget=64 << 30, graph=True, max_batch=None, device=0, precision="fp32",
+                 tuning=None):
         self.ctx = ctx or shared_context(device, precision)
         self.budget = int(budget)
         self.graph = bool(graph)
         self.max_batch = max_batch
+        if tuning:
+            self.ctx.set_tuning(**tuning)
 
     def prepare(self, req):
         """Build the device description of a request's scene (host work done ahead of arrival)."""

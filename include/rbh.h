@@ -71,7 +71,7 @@ typedef struct rbh_params {
   int use_pre_stabilization;              /* Parameters.use_pre_stabilization */
   int proximal_solver;                    /* RBH_SCHEME_* (soft-body Parameters.proximal_solver) */
   int friction_model;                     /* RBH_FRICTION_* */
-  int pad;
+  int use_post_stabilization;             /* Parameters.use_post_stabilization (steppers.py:15-36, 90-91) */
   unsigned long long coloring_seed;       /* seed of the random colouring */
 } rbh_params;
 
@@ -106,6 +106,8 @@ typedef struct rbh_step_log {
   double natural_residual;     /* ||x - prox(x - r0 (A x + b))||_inf / max(1, ||x||_inf), FP64 */
   double max_penetration;      /* Contacts.get_largest_penetration */
   double final_merit;          /* last accepted merit ||x - sol||_inf */
+  int post_iterations;         /* post-stabilisation solve 'post_stabilization_iterations' (0: not run) */
+  int post_exitcode;           /* 'post_stabilization_exitcode'; -1 when the step ran no post-stabilisation */
 } rbh_step_log;
 
 /* Launch shapes of the step (0 = default).  No reference counterpart: these

@@ -6,8 +6,12 @@ velocity_update (state_storage.py:239-248) -> position_update (:251-302:
 r += v dt; q <- unit(q + 0.5 dt (0, w) q), free bodies only) -> energies
 (:188-237) and the largest penetration (contacts.py:223-236) for the log.
 
-Post-stabilisation (steppers.py:15-36, off by default) and hinges are outside
-this build's scope; ``step`` raises NotImplementedError when they are enabled.
+With use_post_stabilization the step ends with ``post_stabilize``
+(steppers.py:15-36, 90-91; constraints/post_stabilization.py), pinned by
+tests/test_oracle_stepper.py: a sphere sunk into the ground is lifted by
+exactly its deepest penetration without turning, and contacts shallower than
+min_gap_value leave the scene untouched.  Hinges are outside this build's
+scope.
 """
 import numpy as np
 
@@ -57,14 +61,50 @@ def integrate(scene, delta_u, dt):
         scene.q[k] = Q.unit(q)
 
 
+def post_stabilize(scene, P, groups, prm):
+    """steppers.py:15-36 with constraints/post_stabilization.py:21-104.
+
+    The contact problem's J, W J^T and diag are kept; the right-hand side is
+    the gap g of contacts with g < -min_gap_value on the normal row (zero
+    elsewhere); r = 1/diag; the sweep projects x_n onto R+ and sets the
+    friction rows to zero (here: mu = 0, whose projections are the origin);
+    the solve driver is the contact solve's (gauss_seidel_solver.solve, same
+    sweep order); delta_r = W J^T sol is applied by position_update with
+    dt = 1 (state_storage.py:251-302).  Returns the solve record, or None
+    when no contact is deep enough (the reference returns before solving).
+    """
+    g = np.where(P.gap < -prm["min_gap_value"], P.gap, 0.0)
+    if not g.any():
+        return None
+    Q2 = problem.Problem()
+    for name, val in vars(P).items():
+        setattr(Q2, name, val)
+    Q2.b = np.zeros((P.K, 4))
+    Q2.b[:, 0] = g
+    Q2.r = 1.0 / P.diag
+    Q2.r0 = Q2.r.copy()
+    Q2.x = np.zeros((P.K, 4))
+    Q2.sol = np.zeros((P.K, 4))
+    Q2.mu = np.zeros_like(P.mu)
+    Q2.mu3 = np.zeros_like(P.mu3)
+    res = problem.solve(Q2, groups, prm)
+    dr = problem.body_w(Q2, Q2.sol)
+    for k in range(scene.N):
+        if scene.fixed[k]:
+            continue
+        scene.r[k] = scene.r[k] + dr[k, :3] * 1.0
+        q = scene.q[k] + Q.prod(np.array([0.0, *dr[k, 3:]]), scene.q[k]) * 1.0 * 0.5
+        scene.q[k] = Q.unit(q)
+    res["problem"] = Q2
+    return res
+
+
 def step(scene, dt, margins=None):
     """Advance ``scene`` in place by dt.  Returns a dict with the contacts, the
     solved problem, the colouring and the per-step log.  ``margins`` (dict)
     receives the decision margins of the candidate triangles, keyed by
     (SDF owner, triangle owner, triangle)."""
     prm = scene.params
-    if prm.get("use_post_stabilization"):
-        raise NotImplementedError("post-stabilisation is outside this build's scope")
     Winv, f_ext, du_ext = problem.body_mass_blocks(scene)
     delta_u = dt * du_ext
     stats = {}
@@ -82,6 +122,13 @@ def step(scene, dt, margins=None):
                    exitcode=res["exitcode"], rejects=res["rejects"])
         log["natural_residual"] = problem.natural_residual(P, prm)
     integrate(scene, delta_u, dt)
+    if contacts.K > 0 and prm.get("use_post_stabilization"):
+        post = post_stabilize(scene, out["problem"], out["groups"], prm)
+        if post is not None:
+            log.update(post_stabilization_residuals=post["residuals"],
+                       post_stabilization_iterations=post["iterations"],
+                       post_stabilization_exitcode=post["exitcode"])
+            out["post"] = post
     scene.step_index += 1
     kin, pot = energies(scene)
     log.update(dt=dt, contact_points=contacts.K, kinetic_energy=kin, potential_energy=pot,

@@ -132,6 +132,7 @@ struct rbh_ctx {
   size_t solve_lds = 0;
   int max_iter_batch = 1, coop_max = 0, last_units = 0;
   bool multi_ok = false;
+  bool any_post = false;    // a scene of the batch has post-stabilisation on (set per run_steps call)
   long long ws_gen = 0;
   // graph
   hipGraph_t graph = nullptr;
@@ -309,6 +310,7 @@ SceneParams to_device_params(const rbh_params& p, int nmat, int mat_off, int res
   s.max_iter = p.max_iterations; s.fw_max_iter = p.contact_optimization_max_iterations;
   s.ell_max_iter = p.ellipsoid_max_iterations; s.use_bounce = p.use_bounce;
   s.use_pre_stab = p.use_pre_stabilization; s.scheme = p.proximal_solver; s.friction = p.friction_model;
+  s.use_post_stab = p.use_post_stabilization ? 1 : 0;
   s.nmat = nmat; s.mat_off = mat_off; s.res_off = res_off; s.seed = p.coloring_seed;
   return s;
 }
@@ -612,6 +614,26 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   }
   mark(ST_SOLVE);
   hipLaunchKernelGGL(k_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
+  if (c->any_post) {
+    // post-stabilisation (steppers.py:90-91): the same solve on the rewritten
+    // rows, then a position update with dt = 1
+    hipLaunchKernelGGL(k_post_rows<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);
+    Dev<R> dp = d;
+    dp.post = 1;
+    if (multi) {
+      hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(d.merit), 2 * c->max_iter_batch, 0);
+      hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(d.bar), 16, 0);
+      void* args[] = {&dp};
+      hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<void*>(k_solve_multi<R>), dim3(multi_grid),
+                                                dim3(SOLVE_BLOCK), args, c->solve_lds, s);
+      if (e != hipSuccess) return e;
+    } else if (batch_waves(c) == 8) {
+      hipLaunchKernelGGL((k_solve<R, 512>), dim3(ns), dim3(512), c->solve_lds, s, dp);
+    } else {
+      hipLaunchKernelGGL((k_solve<R, SOLVE_BLOCK>), dim3(ns), dim3(SOLVE_BLOCK), c->solve_lds, s, dp);
+    }
+    hipLaunchKernelGGL(k_post_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
+  }
   hipLaunchKernelGGL(k_scene_finalize<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
   mark(ST_INT);
   return hipGetLastError();
@@ -654,6 +676,8 @@ template <typename R> int setup_solve_lds(rbh_ctx* c) {
 }
 
 template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flags) {
+  c->any_post = false;
+  for (int slot : c->batch) c->any_post = c->any_post || c->scenes[slot].params.use_post_stabilization != 0;
   const bool timing = (flags & RBH_STEP_TIMING) != 0;
   const bool use_graph = (flags & RBH_STEP_GRAPH) != 0;
   const int kflags = flags & RBH_STEP_COUNT_OVERLAPS;
@@ -1240,6 +1264,8 @@ int rbh_get_log(rbh_ctx* c, int id, rbh_step_log* log, double* residuals, int ca
     log->natural_residual = st.nat_r / std::max(1.0, st.nat_x);
     log->max_penetration = st.max_pen;
     log->final_merit = st.final_merit;
+    log->post_iterations = st.post_active ? st.post_iters : 0;
+    log->post_exitcode = st.post_active ? st.post_exit : -1;
   }
   if (residuals && cap > 0) {
     int m = std::min(cap, s.params.max_iterations);

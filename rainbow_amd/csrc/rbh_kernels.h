@@ -25,7 +25,7 @@ struct SceneParams {
   double time_step, envelope, abs_tol, rel_tol, nu_reduce, nu_increase, too_small;
   double fw_tol, gap_reduction, min_gap, max_gap, ell_tol, ell_expansion;
   int max_iter, fw_max_iter, ell_max_iter, use_bounce, use_pre_stab, scheme, friction;
-  int nmat, mat_off, res_off, pad;
+  int nmat, mat_off, res_off, use_post_stab;
   unsigned long long seed;
 };
 
@@ -34,7 +34,8 @@ struct SceneState {
   int k_off, k_cnt;    // contacts of this step in the batch contact arrays
   int ncolor;
   int iterations, exitcode, rejects, sweeps;
-  int n_pairs, n_cand, solved, pad;
+  int n_pairs, n_cand, solved, post_active;   // post_active: a contact is deeper than min_gap (post-stabilisation)
+  int post_iters, post_exit;                  // the post-stabilisation solve's iterations and exit code
   unsigned long long n_overlaps;
   unsigned long long fw_evals;   // distinct trilinear cell evaluations in Frank-Wolfe
   double nat_r, nat_x;     // natural-map residual: max |x - prox(...)| and max |x| (FP64)
@@ -119,6 +120,7 @@ template <typename R> struct Dev {
   V4<R>* s_JL; V4<R>* s_JA; V4<R>* s_JB;   // rows 4 pos + r of J: (lin_r, b_r), (angA_r, r0_r), (angB_r, mu_r)
   R* xr0; R* xr1;                      // iterates by row (4 pos + r), for the row-major sweeps
   int* err;                           // [0] flags, [1..] needed capacities
+  int post;                           // 1: the solve kernels run the post-stabilisation problem
   double dt;                           // > 0: use for every scene, else the scene's time_step
   int flags;
 };
@@ -848,6 +850,7 @@ template <typename R> __global__ void k_scene_reset(Dev<R> d) {
   SceneState& st = d.ss[d.bs_slot[s]];
   st.k_off = 0x7fffffff; st.k_cnt = 0; st.ncolor = 0; st.n_overlaps = 0; st.fw_evals = 0; st.max_pen = 0.0;
   st.solved = 0; st.nat_r = 0.0; st.nat_x = 0.0; st.n_cand = 0; st.sweeps = 0;
+  st.post_active = 0; st.post_iters = 0; st.post_exit = 0;
 }
 
 template <typename R> __global__ void k_scene_finalize(Dev<R> d) {

@@ -747,6 +747,67 @@ template <typename R> __device__ __forceinline__ void gather_contact(const Dev<R
   }
 }
 
+// steppers.py:15-36 with constraints/post_stabilization.py:21-41, 99-104: the
+// post-stabilisation problem keeps the contact problem's rows (J, W J^T and
+// r = 1/diag) and sweep order; its right-hand side is the gap of a contact
+// deeper than min_gap_value on the normal row and 0 elsewhere, its friction
+// rows are projected to zero (mu = 0), and it starts from x = 0.  Rewrites
+// the sweep-order arrays of every layout in place (the contact solve is done
+// with them) and flags the scenes that have something to correct.
+template <typename R> __global__ void k_post_rows(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nk = *d.n_contacts;
+  if (nk > d.cap_contacts) return;
+  for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < nk; pos += gridDim.x * blockDim.x) {
+    const int k = d.perm[pos];
+    const int slot = d.b_slot[d.c_ia[k]];
+    const SceneParams& sp = d.sp[slot];
+    if (!sp.use_post_stab) continue;
+    const R g = d.c_p[k].w;
+    const bool deep = (double)g < -sp.min_gap;
+    const R zero = R(0);
+    d.s_b[pos] = V4<R>{deep ? g : zero, zero, zero, zero};
+    V4<R> n = d.s_n[pos], ra = d.s_rA[pos], rb = d.s_rB[pos];
+    n.w = zero; ra.w = zero; rb.w = zero;
+    d.s_n[pos] = n; d.s_rA[pos] = ra; d.s_rB[pos] = rb;
+    d.x0[pos] = V4<R>{zero, zero, zero, zero};
+    d.x1[pos] = V4<R>{zero, zero, zero, zero};
+    for (int r = 0; r < 4; ++r) {
+      const size_t row = 4 * (size_t)pos + r;
+      V4<R> jl = d.s_JL[row], jb = d.s_JB[row];
+      jl.w = (r == 0 && deep) ? g : zero;
+      jb.w = zero;
+      d.s_JL[row] = jl;
+      d.s_JB[row] = jb;
+      d.xr0[row] = zero;
+      d.xr1[row] = zero;
+    }
+    if (deep) d.ss[slot].post_active = 1;
+  }
+}
+
+// state_storage.py:251-302 with dt = 1 and u = W J^T sol of the
+// post-stabilisation problem (steppers.py:33-36): positions only.
+template <typename R> __global__ void k_post_integrate(Dev<R> d) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= d.nb || aborted(d.err)) return;
+  const int b = d.bb_body[f];
+  if (d.b_fixed[b]) return;
+  const int slot = d.b_slot[b];
+  if (!(d.sp[slot].use_post_stab && d.ss[slot].post_active)) return;
+  const R one = R(1);
+  const V4<R> s0 = d.ws0[b], s1 = d.ws1[b];
+  V4<R> r = d.r[b];
+  r.x = r.x + s0.x * one; r.y = r.y + s0.y * one; r.z = r.z + s0.z * one;
+  Quat<R> q = quat(d.q[b]);
+  Quat<R> dq = qprod(Quat<R>{R(0), s1.x, s1.y, s1.z}, q);
+  Quat<R> qn = {q.s + dq.s * one * R(0.5), q.x + dq.x * one * R(0.5), q.y + dq.y * one * R(0.5),
+                q.z + dq.z * one * R(0.5)};
+  qn = qunit(qn);
+  d.r[b] = r;
+  d.q[b] = V4<R>{qn.s, qn.x, qn.y, qn.z};
+}
+
 // colour offsets of lane l: 2 K_l + 2 entries (an island's colours stay below
 // twice its size, which the colouring checks)
 template <typename R> __device__ __forceinline__ const int* lane_coff(const Dev<R>& d, int l) {
@@ -923,6 +984,11 @@ struct SolveCtl {
 };
 
 template <typename R> __device__ void write_solve_state(const Dev<R>& d, SceneState& st, const SolveCtl& c) {
+  if (d.post) {   // the post-stabilisation solve keeps the contact solve's record
+    st.post_iters = c.iters;
+    st.post_exit = c.exitcode;
+    return;
+  }
   st.solved = 1;
   st.iterations = c.iters;
   st.exitcode = c.exitcode;
@@ -1133,7 +1199,7 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
       if (!grid_barrier(d.bar, gridDim.x, d.err, s_ok)) return;
       merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     }
-    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0) ? res : nullptr, it);
+    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it);
     // accepted sweep: ws <- ww, else ww <- ws
     for (int i = threadIdx.x; i < nbody; i += blockDim.x) {
       const int s = LDS ? i : d.u_blist[b0 + i];
@@ -1155,7 +1221,7 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
       Item<R> c;
       load_item(d, xs, kk, c);
       const V4<R> x = c.xo;
-      d.c_imp[d.perm[kk]] = x;
+      if (!d.post) d.c_imp[d.perm[kk]] = x;
       ContactRow<R> J;
       contact_rows(c.n, c.rA, c.rB, J);
       const V4<R> mu = {c.n.w, c.rA.w, c.rB.w, R(0)};
@@ -1177,7 +1243,9 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
     }
   const double rm = block_max(rloc, sh_red), xm = block_max(xmax, sh_red);
   if (threadIdx.x == 0) {
-    if (COOP) {
+    if (d.post) {
+      // the natural-map residual stays the contact solve's
+    } else if (COOP) {
       if (rm > 0.0) atomic_max_nonneg(&st.nat_r, rm);
       if (xm > 0.0) atomic_max_nonneg(&st.nat_x, xm);
     } else {
@@ -1438,7 +1506,7 @@ __device__ __forceinline__ int sweep_decide(const Dev<R>& d, const SceneParams& 
     merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   }
   double* res = d.residuals + sp.res_off;
-  const int dec = ctl.decide(sp, merit, (lead && threadIdx.x == 0) ? res : nullptr, it);
+  const int dec = ctl.decide(sp, merit, (lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it);
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     if (dec & 1) { B.sl[i] = B.wl[i]; B.sa[i] = B.wa[i]; }
     else { B.wl[i] = B.sl[i]; B.wa[i] = B.sa[i]; }
@@ -1469,7 +1537,9 @@ __device__ void finish_group(const Dev<R>& d, SceneState& st, const SolveCtl& ct
   }
   const double rm = block_max(rloc, sh_red), xm = block_max(xmax, sh_red);
   if (threadIdx.x == 0) {
-    if (COOP) {
+    if (d.post) {
+      // the natural-map residual stays the contact solve's
+    } else if (COOP) {
       if (rm > 0.0) atomic_max_nonneg(&st.nat_r, rm);
       if (xm > 0.0) atomic_max_nonneg(&st.nat_x, xm);
     } else {
@@ -1553,7 +1623,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     V4<R> x;
     quad_finish_contact(sp, L.JL, L.JA, L.JB, xs, i, L.B.sl[lc.x], L.B.sa[lc.x], L.B.sl[lc.y], L.B.sa[lc.y], x,
                         rloc, xmax);
-    d.c_imp[d.perm[gs.p0 + i]] = x;
+    if (!d.post) d.c_imp[d.perm[gs.p0 + i]] = x;
   }
   finish_group<R, COOP>(d, st, ctl, gr.lead, L.B, b0, nb, rloc, xmax, sh_red);
 }
@@ -1654,7 +1724,7 @@ __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group&
       V4<R> x;
       quad_finish_contact(sp, d.s_JL, d.s_JA, d.s_JB, xs, i, L.B.sl[sa], L.B.sa[sa], L.B.sl[sb], L.B.sa[sb], x,
                           rloc, xmax);
-      d.c_imp[d.perm[i]] = x;
+      if (!d.post) d.c_imp[d.perm[i]] = x;
     }
   }
   finish_group<R, COOP>(d, st, ctl, gr.lead, L.B, b0, nb, rloc, xmax, sh_red);
@@ -1684,8 +1754,10 @@ template <typename R, int NT> __global__ void __launch_bounds__(NT) k_solve(Dev<
   const int slot = d.bs_slot[bs];
   const SceneParams& sp = d.sp[slot];
   SceneState& st = d.ss[slot];
+  if (d.post && !(sp.use_post_stab && st.post_active)) return;
   double* res = d.residuals + sp.res_off;
-  for (int i = threadIdx.x; i < sp.max_iter; i += NT) res[i] = INFINITY;
+  if (!d.post)
+    for (int i = threadIdx.x; i < sp.max_iter; i += NT) res[i] = INFINITY;
   if (st.k_cnt == 0) {
     if (threadIdx.x == 0) no_contacts<R>(st);
     return;
@@ -1705,8 +1777,9 @@ template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve_mul
   const int slot = d.bs_slot[0];
   const SceneParams& sp = d.sp[slot];
   SceneState& st = d.ss[slot];
+  if (d.post && !(sp.use_post_stab && st.post_active)) return;   // uniform over the grid
   double* res = d.residuals + sp.res_off;
-  if (blockIdx.x == 0)
+  if (blockIdx.x == 0 && !d.post)
     for (int i = threadIdx.x; i < sp.max_iter; i += SOLVE_BLOCK) res[i] = INFINITY;
   if (st.k_cnt == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) no_contacts<R>(st);

@@ -34,7 +34,7 @@ class Params(C.Structure):
         "max_gap_value", "ellipsoid_tolerance", "ellipsoid_expansion")] + \
         [(n, C.c_int) for n in ("max_iterations", "contact_optimization_max_iterations",
                                 "ellipsoid_max_iterations", "use_bounce", "use_pre_stabilization",
-                                "proximal_solver", "friction_model", "pad")] + \
+                                "proximal_solver", "friction_model", "use_post_stabilization")] + \
         [("coloring_seed", C.c_ulonglong)]
 
 
@@ -50,7 +50,8 @@ class StepLog(C.Structure):
     _fields_ = [("step_index", C.c_int), ("contact_points", C.c_int), ("iterations", C.c_int),
                 ("exitcode", C.c_int), ("rejects", C.c_int), ("sweeps", C.c_int), ("colors", C.c_int),
                 ("solved", C.c_int), ("number_of_overlaps", C.c_ulonglong), ("fw_cell_evals", C.c_ulonglong),
-                ("natural_residual", C.c_double), ("max_penetration", C.c_double), ("final_merit", C.c_double)]
+                ("natural_residual", C.c_double), ("max_penetration", C.c_double), ("final_merit", C.c_double),
+                ("post_iterations", C.c_int), ("post_exitcode", C.c_int)]
 
 
 class Tuning(C.Structure):

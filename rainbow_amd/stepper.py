@@ -42,6 +42,7 @@ def params_struct(p):
     s.ellipsoid_max_iterations = int(p.ellipsoid_max_iterations)
     s.use_bounce = int(bool(p.use_bounce))
     s.use_pre_stabilization = int(bool(p.use_pre_stabilization))
+    s.use_post_stabilization = int(bool(getattr(p, "use_post_stabilization", False)))
     scheme = getattr(p, "proximal_solver", "parallel_gauss_seidel")
     if scheme not in native.SCHEMES:
         raise NotImplementedError(
@@ -183,8 +184,6 @@ class HipStepper:
     def _prepare(self, engine):
         if engine.hinges:
             raise NotImplementedError("hinge joints are outside the GPU step's scope")
-        if engine.params.use_post_stabilization:
-            raise NotImplementedError("post-stabilisation is outside the GPU step's scope")
         prec = getattr(engine.params, "precision", "fp32")
         ctx = shared_context(getattr(engine.params, "device", 0), prec)
         key = (id(engine), getattr(engine, "_version", 0), params_key(engine.params), prec)
@@ -224,6 +223,10 @@ class HipStepper:
         if lg["contact_points"] > 0:
             d.update(residuals=lg["residuals"].copy(), iterations=int(lg["iterations"]),
                      exitcode=int(lg["exitcode"]), solver_time=t["coloring"] + t["solver_time"])
+        if lg["post_exitcode"] >= 0:
+            # steppers.py:33-34: the post-stabilisation solve's keys carry its prefix
+            d.update(post_stabilization_iterations=int(lg["post_iterations"]),
+                     post_stabilization_exitcode=int(lg["post_exitcode"]))
         d.update(stepper_time=t["stepper_time"], dt=dt, contact_points=int(lg["contact_points"]),
                  kinetic_energy=kin, potential_energy=pot, max_penetration=float(lg["max_penetration"]),
                  natural_residual=float(lg["natural_residual"]), rejects=int(lg["rejects"]),

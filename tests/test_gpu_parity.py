@@ -167,6 +167,28 @@ def test_big_island_fp64(bank):
     assert exact >= 1
 
 
+@pytest.mark.parametrize("scheme", ["parallel_gauss_seidel", "gauss_seidel"])
+def test_post_stabilization_fp64(bank, scheme):
+    """steppers.py:90-91: with use_post_stabilization the step ends with the
+    post-stabilisation solve and position update, exactly as the oracle's."""
+    eng = mixed_scene(bank, 8, 2, sink=0.3, params=dict(precision="fp64", max_iterations=40, proximal_solver=scheme,
+                                                       use_post_stabilization=True))
+    sc = OS.from_engine(eng)
+    exact = ran = 0
+    for _ in range(3):
+        m = {}
+        out = OST.step(sc, 0.01, m)
+        c, state, lg = gpu_step(eng, 0.01)
+        ok = assert_fp64_parity(c, state, lg, out, sc, m)
+        if ok and "post" in out:
+            ran += 1
+            assert lg["post_iterations"] == out["log"]["post_stabilization_iterations"]
+            assert lg["post_exitcode"] == out["log"]["post_stabilization_exitcode"]
+        exact += ok
+        resync(eng, sc)
+    assert exact >= 2 and ran >= 1
+
+
 @pytest.mark.parametrize("scheme,friction", [("gauss_seidel", "isotropic"), ("parallel_jacobi", "isotropic"),
                                              ("parallel_jacobi_hybrid", "isotropic"),
                                              ("parallel_gauss_seidel", "anisotropic")])

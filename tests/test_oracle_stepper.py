@@ -66,3 +66,53 @@ def test_simulate_substeps(oracle_bank32):
     assert [l["dt"] for l in logs] == pytest.approx([0.01, 0.01, 0.005])
     with pytest.raises(ValueError):
         OST.simulate(sc, 0.0)
+
+
+def sunk_sphere(bank, depth, params=None):
+    eng = scenes.new_engine(bank, params, backend=None)
+    API.create_gravity_force(eng, "gravity", 9.81, np.array([0.0, 1.0, 0.0]))
+    scenes.add_ground(eng, bank)
+    sph = bank.get("sphere_shape", lambda: MESH.create_sphere(0.5, 16, 16))
+    scenes.add_body(eng, "s", sph, np.array([0.0, 0.5 - depth, 0.0]), q=np.array([1.0, 0.0, 0.0, 0.0]))
+    return OS.from_engine(eng)
+
+
+def test_post_stabilization_lifts_by_the_deepest_penetration(oracle_bank32):
+    """A sphere sunk into the ground: its deepest contact is the bottom pole,
+    whose arm passes through the centre of mass along the ground's normal, and
+    every other contact is shallower.  The post-stabilisation problem's
+    solution is therefore a pure lift by exactly that penetration, no turn."""
+    import oracle.collision as OC
+    import oracle.problem as OP
+    sc = sunk_sphere(oracle_bank32, 0.02, params=dict(max_iterations=200, absolute_tolerance=0.0,
+                                                      relative_tolerance=0.0))
+    contacts = OC.detect(sc)
+    assert contacts.K > 1
+    Winv, f_ext, du_ext = OP.body_mass_blocks(sc)
+    P = OP.assemble(sc, contacts, Winv, du_ext, 0.01)
+    groups, _ = OST.sweep_groups(sc, contacts)
+    r0, q0 = sc.r.copy(), sc.q.copy()
+    deepest = -float(contacts.gap.min())
+    assert deepest == pytest.approx(0.02, abs=1e-9)
+    res = OST.post_stabilize(sc, P, groups, sc.params)
+    assert res is not None
+    assert sc.r[1] - r0[1] == pytest.approx([0.0, deepest, 0.0], abs=1e-9)
+    assert np.allclose(sc.q[1], q0[1], atol=1e-9)
+    assert np.array_equal(sc.r[0], r0[0])          # the fixed ground does not move
+    # friction rows stay zero, normal impulses non-negative
+    assert np.all(res["problem"].sol[:, 1:] == 0.0) and np.all(res["problem"].sol[:, 0] >= 0.0)
+
+
+def test_post_stabilization_skips_shallow_contacts(oracle_bank32):
+    """No contact deeper than min_gap_value: nothing is solved or moved."""
+    import oracle.collision as OC
+    import oracle.problem as OP
+    sc = sunk_sphere(oracle_bank32, 0.0005)
+    contacts = OC.detect(sc)
+    assert contacts.K > 0 and contacts.gap.min() > -sc.params["min_gap_value"]
+    Winv, f_ext, du_ext = OP.body_mass_blocks(sc)
+    P = OP.assemble(sc, contacts, Winv, du_ext, 0.01)
+    groups, _ = OST.sweep_groups(sc, contacts)
+    r0 = sc.r.copy()
+    assert OST.post_stabilize(sc, P, groups, sc.params) is None
+    assert np.array_equal(sc.r, r0)

@@ -160,6 +160,16 @@ int rbh_shape_upload(rbh_ctx* ctx, const double* V, int nv, const int* T, int nt
 int rbh_scene_create(rbh_ctx* ctx, const rbh_scene_desc* desc, int* scene_id);
 int rbh_scene_destroy(rbh_ctx* ctx, int scene_id);
 int rbh_scene_set_params(rbh_ctx* ctx, int scene_id, const rbh_params* params);
+/* The scene's hinge joints, replacing any set before (n = 0 removes them):
+ * Engine.hinges in creation order with the sockets api.set_hinge computes
+ * (api.py:86-156, types.py:329-353) -- parent and child as scene body
+ * indices, arm_p / arm_c (3n) the joint origin and axis_p / axis_c (3n) the
+ * joint axis in the parent's and the child's body frames.  Each step then
+ * solves the hinge problem of constraints/hinges.py:16-218 together with the
+ * contacts (gauss_seidel_solver.py:14-101).  At most 1024 bodies of a scene
+ * may be joined by hinges; a hinge needs two distinct bodies. */
+int rbh_scene_set_hinges(rbh_ctx* ctx, int scene_id, int n, const int* parent, const int* child, const double* arm_p,
+                         const double* arm_c, const double* axis_p, const double* axis_c);
 /* Body state: r (3n), q (4n, s x y z), v (3n), w (3n); replaces
  * StateStorage.copy_from_engine / copy_to_engine (state_storage.py:43-68). */
 int rbh_scene_set_state(rbh_ctx* ctx, int scene_id, const double* r, const double* q, const double* v,

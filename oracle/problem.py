@@ -22,6 +22,8 @@
                          accept -> sol = x and r *= nu_increase when the merit
                          dropped by less than too_small_merit_change; absolute and
                          relative stopping tests; exit codes 0/1/2.
+* ``solve_multi``        the same driver over several problems (contacts, then hinges),
+                         stopping on the largest of their merits.
 * ``natural_residual``   ||x - prox(x - r (A x + b))||_inf / max(1, ||x||_inf).
 
 Pinned by tests/test_oracle_problem.py: prox operators by closed forms (inside
@@ -31,7 +33,9 @@ of sum a_i^2 z_i^2 / (a_i^2 + t)^2 = 1; the sweep by the dense-matrix form
 z = x - r (J W J^T x + b) on random problems and by the single-contact closed
 form x_n = max(0, -b_n / A_nn); the block Jacobian by the reference's scalar
 assembly loop (contacts.py:130-185) restated literally in the test; the solve
-driver by hand-computed accept/reject sequences.
+driver by hand-computed accept/reject sequences; solve_multi by the same
+sequences (one problem: identical to ``solve``; two scripted problems: the
+residual is the larger contribution, a rejected problem's last merit).
 """
 import numpy as np
 
@@ -271,6 +275,70 @@ def solve(P, groups, prm, scheme=None):
                 P.r = P.r * nu_increase
             last_merit = merit
             global_merit = max(0.0, merit)
+        residuals[it] = global_merit
+        if global_merit < prm["absolute_tolerance"]:
+            return dict(residuals=residuals, iterations=it, exitcode=1, rejects=rejects)
+        if np.abs(global_merit - global_last_merit) < prm["relative_tolerance"] * global_last_merit:
+            return dict(residuals=residuals, iterations=it, exitcode=2, rejects=rejects)
+        global_last_merit = global_merit
+    return dict(residuals=residuals, iterations=max_it, exitcode=0, rejects=rejects)
+
+
+class ContactSolvable:
+    """The contact problem as seen by the multi-problem solve driver."""
+
+    def __init__(self, P, groups, prm, scheme=None):
+        self.__dict__.update(P=P, groups=groups, prm=prm, scheme=scheme or prm["proximal_solver"])
+
+    def sweep(self):
+        P, scheme = self.__dict__["P"], self.__dict__["scheme"]
+        if scheme in ("gauss_seidel", "parallel_gauss_seidel"):
+            sweep_gs(P, self.__dict__["groups"], self.__dict__["prm"])
+        elif scheme == "parallel_jacobi":
+            sweep_jacobi(P, self.__dict__["prm"], hybrid=False)
+        elif scheme in ("parallel_jacobi_hybrid", "parallel_jacboi_hybrid"):
+            sweep_jacobi(P, self.__dict__["prm"], hybrid=True)
+        else:
+            raise NotImplementedError(scheme)
+
+    def __getattr__(self, name):
+        return getattr(self.__dict__["P"], name)
+
+    def __setattr__(self, name, value):
+        setattr(self.__dict__["P"], name, value)
+
+
+def solve_multi(problems, prm):
+    """gauss_seidel_solver.py:14-101 over several problems (steppers.py:68-83:
+    the contact problem first, then the hinges).  Each problem keeps its own
+    merit ||x - sol||_inf, accept/reject and r-factors; the residual log and
+    the stopping tests use the largest contribution, where a rejected problem
+    contributes its last accepted merit.  ``rejects`` counts the first
+    problem's rejected sweeps.
+    """
+    nu_reduce, nu_increase = prm["nu_reduce"], prm["nu_increase"]
+    too_small = prm["too_small_merit_change"]
+    max_it = int(prm["max_iterations"])
+    residuals = np.full(max_it, np.inf)
+    last = [np.inf for _ in problems]
+    global_last_merit = np.inf
+    rejects = 0
+    for it in range(max_it):
+        global_merit = 0.0
+        for i, p in enumerate(problems):
+            p.sweep()
+            merit = float(np.max(np.abs(p.x - p.sol))) if p.x.size else 0.0
+            if merit > last[i]:
+                p.r = p.r * nu_reduce
+                p.x = p.sol.copy()
+                global_merit = max(global_merit, last[i])
+                rejects += 1 if i == 0 else 0
+            else:
+                p.sol = p.x.copy()
+                if last[i] - merit < too_small:
+                    p.r = p.r * nu_increase
+                last[i] = merit
+                global_merit = max(global_merit, merit)
         residuals[it] = global_merit
         if global_merit < prm["absolute_tolerance"]:
             return dict(residuals=residuals, iterations=it, exitcode=1, rejects=rejects)

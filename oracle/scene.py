@@ -53,6 +53,7 @@ class Scene:
         self.eps = np.zeros((1, 1))
         self.params = dict(PARAM_DEFAULTS)
         self.step_index = 0       # number of steps taken (keys the colouring RNG)
+        self.hinges = None        # oracle.hinges.HingeSet (Engine.hinges) or None
 
     @property
     def N(self):
@@ -109,6 +110,8 @@ def from_engine(engine):
     for f in engine.forces.values():
         if f.force_type == "Gravity":
             sc.gravity.append((float(f.g), np.asarray(f.up, dtype=np.float64)))
+    from . import hinges
+    sc.hinges = hinges.from_engine(engine, {b.name: k for k, b in enumerate(bodies)})
     p = engine.params
     sc.params = {k: getattr(p, k, v) for k, v in PARAM_DEFAULTS.items()}
     sc.step_index = int(getattr(engine, "step_index", 0))

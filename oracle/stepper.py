@@ -10,12 +10,15 @@ With use_post_stabilization the step ends with ``post_stabilize``
 (steppers.py:15-36, 90-91; constraints/post_stabilization.py), pinned by
 tests/test_oracle_stepper.py: a sphere sunk into the ground is lifted by
 exactly its deepest penetration without turning, and contacts shallower than
-min_gap_value leave the scene untouched.  Hinges are outside this build's
-scope.
+min_gap_value leave the scene untouched.
+
+With hinges (Engine.hinges) the contact problem and the hinge problem
+(oracle/hinges.py) are solved by one driver, problem.solve_multi
+(steppers.py:68-83), and delta_u adds both problems' W J^T sol.
 """
 import numpy as np
 
-from . import coloring, collision, problem
+from . import coloring, collision, hinges, problem
 from . import quat as Q
 
 
@@ -112,7 +115,8 @@ def step(scene, dt, margins=None):
     out = dict(contacts=contacts, problem=None, colors=None, log={})
     log = out["log"]
     log["number_of_overlaps"] = stats["number_of_overlaps"]
-    if contacts.K > 0:
+    H = scene.hinges if scene.hinges is not None and scene.hinges.K > 0 else None
+    if contacts.K > 0 and H is None:
         P = problem.assemble(scene, contacts, Winv, du_ext, dt)
         groups, C = sweep_groups(scene, contacts)
         res = problem.solve(P, groups, prm)
@@ -121,6 +125,25 @@ def step(scene, dt, margins=None):
         log.update(residuals=res["residuals"], iterations=res["iterations"],
                    exitcode=res["exitcode"], rejects=res["rejects"])
         log["natural_residual"] = problem.natural_residual(P, prm)
+    elif H is not None:
+        # steppers.py:68-83: contacts (when there are any), then hinges, one solve
+        probs = []
+        if contacts.K > 0:
+            P = problem.assemble(scene, contacts, Winv, du_ext, dt)
+            groups, C = sweep_groups(scene, contacts)
+            probs.append(problem.ContactSolvable(P, groups, prm))
+            out.update(problem=P, colors=C, groups=groups)
+        HP = hinges.assemble(scene, H, Winv, du_ext, dt)
+        probs.append(hinges.Solvable(HP))
+        res = problem.solve_multi(probs, prm)
+        if contacts.K > 0:
+            delta_u = delta_u + problem.body_w(P, P.sol)
+        delta_u = delta_u + hinges.body_w(HP, HP.sol)
+        out["hinges"] = HP
+        log.update(residuals=res["residuals"], iterations=res["iterations"],
+                   exitcode=res["exitcode"], rejects=res["rejects"])
+        if contacts.K > 0:
+            log["natural_residual"] = problem.natural_residual(P, prm)
     integrate(scene, delta_u, dt)
     if contacts.K > 0 and prm.get("use_post_stabilization"):
         post = post_stabilize(scene, out["problem"], out["groups"], prm)

@@ -68,6 +68,52 @@ def create_hinge(engine, hinge_name: str) -> None:
     _touch(engine)
 
 
+def set_hinge(engine, hinge_name: str, parent_name: str, child_name: str, origin: np.ndarray, axis: np.ndarray,
+              mode: str = "world") -> None:
+    """api.py:86-156: the joint frame (origin, axis as its local z) in world,
+    parent-body or parent-model coordinates, stored as sockets in both bodies'
+    frames (types.py:329-353: arm = socket origin, axis = socket z)."""
+    if hinge_name in engine.hinges:
+        hinge = engine.hinges[hinge_name]
+    else:
+        raise RuntimeError("set_hinge() no such rigid body exist with that name")
+    if parent_name in engine.bodies:
+        parent = engine.bodies[parent_name]
+    else:
+        raise RuntimeError("set_hinge() no such rigid body exist with that name")
+    if child_name in engine.bodies:
+        child = engine.bodies[child_name]
+    else:
+        raise RuntimeError("set_hinge() no such rigid body exist with that name")
+    origin = np.asarray(origin, dtype=np.float64)
+    axis = np.asarray(axis, dtype=np.float64)
+    if mode == "world":
+        o_world, s_world = origin.copy(), axis.copy()
+    elif mode == "body":
+        o_world = Q.rotate(parent.q, origin) + parent.r
+        s_world = Q.rotate(parent.q, axis)
+    elif mode == "model":
+        o_body = Q.rotate(Q.conjugate(parent.shape.q), origin - parent.shape.r)
+        s_body = Q.rotate(Q.conjugate(parent.shape.q), axis)
+        o_world = Q.rotate(parent.q, o_body) + parent.r
+        s_world = Q.rotate(parent.q, s_body)
+    else:
+        raise ValueError(f"set_hinge() no such mode {mode} is supported")
+    t, b, n = Q.make_orthonormal_vectors(s_world)
+    q_world = Q.from_matrix(np.column_stack([t, b, n]))
+    z = np.array([0.0, 0.0, 1.0])
+    q_ps = Q.prod(Q.conjugate(parent.q), q_world)
+    q_cs = Q.prod(Q.conjugate(child.q), q_world)
+    hinge.parent, hinge.child = parent, child
+    hinge.socket_p = (Q.rotate(Q.conjugate(parent.q), o_world - parent.r), q_ps)
+    hinge.socket_c = (Q.rotate(Q.conjugate(child.q), o_world - child.r), q_cs)
+    hinge.arm_p = hinge.socket_p[0].copy()
+    hinge.arm_c = hinge.socket_c[0].copy()
+    hinge.axis_p = Q.rotate(q_ps, z)
+    hinge.axis_c = Q.rotate(q_cs, z)
+    _touch(engine)
+
+
 def create_mesh(V: np.ndarray, T: np.ndarray) -> MESH.Mesh:
     """api.py:159-173: spatial re-ordering along the direction of most variance."""
     axis = MESH.direction_of_most_variance(V)

@@ -76,6 +76,8 @@ struct SceneHost {
   bool live = false;
   int body_off = 0, nbody = 0, tri_off = 0, ntri = 0, mat_off = 0, nmat = 0, res_off = 0, res_n = 0;
   rbh_params params{};
+  std::vector<int> h_parent, h_child;   // hinges: scene-local body indices
+  std::vector<double> h_geo;            // 12 per hinge: arm_p, arm_c, axis_p, axis_c (body frames)
   long long ws_gen = -1;   // workspace generation of the scene's last step
   int batch_index = -1;    // index inside that batch
 };
@@ -100,8 +102,13 @@ struct rbh_ctx {
   // body arena
   RangeAlloc bodies, tris, res;
   DBuf r, q, v, w, b_shape, b_slot, b_fixed, b_mat, b_tri, b_mass, b_inertia, b_grav, b_damp;
-  DBuf W0, W1, du0, du1, lo, hi, ws0, ws1, ww0, ww1;
+  DBuf W0, W1, du0, du1, lo, hi, ws0, ws1, ww0, ww1, hd0, hd1;
   DBuf tlo, thi, clo, chi;
+  // hinges of all live scenes, packed by scene slot (rebuilt when they change)
+  DBuf h_ids, h_geo, hb_body, hb_inc_off, hb_inc, h_J, h_WJ, h_b, h_r0, h_x, h_sol, d_hmer;
+  bool hinges_dirty = false;
+  bool any_hinge = false;   // a scene of the batch has hinges (set per run_steps call)
+  int hinge_body_max = 0;   // most hinge bodies of a batch scene (the hinge solve's LDS)
   // scenes
   std::vector<SceneHost> scenes;
   std::vector<SceneParams> sp_host;
@@ -201,7 +208,8 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part, &c->u_T, &c->isl_unit,
           &c->u_nbody, &c->u_boff, &c->f_loc, &c->f_unit, &c->u_blist, &c->s_loc, &c->s_W, &c->l_cnt,
           &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs, &c->s_JL, &c->s_JA, &c->s_JB,
-          &c->xr0, &c->xr1};
+          &c->xr0, &c->xr1, &c->hd0, &c->hd1, &c->h_ids, &c->h_geo, &c->hb_body, &c->hb_inc_off, &c->hb_inc,
+          &c->h_J, &c->h_WJ, &c->h_b, &c->h_r0, &c->h_x, &c->h_sol, &c->d_hmer};
 }
 
 size_t total_bytes(rbh_ctx* c) {
@@ -233,7 +241,7 @@ int body_elem_sizes(rbh_ctx* c, std::vector<std::pair<DBuf*, size_t>>& out) {
          {&c->b_slot, 4}, {&c->b_fixed, 4}, {&c->b_mat, 4},     {&c->b_tri, 4},    {&c->b_mass, R},
          {&c->b_inertia, V}, {&c->b_grav, V}, {&c->b_damp, V},  {&c->W0, V},       {&c->W1, V},
          {&c->du0, V},    {&c->du1, V},     {&c->lo, V},        {&c->hi, V},       {&c->ws0, V},
-         {&c->ws1, V},    {&c->ww0, V},     {&c->ww1, V}};
+         {&c->ws1, V},    {&c->ww0, V},     {&c->ww1, V},      {&c->hd0, V},      {&c->hd1, V}};
   return 0;
 }
 
@@ -268,6 +276,7 @@ int grow_tris(rbh_ctx* c, int need) {
 int grow_res(rbh_ctx* c, int need) {
   int cap = std::max(need, std::max(4096, c->res.cap * 2));
   int rc = ensure(c, c->d_res, (size_t)cap * 8, true);
+  if (!rc) rc = ensure(c, c->d_hmer, (size_t)cap * 8, true);
   if (rc) return rc;
   c->res.grow(cap);
   c->arena_gen++;
@@ -324,6 +333,71 @@ int sync_scenes(rbh_ctx* c) {
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->scenes_dirty = false;
+  return 0;
+}
+
+// Pack the hinges of every live scene (scene order) into the device arrays:
+// per hinge the arena bodies and hinge-body slots of parent and child, the
+// body-frame arms and axes; per scene the list of bodies its hinges join
+// (ascending body index) and, per such body, its hinges in hinge order.
+template <typename R> int sync_hinges(rbh_ctx* c) {
+  if (!c->hinges_dirty) return 0;
+  std::vector<int4> ids;
+  std::vector<V4<R>> geo;
+  std::vector<int> hbody, inc_off, inc;
+  for (size_t slot = 0; slot < c->scenes.size(); ++slot) {
+    SceneHost& s = c->scenes[slot];
+    SceneParams& sp = c->sp_host[slot];
+    sp.h_off = (int)ids.size();
+    sp.hb_off = (int)hbody.size();
+    sp.h_cnt = 0;
+    sp.hb_cnt = 0;
+    if (!s.live || s.h_parent.empty()) continue;
+    const int K = (int)s.h_parent.size();
+    std::vector<int> loc(s.nbody, -1);
+    for (int k = 0; k < K; ++k) loc[s.h_parent[k]] = loc[s.h_child[k]] = 0;
+    std::vector<int> list;
+    for (int b = 0; b < s.nbody; ++b)
+      if (loc[b] == 0) { loc[b] = (int)list.size(); list.push_back(b); }
+    std::vector<std::vector<int>> per(list.size());
+    for (int k = 0; k < K; ++k) {
+      const int h = sp.h_off + k;
+      const int p = s.h_parent[k], ch = s.h_child[k];
+      ids.push_back(int4{s.body_off + p, s.body_off + ch, loc[p], loc[ch]});
+      for (int j = 0; j < 4; ++j)
+        geo.push_back(V4<R>{R(s.h_geo[12 * k + 3 * j]), R(s.h_geo[12 * k + 3 * j + 1]), R(s.h_geo[12 * k + 3 * j + 2]),
+                            R(0)});
+      per[loc[p]].push_back(2 * h);
+      per[loc[ch]].push_back(2 * h + 1);
+    }
+    for (size_t j = 0; j < list.size(); ++j) {
+      hbody.push_back(s.body_off + list[j]);
+      inc_off.push_back((int)inc.size());
+      inc.insert(inc.end(), per[j].begin(), per[j].end());
+    }
+    sp.h_cnt = K;
+    sp.hb_cnt = (int)list.size();
+  }
+  inc_off.push_back((int)inc.size());
+  const size_t nh = std::max<size_t>(ids.size(), 1), V = 4 * (size_t)c->prec;
+  int rc = upload(c, c->h_ids, ids);
+  if (!rc) rc = upload(c, c->h_geo, geo);
+  if (!rc) rc = upload(c, c->hb_body, hbody);
+  if (!rc) rc = upload(c, c->hb_inc_off, inc_off);
+  if (!rc) rc = upload(c, c->hb_inc, inc);
+  if (!rc) rc = ensure(c, c->h_J, 20 * nh * V);
+  if (!rc) rc = ensure(c, c->h_WJ, 20 * nh * V);
+  if (!rc) rc = ensure(c, c->h_b, 5 * nh * c->prec);
+  if (!rc) rc = ensure(c, c->h_r0, 5 * nh * c->prec);
+  if (!rc) rc = ensure(c, c->h_x, 5 * nh * c->prec);
+  if (!rc) rc = ensure(c, c->h_sol, 5 * nh * c->prec);
+  if (!rc) rc = ensure(c, c->d_hmer, (size_t)std::max(c->res.cap, 1) * 8, true);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->hinges_dirty = false;
+  c->scenes_dirty = true;
+  c->arena_gen++;
+  drop_graph(c);
   return 0;
 }
 
@@ -400,6 +474,11 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.perm = P(int*, perm); d.s_id = P(int2*, s_id); d.s_n = P(V4<R>*, s_n); d.s_rA = P(V4<R>*, s_rA);
   d.s_rB = P(V4<R>*, s_rB); d.s_b = P(V4<R>*, s_b); d.s_r0 = P(V4<R>*, s_r0); d.x0 = P(V4<R>*, x0);
   d.x1 = P(V4<R>*, x1);
+  d.h_ids = P(const int4*, h_ids); d.h_geo = P(const V4<R>*, h_geo); d.hb_body = P(const int*, hb_body);
+  d.hb_inc_off = P(const int*, hb_inc_off); d.hb_inc = P(const int*, hb_inc);
+  d.h_J = P(V4<R>*, h_J); d.h_WJ = P(V4<R>*, h_WJ); d.h_b = P(R*, h_b); d.h_r0 = P(R*, h_r0);
+  d.h_x = P(R*, h_x); d.h_sol = P(R*, h_sol); d.h_mer = P(double*, d_hmer);
+  d.hd0 = P(V4<R>*, hd0); d.hd1 = P(V4<R>*, hd1);
   d.err = P(int*, errbuf);
   d.dt = dt;
   d.flags = flags;
@@ -544,6 +623,15 @@ hipEvent_t* timing_row(rbh_ctx* c, int i, int todo) {
 }
 
 // ev: ST_N events to record the stage boundaries into, or nullptr.
+// The hinge problem around the contact solve (rbh_hinge.h): rows and the
+// recording pass before it, the final pass after it.
+template <typename R> void hinge_pass(rbh_ctx* c, const Dev<R>& d, int pass) {
+  if (!c->any_hinge) return;
+  if (pass == 0) hipLaunchKernelGGL(k_hinge_rows<R>, dim3(c->ns), dim3(64), 0, c->stream, d);
+  const size_t lds = std::max<size_t>(2 * (size_t)c->hinge_body_max * sizeof(V4<R>), 16);
+  hipLaunchKernelGGL(k_hinge_solve<R>, dim3(c->ns), dim3(64), lds, c->stream, d, pass);
+}
+
 template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, hipEvent_t* ev, bool multi, int multi_grid) {
   Dev<R> d = make_dev<R>(c, dt, flags, multi);
   hipStream_t s = c->stream;
@@ -601,17 +689,20 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(d.merit), 2 * c->max_iter_batch, 0);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(d.bar), 16, 0);
     mark(ST_COLOR);
+    hinge_pass<R>(c, d, 0);
     void* args[] = {&d};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<void*>(k_solve_multi<R>), dim3(multi_grid),
                                               dim3(SOLVE_BLOCK), args, c->solve_lds, s);
     if (e != hipSuccess) return e;
   } else {
     mark(ST_COLOR);
+    hinge_pass<R>(c, d, 0);
     if (batch_waves(c) == 8)
       hipLaunchKernelGGL((k_solve<R, 512>), dim3(ns), dim3(512), c->solve_lds, s, d);
     else
       hipLaunchKernelGGL((k_solve<R, SOLVE_BLOCK>), dim3(ns), dim3(SOLVE_BLOCK), c->solve_lds, s, d);
   }
+  hinge_pass<R>(c, d, 1);
   mark(ST_SOLVE);
   hipLaunchKernelGGL(k_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
   if (c->any_post) {
@@ -670,6 +761,12 @@ template <typename R> int setup_solve_lds(rbh_ctx* c) {
       (void)hipGetLastError();
       return fail(c, RBH_E_HIP, "solve kernels cannot use " + std::to_string(bytes) + " bytes of LDS");
     }
+  const int hbytes = (int)(2 * HINGE_BODY_MAX * sizeof(V4<R>));
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_hinge_solve<R>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          hbytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, RBH_E_HIP, "hinge solve cannot use " + std::to_string(hbytes) + " bytes of LDS");
+  }
   c->solve_lds = bytes;
   c->coop_max = 0;
   return 0;
@@ -677,7 +774,13 @@ template <typename R> int setup_solve_lds(rbh_ctx* c) {
 
 template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flags) {
   c->any_post = false;
-  for (int slot : c->batch) c->any_post = c->any_post || c->scenes[slot].params.use_post_stabilization != 0;
+  c->any_hinge = false;
+  c->hinge_body_max = 0;
+  for (int slot : c->batch) {
+    c->any_post = c->any_post || c->scenes[slot].params.use_post_stabilization != 0;
+    c->any_hinge = c->any_hinge || c->sp_host[slot].h_cnt > 0;
+    c->hinge_body_max = std::max(c->hinge_body_max, c->sp_host[slot].hb_cnt);
+  }
   const bool timing = (flags & RBH_STEP_TIMING) != 0;
   const bool use_graph = (flags & RBH_STEP_GRAPH) != 0;
   const int kflags = flags & RBH_STEP_COUNT_OVERLAPS;
@@ -1152,6 +1255,7 @@ int rbh_scene_create(rbh_ctx* c, const rbh_scene_desc* d, int* scene_id) {
   }
   c->sp_host[slot] = to_device_params(s.params, s.nmat, s.mat_off, s.res_off);
   c->scenes_dirty = true;
+  c->hinges_dirty = true;
   if ((rc = sync_scenes(c))) return rc;
   SceneState z{};
   HIPCHK(c, hipMemcpyAsync(reinterpret_cast<SceneState*>(c->d_ss.p) + slot, &z, sizeof(z), hipMemcpyHostToDevice,
@@ -1171,10 +1275,44 @@ int rbh_scene_destroy(rbh_ctx* c, int id) {
   c->tris.release(s.tri_off, s.ntri);
   c->res.release(s.res_off, s.res_n);
   s.live = false;
+  s.h_parent.clear(); s.h_child.clear(); s.h_geo.clear();
+  c->hinges_dirty = true;
   c->arena_gen++;
   drop_graph(c);
   c->batch.clear();
   return 0;
+}
+
+int rbh_scene_set_hinges(rbh_ctx* c, int id, int n, const int* parent, const int* child, const double* arm_p,
+                         const double* arm_c, const double* axis_p, const double* axis_c) {
+  if (!c || !valid_scene(c, id)) return fail(c, RBH_E_ARG, "unknown scene");
+  if (n < 0 || (n > 0 && (!parent || !child || !arm_p || !arm_c || !axis_p || !axis_c)))
+    return fail(c, RBH_E_ARG, "bad hinge arguments");
+  SceneHost& s = c->scenes[id];
+  std::vector<char> joined(s.nbody, 0);
+  int nj = 0;
+  for (int k = 0; k < n; ++k) {
+    if (parent[k] < 0 || parent[k] >= s.nbody || child[k] < 0 || child[k] >= s.nbody)
+      return fail(c, RBH_E_ARG, "hinge " + std::to_string(k) + ": body index out of range");
+    if (parent[k] == child[k]) return fail(c, RBH_E_ARG, "hinge " + std::to_string(k) + ": parent and child are one body");
+    for (int b : {parent[k], child[k]}) if (!joined[b]) { joined[b] = 1; ++nj; }
+  }
+  if (nj > HINGE_BODY_MAX)
+    return fail(c, RBH_E_ARG, "hinges join more than " + std::to_string(HINGE_BODY_MAX) + " bodies of one scene");
+  s.h_parent.assign(parent, parent + n);
+  s.h_child.assign(child, child + n);
+  s.h_geo.assign(12 * (size_t)n, 0.0);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < 3; ++j) {
+      s.h_geo[12 * k + j] = arm_p[3 * k + j];
+      s.h_geo[12 * k + 3 + j] = arm_c[3 * k + j];
+      s.h_geo[12 * k + 6 + j] = axis_p[3 * k + j];
+      s.h_geo[12 * k + 9 + j] = axis_c[3 * k + j];
+    }
+  c->hinges_dirty = true;
+  c->batch.clear();
+  (void)hipSetDevice(c->device);
+  return c->prec == 4 ? sync_hinges<float>(c) : sync_hinges<double>(c);
 }
 
 int rbh_scene_set_params(rbh_ctx* c, int id, const rbh_params* p) {
@@ -1194,8 +1332,10 @@ int rbh_scene_set_params(rbh_ctx* c, int id, const rbh_params* p) {
   s.params = *p;
   c->sp_host[id] = to_device_params(s.params, s.nmat, s.mat_off, s.res_off);
   c->scenes_dirty = true;
+  c->hinges_dirty = true;   // re-applies the scene's hinge offsets to its parameters
   drop_graph(c);
-  return sync_scenes(c);
+  int rc2 = c->prec == 4 ? sync_hinges<float>(c) : sync_hinges<double>(c);
+  return rc2 ? rc2 : sync_scenes(c);
 }
 
 int rbh_scene_set_state(rbh_ctx* c, int id, const double* r, const double* q, const double* v, const double* w) {
@@ -1230,6 +1370,7 @@ int rbh_step(rbh_ctx* c, const int* ids, int n, int n_steps, double dt, int flag
   if (!(dt > 0.0) && dt != 0.0 && !(dt < 0.0)) return fail(c, RBH_E_ARG, "dt is NaN");
   (void)hipSetDevice(c->device);
   int rc = c->prec == 4 ? sync_shapes<float>(c) : sync_shapes<double>(c);
+  if (!rc) rc = c->prec == 4 ? sync_hinges<float>(c) : sync_hinges<double>(c);
   if (!rc) rc = sync_scenes(c);
   if (!rc) rc = build_batch(c, ids, n);
   if (rc) return rc;

@@ -26,6 +26,8 @@ struct SceneParams {
   double fw_tol, gap_reduction, min_gap, max_gap, ell_tol, ell_expansion;
   int max_iter, fw_max_iter, ell_max_iter, use_bounce, use_pre_stab, scheme, friction;
   int nmat, mat_off, res_off, use_post_stab;
+  int h_off, h_cnt;     // hinges of the scene in the hinge arrays
+  int hb_off, hb_cnt;   // bodies joined by those hinges (hinge-body list)
   unsigned long long seed;
 };
 
@@ -119,6 +121,15 @@ template <typename R> struct Dev {
   V4<R>* s_W;                          // 4 per position: W0, W1 of body A, then of body B (zero if fixed)
   V4<R>* s_JL; V4<R>* s_JA; V4<R>* s_JB;   // rows 4 pos + r of J: (lin_r, b_r), (angA_r, r0_r), (angB_r, mu_r)
   R* xr0; R* xr1;                      // iterates by row (4 pos + r), for the row-major sweeps
+  // hinges (constraints/hinges.py): static per hinge, then per-step rows and iterates
+  const int4* h_ids;                   // (parent, child) arena bodies, (parent, child) hinge-body slots
+  const V4<R>* h_geo;                  // 4 per hinge: arm_p, arm_c, axis_p, axis_c (body frames)
+  const int* hb_body;                  // hinge-body slot -> arena body
+  const int* hb_inc_off; const int* hb_inc;   // hinge-body slot -> 2 h + side (0 parent, 1 child), hinge order
+  V4<R>* h_J; V4<R>* h_WJ;             // 20 per hinge: row i at 4 i: parent lin, parent ang, child lin, child ang
+  R* h_b; R* h_r0; R* h_x; R* h_sol;   // 5 per hinge
+  double* h_mer;                       // per sweep: the hinge problem's merit contribution (at res_off)
+  V4<R>* hd0; V4<R>* hd1;              // per arena body: hinge W J^T sol (linear, angular)
   int* err;                           // [0] flags, [1..] needed capacities
   int post;                           // 1: the solve kernels run the post-stabilisation problem
   double dt;                           // > 0: use for every scene, else the scene's time_step
@@ -166,7 +177,7 @@ template <typename R> __global__ void k_body_prep(Dev<R> d) {
   if (f >= d.nb) return;
   int b = d.bb_body[f];
   V4<R> z = {R(0), R(0), R(0), R(0)};
-  d.ws0[b] = z; d.ws1[b] = z; d.ww0[b] = z; d.ww1[b] = z;
+  d.ws0[b] = z; d.ws1[b] = z; d.ww0[b] = z; d.ww1[b] = z; d.hd0[b] = z; d.hd1[b] = z;
   if (d.b_fixed[b]) {
     d.b_W0[b] = z; d.b_W1[b] = z; d.b_du0[b] = z; d.b_du1[b] = z;
     return;
@@ -820,20 +831,24 @@ template <typename R> __global__ void k_assemble(Dev<R> d) {
 }
 
 #include "rbh_solve.h"
+#include "rbh_hinge.h"
 
 // ========================================================== integration
-// state_storage.py:239-302: u += dt W f_ext + W J^T sol; r += v dt;
-// q <- unit(q + (0, w) q dt 0.5); fixed bodies untouched.
+// state_storage.py:239-302: u += dt W f_ext + W J^T sol (contacts, then
+// hinges, steppers.py:64, 82-85); r += v dt; q <- unit(q + (0, w) q dt 0.5);
+// fixed bodies untouched.
 template <typename R> __global__ void k_integrate(Dev<R> d) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= d.nb || aborted(d.err)) return;
   int b = d.bb_body[f];
   if (d.b_fixed[b]) return;
   const R dt = scene_dt(d, d.b_slot[b]);
-  V4<R> du0 = d.b_du0[b], du1 = d.b_du1[b], s0 = d.ws0[b], s1 = d.ws1[b];
+  V4<R> du0 = d.b_du0[b], du1 = d.b_du1[b], s0 = d.ws0[b], s1 = d.ws1[b], h0 = d.hd0[b], h1 = d.hd1[b];
   V4<R> v = d.v[b], w = d.w[b];
-  v.x = v.x + (dt * du0.x + s0.x); v.y = v.y + (dt * du0.y + s0.y); v.z = v.z + (dt * du0.z + s0.z);
-  w.x = w.x + (dt * du1.x + s1.x); w.y = w.y + (dt * du1.y + s1.y); w.z = w.z + (dt * du1.z + s1.z);
+  v.x = v.x + ((dt * du0.x + s0.x) + h0.x); v.y = v.y + ((dt * du0.y + s0.y) + h0.y);
+  v.z = v.z + ((dt * du0.z + s0.z) + h0.z);
+  w.x = w.x + ((dt * du1.x + s1.x) + h1.x); w.y = w.y + ((dt * du1.y + s1.y) + h1.y);
+  w.z = w.z + ((dt * du1.z + s1.z) + h1.z);
   V4<R> r = d.r[b];
   r.x = r.x + v.x * dt; r.y = r.y + v.y * dt; r.z = r.z + v.z * dt;
   Quat<R> q = quat(d.q[b]);

@@ -960,8 +960,9 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
 struct SolveCtl {
   double last_merit = INFINITY, glast = INFINITY, rscale = 1.0, gm = 0.0;
   int acc = 0, rejects = 0, exitcode = 0, iters = 0, sweeps = 0;
-  // returns bit0 accept, bit1 stop
-  __device__ int decide(const SceneParams& sp, double merit, double* res, int it) {
+  // accept (bit0) or reject, r-factor update; gm = this problem's contribution
+  // to the global merit (its merit, or its last accepted merit on a reject)
+  __device__ int judge(const SceneParams& sp, double merit) {
     int dec = 0;
     if (merit > last_merit) {
       rscale *= sp.nu_reduce;
@@ -973,6 +974,13 @@ struct SolveCtl {
       last_merit = merit;
       gm = fmax(0.0, merit);
     }
+    return dec;
+  }
+  // returns bit0 accept, bit1 stop; `other` is the contribution of the
+  // scene's other problem (the hinges) to the global merit of this sweep
+  __device__ int decide(const SceneParams& sp, double merit, double* res, int it, double other = 0.0) {
+    int dec = judge(sp, merit);
+    gm = fmax(gm, other);
     if (res) res[it] = gm;
     if (gm < sp.abs_tol) { dec |= 2; exitcode = 1; }
     else if (fabs(gm - glast) < sp.rel_tol * glast) { dec |= 2; exitcode = 2; }
@@ -982,6 +990,12 @@ struct SolveCtl {
     return dec;
   }
 };
+
+// the hinge problem's contribution to sweep `it` (recorded before the contact
+// solve; gauss_seidel_solver.py:47-73 takes the largest over the problems)
+template <typename R> __device__ __forceinline__ double hinge_merit(const Dev<R>& d, const SceneParams& sp, int it) {
+  return (sp.h_cnt > 0 && !d.post) ? d.h_mer[sp.res_off + it] : 0.0;
+}
 
 template <typename R> __device__ void write_solve_state(const Dev<R>& d, SceneState& st, const SolveCtl& c) {
   if (d.post) {   // the post-stabilisation solve keeps the contact solve's record
@@ -1199,7 +1213,8 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
       if (!grid_barrier(d.bar, gridDim.x, d.err, s_ok)) return;
       merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     }
-    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it);
+    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it,
+                               hinge_merit(d, sp, it));
     // accepted sweep: ws <- ww, else ww <- ws
     for (int i = threadIdx.x; i < nbody; i += blockDim.x) {
       const int s = LDS ? i : d.u_blist[b0 + i];
@@ -1506,7 +1521,8 @@ __device__ __forceinline__ int sweep_decide(const Dev<R>& d, const SceneParams& 
     merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   }
   double* res = d.residuals + sp.res_off;
-  const int dec = ctl.decide(sp, merit, (lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it);
+  const int dec = ctl.decide(sp, merit, (lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it,
+                             hinge_merit(d, sp, it));
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     if (dec & 1) { B.sl[i] = B.wl[i]; B.sa[i] = B.wa[i]; }
     else { B.wl[i] = B.sl[i]; B.wa[i] = B.sa[i]; }

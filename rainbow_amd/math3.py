@@ -115,3 +115,21 @@ def from_matrix(M):
 def unit3(a):
     a = np.asarray(a, dtype=np.float64)
     return a / np.linalg.norm(a)
+
+
+def make_orthonormal_vectors(n):
+    """vector3.py:75-113: (t, b, n) with t guessed from the coordinate axis of
+    the smallest |n_i| (a later axis wins a tie), b = unit(n x t), t = b x n."""
+    n = np.asarray(n, dtype=np.float64)
+    n = n / np.linalg.norm(n)
+    ax = np.fabs(n)
+    if ax[0] <= ax[1] and ax[0] <= ax[2]:
+        t = np.array([1.0, 0.0, 0.0])
+    if ax[1] <= ax[0] and ax[1] <= ax[2]:
+        t = np.array([0.0, 1.0, 0.0])
+    if ax[2] <= ax[0] and ax[2] <= ax[1]:
+        t = np.array([0.0, 0.0, 1.0])
+    b = np.cross(n, t)
+    b = b / np.linalg.norm(b)
+    t = np.cross(b, n)
+    return t, b, n

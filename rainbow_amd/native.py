@@ -81,6 +81,8 @@ _PROTOS = {
     "rbh_scene_create": [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_int)],
     "rbh_scene_destroy": [C.c_void_p, C.c_int],
     "rbh_scene_set_params": [C.c_void_p, C.c_int, C.POINTER(Params)],
+    "rbh_scene_set_hinges": [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_void_p],
     "rbh_scene_set_state": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "rbh_scene_get_state": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "rbh_scene_set_step_index": [C.c_void_p, C.c_int, C.c_int],
@@ -239,6 +241,13 @@ class Context:
 
     def set_params(self, sid, params):
         self._check(self.lib.rbh_scene_set_params(self.h, int(sid), C.byref(params)))
+
+    def set_hinges(self, sid, parent, child, arm_p, arm_c, axis_p, axis_c):
+        """Hinges of the scene (scene body indices; body-frame arms and axes, (n, 3))."""
+        parent, child = _i32(parent), _i32(child)
+        arrs = [_f64(np.asarray(a, dtype=np.float64).reshape(-1, 3)) for a in (arm_p, arm_c, axis_p, axis_c)]
+        self._check(self.lib.rbh_scene_set_hinges(self.h, int(sid), len(parent), _ptr(parent), _ptr(child),
+                                                  *[_ptr(a) for a in arrs]))
 
     def set_state(self, sid, r=None, q=None, v=None, w=None):
         arrs = [None if a is None else _f64(a) for a in (r, q, v, w)]

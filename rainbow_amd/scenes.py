@@ -27,6 +27,9 @@ Workloads (seeds from numpy.random.default_rng):
                      (scaled to 2R x H x 2R, at y = 1.5 H, turned by Rx(pi));
                      64^3 SDFs, mu 0.4, dt 0.005, 50 iterations.
 
+* ``cube_hinge_chain`` the reference's hinge scene: 5 cubes of edge 10 in a
+                     row joined by 4 hinges about z, the first cube fixed.
+
 ``sdf_builder`` (optional) replaces the GPU SDF construction, e.g. for
 CPU-only tests.
 """
@@ -122,6 +125,40 @@ def box_stack(bank=None, seed=1, backend="hip", params=None, nx=4, ny=4, nz=4):
             for k in range(ny):
                 c = np.array([(i - 1.5) * 1.05, 0.55 + k * 1.05, (j - 1.5) * 1.05]) + rng.uniform(-0.01, 0.01, 3)
                 add_body(eng, "box_%d_%d_%d" % (i, j, k), box, c, q=Q.identity())
+    return eng
+
+
+def cube_hinge_chain(bank=None, n=5, edge=10.0, backend="hip", params=None, ground=False):
+    """scenes/create_cube_hinge_chain.py:13-87: n cubes turned by Rz(pi/4) in
+    a row along x (corner gaps of 0.1 half-diagonal), the first one fixed,
+    neighbours joined by hinges at the midpoint of their centres about the
+    world z axis; gravity along -y.  ``ground`` adds the ground plane one
+    half-diagonal below the chain."""
+    bank = bank or ShapeBank(resolution=32)
+    prm = dict(time_step=0.01, max_iterations=100, absolute_tolerance=0.0, relative_tolerance=0.0)
+    prm.update(params or {})
+    eng = new_engine(bank, prm, backend)
+    set_friction(eng, 0.5)
+    API.create_gravity_force(eng, "gravity", 9.81, np.array([0.0, 1.0, 0.0]))
+    diameter = np.sqrt(2.0 * edge * edge)
+    radius = diameter / 2.0
+    margin = 0.1 * radius
+    length = n * diameter + (n - 1) * margin
+    if ground:
+        add_ground(eng, bank)
+    cube = bank.get("cube_%g" % edge, lambda: MESH.create_box(edge, edge, edge))
+    names = []
+    y = 2.0 * radius if ground else 0.0
+    for k in range(n):
+        name = "link_%d" % k
+        x = k * (diameter + margin) + radius - length / 2.0
+        add_body(eng, name, cube, np.array([x, y, 0.0]), q=Q.Rz(np.pi / 4), fixed=(k == 0))
+        names.append(name)
+    for k in range(n - 1):
+        hname = names[k] + "_" + names[k + 1]
+        API.create_hinge(eng, hname)
+        o = (API.get_position(eng, names[k]) + API.get_position(eng, names[k + 1])) / 2.0
+        API.set_hinge(eng, hname, names[k], names[k + 1], o, np.array([0.0, 0.0, 1.0]), mode="world")
     return eng
 
 

@@ -170,6 +170,17 @@ class ContactList:
         return len(self) > 0
 
 
+def upload_hinges(ctx, sid, engine):
+    """Engine.hinges (creation order) -> rbh_scene_set_hinges."""
+    index = {b.name: k for k, b in enumerate(engine.bodies.values())}
+    hs = sorted(engine.hinges.values(), key=lambda h: h.idx)
+    for h in hs:
+        if h.parent is None or h.child is None:
+            raise RuntimeError("hinge %s has no parent or child: call set_hinge() first" % h.name)
+    ctx.set_hinges(sid, [index[h.parent.name] for h in hs], [index[h.child.name] for h in hs],
+                   [h.arm_p for h in hs], [h.arm_c for h in hs], [h.axis_p for h in hs], [h.axis_c for h in hs])
+
+
 class HipStepper:
     """Drop-in for steppers.SemiImplicitStepper running on one MI355X."""
 
@@ -182,8 +193,6 @@ class HipStepper:
 
     # -- device scene management
     def _prepare(self, engine):
-        if engine.hinges:
-            raise NotImplementedError("hinge joints are outside the GPU step's scope")
         prec = getattr(engine.params, "precision", "fp32")
         ctx = shared_context(getattr(engine.params, "device", 0), prec)
         key = (id(engine), getattr(engine, "_version", 0), params_key(engine.params), prec)
@@ -195,6 +204,8 @@ class HipStepper:
             desc, keep = engine_desc(engine, ctx)
             self.scene = ctx.scene_create(desc)
             del keep
+            if engine.hinges:
+                upload_hinges(ctx, self.scene, engine)
             self._key = key
             self._bodies = list(engine.bodies.values())
         self.ctx.set_step_index(self.scene, getattr(engine, "step_index", 0))
@@ -220,7 +231,7 @@ class HipStepper:
                  contact_optimization=t["contact_determination"], contact_point_generation=0.0,
                  contact_determination=t["contact_determination"], contact_point_reduction=0.0,
                  collision_detection_time=cd)
-        if lg["contact_points"] > 0:
+        if lg["contact_points"] > 0 or engine.hinges:
             d.update(residuals=lg["residuals"].copy(), iterations=int(lg["iterations"]),
                      exitcode=int(lg["exitcode"]), solver_time=t["coloring"] + t["solver_time"])
         if lg["post_exitcode"] >= 0:

@@ -38,6 +38,7 @@ import numpy as np
 from . import api as API
 from . import mesh as MESH
 from . import math3 as Q
+from .xml_io import read_xml, write_xml   # noqa: F401  (scenes/write_xml.py, and its reader)
 
 
 class ShapeBank:

@@ -105,10 +105,10 @@ struct rbh_ctx {
   DBuf W0, W1, du0, du1, lo, hi, ws0, ws1, ww0, ww1, hd0, hd1;
   DBuf tlo, thi, clo, chi;
   // hinges of all live scenes, packed by scene slot (rebuilt when they change)
-  DBuf h_ids, h_geo, hb_body, hb_inc_off, hb_inc, h_J, h_WJ, h_b, h_r0, h_x, h_sol, d_hmer;
+  DBuf h_ids, h_geo, hb_body, hb_inc_off, hb_inc, h_J, h_WJ, h_b, h_r0, h_sol, d_hmer;
   bool hinges_dirty = false;
   bool any_hinge = false;   // a scene of the batch has hinges (set per run_steps call)
-  int hinge_body_max = 0;   // most hinge bodies of a batch scene (the hinge solve's LDS)
+  size_t hinge_lds = 0;     // dynamic LDS of the hinge solve for the batch (its largest scene)
   // scenes
   std::vector<SceneHost> scenes;
   std::vector<SceneParams> sp_host;
@@ -209,7 +209,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->u_nbody, &c->u_boff, &c->f_loc, &c->f_unit, &c->u_blist, &c->s_loc, &c->s_W, &c->l_cnt,
           &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs, &c->s_JL, &c->s_JA, &c->s_JB,
           &c->xr0, &c->xr1, &c->hd0, &c->hd1, &c->h_ids, &c->h_geo, &c->hb_body, &c->hb_inc_off, &c->hb_inc,
-          &c->h_J, &c->h_WJ, &c->h_b, &c->h_r0, &c->h_x, &c->h_sol, &c->d_hmer};
+          &c->h_J, &c->h_WJ, &c->h_b, &c->h_r0, &c->h_sol, &c->d_hmer};
 }
 
 size_t total_bytes(rbh_ctx* c) {
@@ -389,7 +389,6 @@ template <typename R> int sync_hinges(rbh_ctx* c) {
   if (!rc) rc = ensure(c, c->h_WJ, 20 * nh * V);
   if (!rc) rc = ensure(c, c->h_b, 5 * nh * c->prec);
   if (!rc) rc = ensure(c, c->h_r0, 5 * nh * c->prec);
-  if (!rc) rc = ensure(c, c->h_x, 5 * nh * c->prec);
   if (!rc) rc = ensure(c, c->h_sol, 5 * nh * c->prec);
   if (!rc) rc = ensure(c, c->d_hmer, (size_t)std::max(c->res.cap, 1) * 8, true);
   if (rc) return rc;
@@ -477,7 +476,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.h_ids = P(const int4*, h_ids); d.h_geo = P(const V4<R>*, h_geo); d.hb_body = P(const int*, hb_body);
   d.hb_inc_off = P(const int*, hb_inc_off); d.hb_inc = P(const int*, hb_inc);
   d.h_J = P(V4<R>*, h_J); d.h_WJ = P(V4<R>*, h_WJ); d.h_b = P(R*, h_b); d.h_r0 = P(R*, h_r0);
-  d.h_x = P(R*, h_x); d.h_sol = P(R*, h_sol); d.h_mer = P(double*, d_hmer);
+  d.h_sol = P(R*, h_sol); d.h_mer = P(double*, d_hmer);
   d.hd0 = P(V4<R>*, hd0); d.hd1 = P(V4<R>*, hd1);
   d.err = P(int*, errbuf);
   d.dt = dt;
@@ -628,8 +627,7 @@ hipEvent_t* timing_row(rbh_ctx* c, int i, int todo) {
 template <typename R> void hinge_pass(rbh_ctx* c, const Dev<R>& d, int pass) {
   if (!c->any_hinge) return;
   if (pass == 0) hipLaunchKernelGGL(k_hinge_rows<R>, dim3(c->ns), dim3(64), 0, c->stream, d);
-  const size_t lds = std::max<size_t>(2 * (size_t)c->hinge_body_max * sizeof(V4<R>), 16);
-  hipLaunchKernelGGL(k_hinge_solve<R>, dim3(c->ns), dim3(64), lds, c->stream, d, pass);
+  hipLaunchKernelGGL(k_hinge_solve<R>, dim3(c->ns), dim3(64), c->hinge_lds, c->stream, d, pass);
 }
 
 template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, hipEvent_t* ev, bool multi, int multi_grid) {
@@ -761,7 +759,7 @@ template <typename R> int setup_solve_lds(rbh_ctx* c) {
       (void)hipGetLastError();
       return fail(c, RBH_E_HIP, "solve kernels cannot use " + std::to_string(bytes) + " bytes of LDS");
     }
-  const int hbytes = (int)(2 * HINGE_BODY_MAX * sizeof(V4<R>));
+  const int hbytes = (int)hinge_lds_bytes<R>(HINGE_BODY_MAX, HINGE_MAX);
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_hinge_solve<R>), hipFuncAttributeMaxDynamicSharedMemorySize,
                           hbytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -775,11 +773,12 @@ template <typename R> int setup_solve_lds(rbh_ctx* c) {
 template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flags) {
   c->any_post = false;
   c->any_hinge = false;
-  c->hinge_body_max = 0;
+  c->hinge_lds = 16;
   for (int slot : c->batch) {
+    const SceneParams& sp = c->sp_host[slot];
     c->any_post = c->any_post || c->scenes[slot].params.use_post_stabilization != 0;
-    c->any_hinge = c->any_hinge || c->sp_host[slot].h_cnt > 0;
-    c->hinge_body_max = std::max(c->hinge_body_max, c->sp_host[slot].hb_cnt);
+    c->any_hinge = c->any_hinge || sp.h_cnt > 0;
+    c->hinge_lds = std::max(c->hinge_lds, hinge_lds_bytes<R>(sp.hb_cnt, sp.h_cnt));
   }
   const bool timing = (flags & RBH_STEP_TIMING) != 0;
   const bool use_graph = (flags & RBH_STEP_GRAPH) != 0;
@@ -1299,6 +1298,7 @@ int rbh_scene_set_hinges(rbh_ctx* c, int id, int n, const int* parent, const int
   }
   if (nj > HINGE_BODY_MAX)
     return fail(c, RBH_E_ARG, "hinges join more than " + std::to_string(HINGE_BODY_MAX) + " bodies of one scene");
+  if (n > HINGE_MAX) return fail(c, RBH_E_ARG, "more than " + std::to_string(HINGE_MAX) + " hinges in one scene");
   s.h_parent.assign(parent, parent + n);
   s.h_child.assign(child, child + n);
   s.h_geo.assign(12 * (size_t)n, 0.0);

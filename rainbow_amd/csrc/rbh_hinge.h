@@ -17,11 +17,14 @@
 //          W J^T sol per body for the velocity update.
 // A hinge sweep is serial over the hinges (Gauss-Seidel); inside one hinge
 // the five rows update from the same w (hinges.py:88), one lane per row, and
-// the twelve body entries of W J^T delta one lane each.
+// the twelve body entries of W J^T delta one lane each.  w and the iterate
+// stay in LDS; each hinge's rows are loaded while the previous hinge
+// updates, so a hinge costs two wave barriers and LDS traffic.
 // (included inside namespace rbh by rbh_kernels.h)
 #pragma once
 
 constexpr int HINGE_BODY_MAX = 1024;   // bodies joined by hinges in one scene (LDS: two V4 each)
+constexpr int HINGE_MAX = 1536;        // hinges of one scene (LDS: 5 iterates each; FP64 total 124 KB)
 
 // One J row (parent lin, parent ang, child lin, child ang) times body-space
 // vectors, term by term in the csr column order of hinges.py:184 (the body of
@@ -101,14 +104,14 @@ template <typename R> __global__ void __launch_bounds__(64) k_hinge_rows(Dev<R> 
       }
       d.h_b[5 * h + i] = (ju + jd) + g[i];
       d.h_r0[5 * h + i] = R(1) / (dg == R(0) ? R(1) : dg);
-      d.h_x[5 * h + i] = R(0);
       d.h_sol[5 * h + i] = R(0);
     }
   }
 }
 
 // w = W J^T x for the scene's hinge bodies (hinges.py:65, :56): one running
-// sum per body entry over the body's hinges in hinge order, rows 0..4.
+// sum per body entry over the body's hinges in hinge order, rows 0..4; x
+// holds the scene's rows, row i of hinge h at 5 (h - sp.h_off) + i.
 template <typename R>
 __device__ __forceinline__ void hinge_body_w(const Dev<R>& d, const SceneParams& sp, const R* x, V4<R>* wl, V4<R>* wa) {
   for (int j = threadIdx.x; j < sp.hb_cnt; j += blockDim.x) {
@@ -117,7 +120,7 @@ __device__ __forceinline__ void hinge_body_w(const Dev<R>& d, const SceneParams&
     for (int e = d.hb_inc_off[g]; e < d.hb_inc_off[g + 1]; ++e) {
       const int inc = d.hb_inc[e], h = inc >> 1, side = inc & 1;
       for (int i = 0; i < 5; ++i) {
-        const R xi = x[5 * h + i];
+        const R xi = x[5 * (h - sp.h_off) + i];
         const V4<R> l = d.h_WJ[20 * h + 4 * i + 2 * side], a = d.h_WJ[20 * h + 4 * i + 2 * side + 1];
         sl.x += l.x * xi; sl.y += l.y * xi; sl.z += l.z * xi;
         sa.x += a.x * xi; sa.y += a.y * xi; sa.z += a.z * xi;
@@ -134,13 +137,75 @@ __device__ __forceinline__ double wave_max_d(double v) {
   return v;
 }
 
-template <typename R> __device__ __forceinline__ R& comp(V4<R>& a, int c) {
-  return c == 0 ? a.x : (c == 1 ? a.y : a.z);
+
+// Per-hinge constants a lane needs for one Gauss-Seidel update: lanes 0-4
+// hold row `lane` (J, b, r0), lanes 0-11 the W J^T entries of body entry
+// `lane` for rows 0-4.  Loaded one hinge ahead of their use.
+template <typename R> struct HingeRegs {
+  int4 id;
+  V4<R> J[4];
+  R b, r0;
+  R wj[5];
+};
+
+template <typename R>
+__device__ __forceinline__ void hinge_load(const Dev<R>& d, int h, int lane, HingeRegs<R>& g) {
+  g.id = d.h_ids[h];
+  if (lane < 5) {
+    const V4<R>* J = d.h_J + 20 * h + 4 * lane;
+    g.J[0] = J[0]; g.J[1] = J[1]; g.J[2] = J[2]; g.J[3] = J[3];
+    g.b = d.h_b[5 * h + lane];
+    g.r0 = d.h_r0[5 * h + lane];
+  }
+  if (lane < 12) {
+    const int part = lane / 3, c = lane - 3 * part;
+    // one element each (a runtime index into a vector register would go through scratch)
+    for (int i = 0; i < 5; ++i) g.wj[i] = reinterpret_cast<const R*>(d.h_WJ + 20 * h + 4 * i + part)[c];
+  }
 }
 
-// One wavefront per batch scene (dynamic LDS: 2 hb_cnt V4).  pass 0 records
-// the hinge merit contributions for the contact solve; pass 1 runs to the
-// joint stop and writes hd0/hd1 (see the top of this file).
+constexpr int HINGE_AHEAD = 4;   // hinges whose rows are loaded ahead of their update
+
+// One hinge's Gauss-Seidel update on w (LDS) and its iterate x (LDS, rows
+// from x0): hinges.py:88 then :101.
+template <typename R>
+__device__ __forceinline__ void hinge_update(V4<R>* wl, V4<R>* wa, R* x, R* s_delta, int x0, R rs,
+                                             const HingeRegs<R>& g, int lane) {
+  const int4 id = g.id;
+  if (lane < 5) {
+    // x_b -= r_b (J w + b)_b with every row of the block from the same w
+    const R jw = hinge_rowdot(id.x < id.y, g.J, xyz(wl[id.z]), xyz(wa[id.z]), xyz(wl[id.w]), xyz(wa[id.w]));
+    const R xo = x[x0 + lane];
+    const R xn = xo - (g.r0 * rs) * (jw + g.b);
+    x[x0 + lane] = xn;
+    s_delta[lane] = xn - xo;
+  }
+  // one wavefront: LDS operations stay in issue order, and a fence-free sync
+  // keeps the loads of the next hinges in flight (a workgroup barrier would
+  // wait for them)
+  wave_sync_lds();
+  if (lane < 12) {
+    // w += W J^T_b delta, one body entry per lane
+    const int part = lane / 3, c = lane - 3 * part;
+    R t = R(0);
+    for (int i = 0; i < 5; ++i) t += g.wj[i] * s_delta[i];
+    V4<R>* wv = (part & 1) ? wa : wl;
+    R* dst = reinterpret_cast<R*>(wv + (part < 2 ? id.z : id.w)) + c;
+    *dst = *dst + t;
+  }
+  wave_sync_lds();
+}
+
+// Bytes of dynamic LDS the hinge solve of a scene needs: w (two V4 per hinge
+// body) and the iterate x (5 per hinge).
+template <typename R> __host__ __device__ constexpr size_t hinge_lds_bytes(int hb, int nh) {
+  return (2 * (size_t)hb * sizeof(V4<R>) + 5 * (size_t)nh * sizeof(R) + 15) & ~(size_t)15;
+}
+
+// One wavefront per batch scene.  pass 0 records the hinge merit
+// contributions for the contact solve; pass 1 runs to the joint stop and
+// writes hd0/hd1 (see the top of this file).  w and x live in LDS; the
+// accepted solution sol in h_sol.
 template <typename R> __global__ void __launch_bounds__(64) k_hinge_solve(Dev<R> d, int pass) {
   extern __shared__ __align__(16) unsigned char hinge_smem[];
   __shared__ R s_delta[5];
@@ -156,12 +221,12 @@ template <typename R> __global__ void __launch_bounds__(64) k_hinge_solve(Dev<R>
   V4<R>* wl = reinterpret_cast<V4<R>*>(hinge_smem);
   V4<R>* wa = wl + sp.hb_cnt;
   const int lane = threadIdx.x;
-  const int h0 = sp.h_off, h1 = sp.h_off + sp.h_cnt;
-  R* x = d.h_x;
-  R* sol = d.h_sol;
+  const int h0 = sp.h_off, h1 = sp.h_off + sp.h_cnt, nrow = 5 * sp.h_cnt;
+  R* x = reinterpret_cast<R*>(wa + sp.hb_cnt);   // row 5 (h - h0) + i
+  R* sol = d.h_sol + 5 * h0;
   double* res = d.residuals + sp.res_off;
   double* hm = d.h_mer + sp.res_off;
-  for (int i = 5 * h0 + lane; i < 5 * h1; i += 64) { x[i] = R(0); sol[i] = R(0); }
+  for (int i = lane; i < nrow; i += 64) { x[i] = R(0); sol[i] = R(0); }
   if (own)
     for (int i = lane; i < sp.max_iter; i += 64) res[i] = INFINITY;
   __syncthreads();
@@ -170,36 +235,23 @@ template <typename R> __global__ void __launch_bounds__(64) k_hinge_solve(Dev<R>
   for (int it = 0; it < n_it; ++it) {
     const R rs = R(ctl.rscale);
     hinge_body_w(d, sp, x, wl, wa);
+    // rows of the next HINGE_AHEAD hinges in flight while one updates
+    HingeRegs<R> ring[HINGE_AHEAD];
+#pragma unroll
+    for (int k = 0; k < HINGE_AHEAD; ++k)
+      if (h0 + k < h1) hinge_load(d, h0 + k, lane, ring[k]);
     __syncthreads();
-    for (int h = h0; h < h1; ++h) {
-      const int4 id = d.h_ids[h];
-      if (lane < 5) {
-        // hinges.py:88: x_b -= r_b (J w + b)_b with every row of the block from the same w
-        const V4<R>* J = d.h_J + 20 * h + 4 * lane;
-        const R jw = hinge_rowdot(id.x < id.y, J, xyz(wl[id.z]), xyz(wa[id.z]), xyz(wl[id.w]), xyz(wa[id.w]));
-        const int i = 5 * h + lane;
-        const R xo = x[i];
-        const R xn = xo - (d.h_r0[i] * rs) * (jw + d.h_b[i]);
-        x[i] = xn;
-        s_delta[lane] = xn - xo;
+    for (int hh = h0; hh < h1; hh += HINGE_AHEAD) {
+#pragma unroll
+      for (int k = 0; k < HINGE_AHEAD; ++k) {
+        const int h = hh + k;
+        if (h >= h1) break;
+        hinge_update(wl, wa, x, s_delta, 5 * (h - h0), rs, ring[k], lane);
+        if (h + HINGE_AHEAD < h1) hinge_load(d, h + HINGE_AHEAD, lane, ring[k]);
       }
-      __syncthreads();
-      if (lane < 12) {
-        // hinges.py:101: w += W J^T_b delta, one body entry per lane
-        const int part = lane / 3, c = lane - 3 * part;
-        R t = R(0);
-        for (int i = 0; i < 5; ++i) {
-          V4<R> a = d.h_WJ[20 * h + 4 * i + part];
-          t += comp(a, c) * s_delta[i];
-        }
-        V4<R>* wv = (part & 1) ? wa : wl;
-        V4<R>& dst = wv[part < 2 ? id.z : id.w];
-        comp(dst, c) = comp(dst, c) + t;
-      }
-      __syncthreads();
     }
     double m = 0.0;
-    for (int i = 5 * h0 + lane; i < 5 * h1; i += 64) m = fmax(m, fabs((double)(x[i] - sol[i])));
+    for (int i = lane; i < nrow; i += 64) m = fmax(m, fabs((double)(x[i] - sol[i])));
     m = wave_max_d(m);
     int dec;
     if (own) {
@@ -208,7 +260,7 @@ template <typename R> __global__ void __launch_bounds__(64) k_hinge_solve(Dev<R>
       dec = ctl.judge(sp, m);
       if (pass == 0 && lane == 0) hm[it] = ctl.gm;
     }
-    for (int i = 5 * h0 + lane; i < 5 * h1; i += 64) {
+    for (int i = lane; i < nrow; i += 64) {
       if (dec & 1) sol[i] = x[i];
       else x[i] = sol[i];
     }

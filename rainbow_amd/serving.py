@@ -21,8 +21,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import native
-from .stepper import engine_desc, gather_state, shared_context
+from . import native, xml_io
+from .stepper import engine_desc, gather_state, shared_context, upload_hinges
 
 
 @dataclass
@@ -71,6 +71,34 @@ class Server:
         if tuning:
             self.ctx.set_tuning(**tuning)
 
+    def _create(self, req):
+        sid = self.ctx.scene_create(req._desc)
+        self.ctx.set_state(sid, *req._state0)
+        self.ctx.set_step_index(sid, int(getattr(req.engine, "step_index", 0)))
+        if getattr(req.engine, "hinges", None):
+            upload_hinges(self.ctx, sid, req.engine)
+        return sid
+
+    def checkpoint(self, req, xml_filename):
+        """Write a finished request's scene, at its final state and step count,
+        as a lossless scene file; ``xml_io.read_xml`` of it is a request that
+        continues bitwise where this one ended."""
+        if req.state is None:
+            raise RuntimeError("checkpoint(): request %d has not finished" % req.rid)
+        eng = req.engine
+        r, q, v, w = req.state
+        saved = [(b.r, b.q, b.v, b.w) for b in eng.bodies.values()]
+        step0 = getattr(eng, "step_index", 0)
+        try:
+            for k, b in enumerate(eng.bodies.values()):
+                b.r, b.q, b.v, b.w = r[k].copy(), q[k].copy(), v[k].copy(), w[k].copy()
+            eng.step_index = step0 + req.n_steps
+            xml_io.write_xml(eng, xml_filename, lossless=True)
+        finally:
+            for b, (r0, q0, v0, w0) in zip(eng.bodies.values(), saved):
+                b.r, b.q, b.v, b.w = r0, q0, v0, w0
+            eng.step_index = step0
+
     def prepare(self, req):
         """Build the device description of a request's scene (host work done ahead of arrival)."""
         req._desc, req._keep = engine_desc(req.engine, self.ctx)
@@ -99,8 +127,7 @@ class Server:
             while waiting and used + waiting[0].footprint <= self.budget and \
                     (self.max_batch is None or len(active) < self.max_batch):
                 req = waiting.pop(0)
-                sid = self.ctx.scene_create(req._desc)
-                self.ctx.set_state(sid, *req._state0)
+                sid = self._create(req)
                 used += req.footprint
                 req.admitted = clock() - t0
                 active[sid] = [req, 0]
@@ -141,9 +168,8 @@ class Server:
 
     def run_alone(self, req):
         """The request by itself from its initial state; returns its final state (r, q, v, w)."""
-        sid = self.ctx.scene_create(req._desc)
+        sid = self._create(req)
         try:
-            self.ctx.set_state(sid, *req._state0)
             self.ctx.step([sid], req.n_steps, 0.0, 0)
             return self.ctx.get_state(sid, len(req.engine.bodies))
         finally:

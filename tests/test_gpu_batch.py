@@ -133,3 +133,29 @@ def test_serving_alone_equals_mixed(bank):
                 assert np.array_equal(x, y), r.rid
     finally:
         ctx.close()
+
+
+def test_serving_hinges_and_checkpoint(bank, tmp_path):
+    """A hinged request is served like any other (bitwise its run alone); a
+    finished request checkpointed to a scene file and read back continues
+    bitwise like the same request run for all the steps at once."""
+    from rainbow_amd import xml_io
+    ctx = native.Context(0, "fp32")
+    try:
+        srv = serving.Server(ctx=ctx, budget=1 << 30)
+        chain = scenes.cube_hinge_chain(bank, n=4, edge=1.0, params=dict(precision="fp32"))
+        drop = scenes.serving_request(4200, bank, nmax=40, params=dict(precision="fp32"))
+        reqs = [srv.prepare(serving.Request(0, chain, 12, 0.0)), srv.prepare(serving.Request(1, drop, 10, 0.0))]
+        srv.run(reqs)
+        for r in reqs:
+            for x, y in zip(srv.run_alone(r), r.state):
+                assert np.array_equal(x, y), r.rid
+        fn = str(tmp_path / "drop.xml")
+        srv.checkpoint(reqs[1], fn)
+        resumed = srv.prepare(serving.Request(2, xml_io.read_xml(fn), 8, 0.0))
+        srv.run([resumed])
+        whole = srv.prepare(serving.Request(3, drop, 18, 0.0))
+        for x, y in zip(srv.run_alone(whole), resumed.state):
+            assert np.array_equal(x, y)
+    finally:
+        ctx.close()

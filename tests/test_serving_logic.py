@@ -25,6 +25,9 @@ class FakeCtx:
     def set_state(self, sid, r, q, v, w):
         self.live[sid]["r"] = r.copy()
 
+    def set_step_index(self, sid, k):
+        self.live[sid]["step_index"] = k
+
     def step(self, ids, n, dt, flags):
         self.batches.append(list(ids))
         for i in ids:

@@ -40,6 +40,10 @@ torus knots (about 10^4 triangles each, 64^3 SDFs) poured into a fixed funnel,
 300 steps from rest; contact-iters/s over the run, and per-stage times of the
 last 50 steps.
 
+hinges (on by default, --no-hinges to skip): the reference's chain of five
+hinged cubes alone and as a batch of 256 scenes, and a chain of 256 cubes;
+100 sweeps per step; hinge-iterations/s beside the FP64 oracle's.
+
 --copy-calibration BYTES runs only a plain device-to-device copy of BYTES
 bytes (the reference kernel for scaling the profiler's byte counters).
 """
@@ -308,6 +312,63 @@ def stress_leg(steps):
                           "natural_residual": lg["natural_residual"]}}
 
 
+def hinges_leg(steps, graph=True):
+    """Hinge chains (constraints/hinges.py): the reference's chain of five cubes
+    of edge 10 alone and as a batch of 256 copies, and one chain of 256 unit
+    cubes; 100 sweeps per step (tolerances 0), FP32.  A hinge-iteration is one
+    hinge's 5-row update in one sweep.  CPU: the FP64 oracle on one step of
+    the same scenes (10 sweeps for the long chain)."""
+    import oracle.scene as OS
+    import oracle.stepper as OST
+    from rainbow_amd import native, scenes
+    from rainbow_amd.stepper import engine_desc, gather_state, upload_hinges
+    bank = scenes.ShapeBank(resolution=32)
+    prm = dict(precision="fp32", max_iterations=100, absolute_tolerance=0.0, relative_tolerance=0.0)
+    out = {}
+    for name, n, edge, copies in (("reference_chain_5", 5, 10.0, 1), ("reference_chain_5_x256", 5, 10.0, 256),
+                                  ("chain_256", 256, 1.0, 1)):
+        ctx = native.Context(0, "fp32")
+        try:
+            engines = [scenes.cube_hinge_chain(bank, n=n, edge=edge, params=prm) for _ in range(copies)]
+            ids = []
+            for e in engines:
+                desc, keep = engine_desc(e, ctx)
+                sid = ctx.scene_create(desc)
+                ctx.set_state(sid, *gather_state(list(e.bodies.values())))
+                upload_hinges(ctx, sid, e)
+                ids.append(sid)
+            flags = native.STEP_GRAPH if (copies > 1 and graph) else 0
+            ctx.step(ids, 10, 0.01, flags)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.step(ids, steps, 0.01, flags)
+            ctx.sync()
+            sec = time.perf_counter() - t0
+            ctx.step(ids[:1], 1, 0.01, native.STEP_TIMING)
+            tm = ctx.get_timings()
+            lg = ctx.get_log(ids[0], 100)
+            hi = copies * (n - 1) * lg["sweeps"] * steps
+            out[name] = {"scenes": copies, "hinges_per_scene": n - 1, "sweeps": lg["sweeps"],
+                         "contacts": lg["contact_points"], "ms_per_step": 1e3 * sec / steps,
+                         "hinge_iters_per_s": hi / sec, "solver_ms_one_scene": round(1e3 * tm["solver_time"], 4)}
+            if copies == 1:
+                sweeps = 100 if n <= 8 else 10
+                sc = OS.from_engine(scenes.cube_hinge_chain(
+                    scenes.ShapeBank(resolution=32), n=n, edge=edge, backend=None,
+                    params=dict(max_iterations=sweeps, absolute_tolerance=0.0, relative_tolerance=0.0)))
+                t0 = time.perf_counter()
+                OST.step(sc, 0.01)
+                cs = time.perf_counter() - t0
+                out[name]["cpu_baseline"] = {"value": (n - 1) * sweeps / cs, "unit": "hinge-iters/s", "cores": 1,
+                                             "kind": "port", "sample": "one oracle step with %d sweeps" % sweeps,
+                                             "seconds": cs}
+        finally:
+            ctx.close()
+        log("hinges %s: %.3g hinge-iters/s, %.3f ms/step" % (name, out[name]["hinge_iters_per_s"],
+                                                             out[name]["ms_per_step"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,7 +385,10 @@ def main():
                     help="arrival rates in requests/s (a saturation run follows)")
     ap.add_argument("--no-stress", action="store_true", help="skip the funnel stress leg")
     ap.add_argument("--stress-steps", type=int, default=300)
-    ap.add_argument("--only", choices=("batched", "serving", "stress"), default=None,
+    ap.add_argument("--no-hinges", action="store_true", help="skip the hinge-chain leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the hinge leg's batched steps one kernel at a time instead of replaying a hipGraph")
+    ap.add_argument("--only", choices=("batched", "serving", "stress", "hinges"), default=None,
                     help="run one of the extra legs alone (for profiling) and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -345,6 +409,9 @@ def main():
         return
     if args.only == "stress":
         print(json.dumps(stress_leg(args.stress_steps)), flush=True)
+        return
+    if args.only == "hinges":
+        print(json.dumps(hinges_leg(50, graph=not args.no_graph)), flush=True)
         return
 
     from rainbow_amd import native, scenes
@@ -429,6 +496,9 @@ def main():
         result["stress"] = stress_leg(args.stress_steps)
         log("stress: %.3g contact-iters/s, %.2f ms/step" % (result["stress"]["contact_iters_per_s"],
                                                             result["stress"]["ms_per_step"]))
+    if not args.no_hinges:
+        log("hinges leg ...")
+        result["hinges"] = hinges_leg(50, graph=not args.no_graph)
     print(json.dumps(result), flush=True)
 
 

@@ -139,6 +139,58 @@ def test_reference_chain_fp32(bank):
     assert np.abs(got - sc.r).max() < 0.05, np.abs(got - sc.r).max()
 
 
+def test_set_hinges_rejects_bad_input(bank):
+    eng = scenes.cube_hinge_chain(bank, n=3, edge=1.0, params=dict(precision="fp32"))
+    ctx = native.Context(0, "fp32")
+    try:
+        desc, keep = engine_desc(eng, ctx)
+        sid = ctx.scene_create(desc)
+        z = np.zeros((1, 3))
+        with pytest.raises(native.RbhError):
+            ctx.set_hinges(sid, [1], [1], z, z, z, z)          # parent and child are one body
+        with pytest.raises(native.RbhError):
+            ctx.set_hinges(sid, [0], [7], z, z, z, z)          # no body 7 in a 3-body scene
+        n = 1537
+        zz = np.zeros((n, 3))
+        with pytest.raises(native.RbhError):
+            ctx.set_hinges(sid, [0] * n, [1] * n, zz, zz, zz, zz)   # more hinges than a scene holds
+        ctx.set_hinges(sid, [0], [1], z, z, z, z)                # a valid one is accepted
+        ctx.set_hinges(sid, [], [], zz[:0], zz[:0], zz[:0], zz[:0])   # and removed again
+    finally:
+        ctx.close()
+
+
+def test_graph_replay_with_hinges(bank):
+    """hipGraph replays of a batch with hinged scenes equal eager steps,
+    replay after replay, with eager work and synchronisations between."""
+    def run(graph):
+        ctx = native.Context(0, "fp32")
+        try:
+            engines = [hinged_pair(bank, dict(precision="fp32")),
+                       scenes.cube_hinge_chain(bank, n=4, edge=1.0, params=dict(precision="fp32"))]
+            ids = []
+            for e in engines:
+                desc, keep = engine_desc(e, ctx)
+                sid = ctx.scene_create(desc)
+                ctx.set_state(sid, *gather_state(list(e.bodies.values())))
+                upload_hinges(ctx, sid, e)
+                ids.append(sid)
+            snaps = []
+            for _ in range(5):
+                ctx.step(ids, 1, 0.01, native.STEP_GRAPH if graph else 0)
+                ctx.sync()
+                snaps.append([list(ctx.get_state(sid, len(e.bodies))) for sid, e in zip(ids, engines)])
+            return snaps
+        finally:
+            ctx.close()
+
+    a, b = run(False), run(True)
+    for sa, sb in zip(a, b):
+        for x, y in zip(sa, sb):
+            for u, v in zip(x, y):
+                assert np.array_equal(u, v)
+
+
 def test_batch_with_hinges_bitwise(bank):
     def make():
         return [hinged_pair(bank, dict(precision="fp32")),

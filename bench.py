@@ -17,6 +17,10 @@ HBM peak, with 288 algorithmic bytes per contact-iteration; its duration is
 the mean over the timed steps of HIP events recorded on the library's stream.
 traffic: HBM bytes per solve launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
 of this same command, read from the committed summary under profiles/.
+roofline.latency: the solve is bound by its dependent colour phases (each
+sweep visits the colours of its longest sweep lane one after another); one
+step after the timed region gives colours x sweeps and the microseconds per
+phase.
 
 cpu_baseline: the FP64 NumPy restatement of the step, single-threaded, timed
 on one step of the same scene from the GPU's end state.
@@ -440,6 +444,14 @@ def main():
     log("timed %d steps in %.3f s; totals %s" % (args.steps, sec, tot))
     assert tot["steps"] == args.steps, tot
 
+    # one more step, outside the timed region, for the solve's latency figures:
+    # its sweeps times the colours of its longest sweep lane are dependent phases
+    ctx.step([sid], 1, dt, native.STEP_TIMING)
+    ctx.sync()
+    lat_tm = ctx.get_timings()
+    lat_lg = ctx.get_log(sid, eng.params.max_iterations)
+    phases = max(1, lat_lg["colors"] * lat_lg["sweeps"])
+
     citers = tot["contact_iters"]
     value = citers / sec
     tm = {k: 1e3 * v for k, v in tm.items()}          # ms per step, mean over the timed steps
@@ -466,7 +478,11 @@ def main():
                    "steps_timed": "%d..%d" % (args.warmup + 1, args.warmup + args.steps)},
         "roofline": {"kernel": "k_solve_multi", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
-                     "bytes_per_launch": bytes_launch, "launch_ms": solve_ms},
+                     "bytes_per_launch": bytes_launch, "launch_ms": solve_ms,
+                     "latency": {"note": "the solve is bound by dependent colour phases, not by HBM",
+                                 "colours": lat_lg["colors"], "sweeps": lat_lg["sweeps"],
+                                 "phases": phases, "launch_ms": round(1e3 * lat_tm["solver_time"], 4),
+                                 "us_per_phase": 1e6 * lat_tm["solver_time"] / phases}},
         "work": {"contacts_per_step": tot["contacts"] / args.steps,
                  "sweeps_per_step": tot["sweeps"] / args.steps,
                  "candidates_per_step": tot["candidates"] / args.steps,
@@ -478,7 +494,7 @@ def main():
     }
     if not args.no_cpu:
         log("cpu baseline: one oracle step ...")
-        cb = cpu_baseline(eng, st, args.warmup + args.steps)
+        cb = cpu_baseline(eng, st, args.warmup + args.steps + 1)
         result["cpu_baseline"] = cb
         log("cpu baseline: %.3g contact-iters/s (%.1f s)" % (cb["value"], cb["seconds"]))
     else:

@@ -119,6 +119,8 @@ typedef struct rbh_tuning {
   int resident_bodies;    /* most bodies of a solve group kept in LDS (multiple of 64; same bounds) */
   int lanes_per_group;    /* a single scene's cooperative grid: lanes per workgroup (default 4) */
   int waves_per_group;    /* wavefronts per workgroup of the batched solve: 4 (default) or 8 */
+  int color_lds_contacts; /* most contacts of a colouring unit whose colouring state is kept in LDS
+                             (default and upper bound 4096; larger units keep it in global memory) */
 } rbh_tuning;
 
 typedef struct rbh_device_info {

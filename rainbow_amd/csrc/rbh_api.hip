@@ -456,6 +456,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.unit_T = c->unit_T;
   d.unit_split = multi ? 0 : (c->tune.lanes_per_scene > 0 ? c->tune.lanes_per_scene : batch_waves(c));
   d.u_T = P(int*, u_T);
+  d.color_cap = c->tune.color_lds_contacts > 0 ? c->tune.color_lds_contacts : UNIT_CAP;
   d.isl_unit = P(int*, isl_unit); d.u_nbody = P(int*, u_nbody); d.u_boff = P(int*, u_boff);
   d.f_loc = P(int*, f_loc); d.f_unit = P(int*, f_unit); d.u_blist = P(int*, u_blist);
   d.s_loc = P(int2*, s_loc); d.s_W = P(V4<R>*, s_W);
@@ -681,7 +682,7 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   scan<int>(c, d.u_nbody, d.u_boff, d.n_units, c->cap_units, nullptr);
   hipLaunchKernelGGL(k_unit_blist<R>, dim3(grid_for(nb, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_color_small<R>, dim3(c->cap_units), dim3(64), 0, s, d);
-  hipLaunchKernelGGL(k_color_units<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_color_units<R>, dim3(c->cap_units), dim3(COLOR_UNIT_BLOCK), 0, s, d);
   hipLaunchKernelGGL(k_order_lanes<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
   if (multi) {
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(d.merit), 2 * c->max_iter_batch, 0);
@@ -1451,7 +1452,7 @@ int rbh_set_tuning(rbh_ctx* c, const rbh_tuning* t) {
   if (t->unit_size < 0 || t->unit_size > 4096 || t->lanes_per_scene < 0 || t->lanes_per_scene > 64 ||
       t->resident_contacts < 0 || t->resident_bodies < 0 || t->lanes_per_group < 0 || t->lanes_per_group > 64 ||
       (t->waves_per_group != 0 && t->waves_per_group != 4 && t->waves_per_group != 8) ||
-      (t->resident_contacts % 64) || (t->resident_bodies % 64))
+      (t->resident_contacts % 64) || (t->resident_bodies % 64) || t->color_lds_contacts < 0)
     return fail(c, RBH_E_ARG, "tuning values out of range (resident capacities are multiples of 64)");
   const int cap = c->prec == 4 ? ResCap<float>::C : ResCap<double>::C;
   if (t->resident_contacts > cap || t->resident_bodies > cap)

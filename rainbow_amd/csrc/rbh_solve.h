@@ -211,11 +211,14 @@ template <typename R> __global__ void k_csr_fill(Dev<R> d) {
 // contacts are adjacent if they share a free body.  Units of up to
 // SMALL_UNIT contacts are coloured by one wavefront with the palettes in LDS
 // (ceil((2K + 2) / 32) words per contact: colours of an island stay below
-// twice its size); larger units by a workgroup of four wavefronts, palettes
-// in global memory (PALW words per contact).
+// twice its size); larger units by a workgroup of eight wavefronts, palettes
+// in global memory (PALW words per contact), the rest of the state in LDS up
+// to UNIT_CAP contacts (gfx950: 160 KB of LDS per workgroup), in global
+// memory beyond.
 constexpr int COLOR_BLOCK = 256;
-constexpr int UNIT_CAP = 2048;        // contacts of a large unit staged in LDS
-constexpr int LIST_CAP = 4096;        // body-list entries staged in LDS
+constexpr int COLOR_UNIT_BLOCK = 512;   // threads colouring one large unit (8 wavefronts)
+constexpr int UNIT_CAP = 4096;        // contacts of a large unit staged in LDS
+constexpr int LIST_CAP = 8192;        // body-list entries staged in LDS
 constexpr int SMALL_UNIT = 256;
 constexpr int SMALL_BODY = 32;        // bodies with at most this many contacts are checked entry by entry
 constexpr int SMALL_LIST = 1024;
@@ -225,14 +228,14 @@ template <int NT, int CAP, int LCAP, int PAL> struct ColorLds {
   // colours a palette can hold (at most PALMAX in either palette layout)
   static constexpr int CC = PALMAX;
   int key[CAP];                       // contact index within its scene (random draws, ranks)
-  int isl[CAP];
   int fa[CAP], fb[CAP];               // free flat bodies (-1 if fixed)
   unsigned short ao[CAP], ac[CAP], bo[CAP], bcn[CAP];
   unsigned short col[CAP];
   unsigned short pc[CAP];             // palette sizes
   unsigned char done[CAP];            // 0 uncoloured, 1 coloured, 2 won this round
   unsigned short list[LCAP];
-  unsigned short bl_off[LCAP], bl_cnt[LCAP];   // bodies with more than SMALL_BODY contacts: their lists
+  // bodies with more than SMALL_BODY contacts (each holds that many list entries): their lists
+  unsigned short bl_off[LCAP / (SMALL_BODY + 1) + 1], bl_cnt[LCAP / (SMALL_BODY + 1) + 1];
   unsigned short ent_off[LCAP], ent_cnt[LCAP]; // per list entry of a smaller body: that body's list (cnt 0: larger body)
   unsigned char lose[CAP];            // drew a colour that another contact of one of its bodies drew
   unsigned seen[NT / 64][CC / 32];    // per wavefront, at the body in hand: colours drawn ...
@@ -247,8 +250,9 @@ template <int NT, int CAP, int LCAP, int PAL> struct ColorLds {
 template <typename S> struct LdsAcc {
   static constexpr bool PER_BODY = true;
   S* s;
+  const int* c_lab; const int* uperm; int p0;   // island labels (read only when palettes run empty)
   __device__ int key(int i) const { return s->key[i]; }
-  __device__ int isl(int i) const { return s->isl[i]; }
+  __device__ int isl(int i) const { return c_lab[uperm[p0 + i]]; }
   __device__ int col(int i) const { return s->col[i]; }
   __device__ void set_col(int i, int c) { s->col[i] = (unsigned short)c; }
   __device__ int done(int i) const { return s->done[i]; }
@@ -331,10 +335,16 @@ __device__ __forceinline__ int pal_select(const unsigned* p, int pw, int idx) {
   return -1;
 }
 
+// sum over the NT threads of a workgroup (sh: NT / 64 ints)
 template <int NT> __device__ __forceinline__ int block_sum(int v, int* sh) {
-  if (NT == 64) return wave_sum(v);
-  int tot;
-  block_exclusive_scan(v, sh, tot);
+  v = wave_sum(v);
+  if (NT == 64) return v;
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int tot = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) tot += sh[w];
+  __syncthreads();
   return tot;
 }
 
@@ -600,7 +610,7 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
     return;
   }
   const unsigned long long seed = sp.seed, stepk = (unsigned long long)st.step_index;
-  bool fits = K <= CAP;
+  bool fits = K <= min(CAP, d.color_cap);
   if (fits) {
     // stage contacts and the bodies' contact lists (claimed once per body)
     if (threadIdx.x == 0) { s.nlist = 0; s.flag = 0; s.nbody = 0; }
@@ -611,7 +621,6 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
       int fa = d.b_fixed[A] ? -1 : d.b2f[A];
       int fb = d.b_fixed[B] ? -1 : d.b2f[B];
       s.key[i] = k - st.k_off;
-      s.isl[i] = d.c_lab[k];
       s.fa[i] = fa;
       s.fb[i] = fb;
       if (fa >= 0) atomicCAS(&d.body_claim[fa], -1, 2 * i);
@@ -628,7 +637,7 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
         for (int e = 0; e < n; ++e) s.list[off + e] = (unsigned short)(d.bc_list[d.bc_off[f] + e] - p0);
         d.body_loc[f] = make_int2(off, n);
         if (n > SMALL_BODY) {
-          const int b = atomicAdd(&s.nbody, 1);   // b < LCAP: every body holds at least one list entry
+          const int b = atomicAdd(&s.nbody, 1);   // such a body holds more than SMALL_BODY of the <= LCAP entries
           s.bl_off[b] = (unsigned short)off;
           s.bl_cnt[b] = (unsigned short)n;
         }
@@ -649,7 +658,7 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
         s.bo[i] = (unsigned short)lb.x; s.bcn[i] = (unsigned short)lb.y;
       }
       __syncthreads();
-      LdsAcc<ColorLds<NT, CAP, LCAP, PAL>> a{&s};
+      LdsAcc<ColorLds<NT, CAP, LCAP, PAL>> a{&s, d.c_lab, d.uperm, p0};
       const int pw = (2 * K + 2 + 31) / 32;
       if (PAL > 0 && K * pw <= PAL)
         color_rounds<NT>(d, a, K, s.pal, pw, seed, stepk, s.sh, &s.flag, &s.mx);
@@ -669,6 +678,9 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
   }
 }
 
+static_assert(sizeof(ColorLds<COLOR_UNIT_BLOCK, UNIT_CAP, LIST_CAP, 0>) <= 160 * 1024 - 2048,
+              "large-unit colouring state must fit a workgroup's LDS");
+
 // units of up to SMALL_UNIT contacts: one wavefront each
 template <typename R> __global__ void __launch_bounds__(64) k_color_small(Dev<R> d) {
   __shared__ ColorLds<64, SMALL_UNIT, SMALL_LIST, SMALL_PAL> s;
@@ -680,9 +692,9 @@ template <typename R> __global__ void __launch_bounds__(64) k_color_small(Dev<R>
   color_unit<R>(d, u, s);
 }
 
-// larger units: a workgroup each
-template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_color_units(Dev<R> d) {
-  __shared__ ColorLds<COLOR_BLOCK, UNIT_CAP, LIST_CAP, 0> s;
+// larger units: a workgroup of COLOR_UNIT_BLOCK threads each
+template <typename R> __global__ void __launch_bounds__(COLOR_UNIT_BLOCK) k_color_units(Dev<R> d) {
+  __shared__ ColorLds<COLOR_UNIT_BLOCK, UNIT_CAP, LIST_CAP, 0> s;
   if (aborted(d.err)) return;
   const int u = blockIdx.x;
   if (u >= *d.n_units) return;

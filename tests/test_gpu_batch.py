@@ -115,6 +115,38 @@ def test_tuning_rejects_bad_values():
         ctx.close()
 
 
+def test_large_unit_colouring_paths_bitwise(bank):
+    """A pile of 1024 spheres (one island of more than 2048 contacts) is
+    coloured the same with its colouring state in LDS and in global memory
+    (color_lds_contacts = 64), and steps to the same states."""
+    ctx = native.Context(0, "fp32")
+    try:
+        eng = scenes.serving_request(4321, bank, nmin=1024, nmax=1024, params=dict(precision="fp32"))
+        desc, keep = engine_desc(eng, ctx)
+        sid = ctx.scene_create(desc)
+        ctx.set_state(sid, *gather_state(list(eng.bodies.values())))
+        ctx.step([sid], 300, 0.0, 0)
+        ctx.sync()
+        start = ctx.get_state(sid, len(eng.bodies))
+        runs = []
+        for cap in (0, 64):
+            ctx.set_tuning(color_lds_contacts=cap)
+            ctx.set_state(sid, *start)
+            ctx.set_step_index(sid, 300)
+            snaps = []
+            for _ in range(3):
+                ctx.step([sid], 1, 0.0, 0)
+                c = ctx.get_contacts(sid)
+                snaps.append([c["color"], c["impulse"]] + list(ctx.get_state(sid, len(eng.bodies))))
+            runs.append(snaps)
+        assert len(runs[0][-1][0]) > 2048, len(runs[0][-1][0])
+        for a, b in zip(*runs):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+    finally:
+        ctx.close()
+
+
 def test_serving_alone_equals_mixed(bank):
     ctx = native.Context(0, "fp32")
     try:

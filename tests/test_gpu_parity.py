@@ -151,20 +151,30 @@ def sphere_row(bank, n=7, gap=0.98, sink=0.03, params=None):
     return eng
 
 
-def test_big_island_fp64(bank):
+@pytest.mark.parametrize("color_lds", [0, 64])
+def test_big_island_fp64(bank, color_lds):
     """An island larger than a one-wavefront colouring unit (palettes in global
-    memory) is coloured and solved exactly as by the oracle."""
-    eng = sphere_row(bank, params=dict(precision="fp64", max_iterations=30, proximal_solver="parallel_gauss_seidel"))
-    sc = OS.from_engine(eng)
-    exact = 0
-    for _ in range(2):
-        m = {}
-        out = OST.step(sc, 0.01, m)
-        c, state, lg = gpu_step(eng, 0.01)
-        assert len(c["ia"]) > 300, len(c["ia"])
-        exact += assert_fp64_parity(c, state, lg, out, sc, m)
-        resync(eng, sc)
-    assert exact >= 1
+    memory) is coloured and solved exactly as by the oracle, with the rest of
+    the colouring state in LDS (default) or, forced by color_lds_contacts = 64,
+    in global memory."""
+    from rainbow_amd.stepper import shared_context
+    ctx = shared_context(0, "fp64")
+    ctx.set_tuning(color_lds_contacts=color_lds)
+    try:
+        eng = sphere_row(bank, params=dict(precision="fp64", max_iterations=30,
+                                           proximal_solver="parallel_gauss_seidel"))
+        sc = OS.from_engine(eng)
+        exact = 0
+        for _ in range(2):
+            m = {}
+            out = OST.step(sc, 0.01, m)
+            c, state, lg = gpu_step(eng, 0.01)
+            assert len(c["ia"]) > 300, len(c["ia"])
+            exact += assert_fp64_parity(c, state, lg, out, sc, m)
+            resync(eng, sc)
+        assert exact >= 1
+    finally:
+        ctx.set_tuning(color_lds_contacts=0)
 
 
 @pytest.mark.parametrize("scheme", ["parallel_gauss_seidel", "gauss_seidel"])

@@ -121,7 +121,8 @@ struct rbh_ctx {
   long long batch_arena_gen = -1;
   int nb = 0, ns = 0;
   DBuf bb_body, bs_slot, bs_foff, b2f, f_end, d_nb;
-  DBuf row_cnt, row_off, pairs, pair_sz, slot_off, n_pairs, slot_flag, slot_scan, n_slots, pair_items, item_off;
+  DBuf row_cnt, row_off, pairs, pair_sz, slot_off, n_pairs, slot_flag, n_slots, pair_items, item_off;
+  DBuf side_cnt, side_off, n_sides;
   DBuf cand_pair, cand_tri, pair_cstart, n_cand, cand_p, cand_n, cand_keep, keep_scan;
   DBuf kidx, kpos, n_kept, kept_flag, kept_scan;
   DBuf c_ia, c_ib, c_tri, c_pair, c_p, c_n, n_contacts, c_rA, c_rB, c_b, c_r0, c_mu, c_imp, c_color;
@@ -197,7 +198,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi, &c->clo, &c->chi,
           &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
           &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs, &c->pair_items, &c->item_off,
-          &c->slot_flag, &c->slot_scan, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
+          &c->slot_flag, &c->side_cnt, &c->side_off, &c->n_sides, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
           &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->kidx, &c->kpos,
           &c->n_kept, &c->kept_flag, &c->kept_scan, &c->c_ia, &c->c_ib, &c->c_tri,
           &c->c_pair, &c->c_p, &c->c_n, &c->n_contacts, &c->c_rA, &c->c_rB, &c->c_b, &c->c_r0, &c->c_mu,
@@ -437,7 +438,8 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.pairs = P(int2*, pairs); d.pair_sz = P(int*, pair_sz); d.slot_off = P(int*, slot_off);
   d.pair_items = P(int*, pair_items); d.item_off = P(int*, item_off);
   d.n_pairs = P(int*, n_pairs); d.cap_pairs = c->cap_pairs;
-  d.slot_flag = P(unsigned char*, slot_flag); d.slot_scan = P(int*, slot_scan); d.n_slots = P(int*, n_slots);
+  d.slot_flag = P(unsigned char*, slot_flag); d.n_slots = P(int*, n_slots);
+  d.side_cnt = P(int*, side_cnt); d.side_off = P(int*, side_off); d.n_sides = P(int*, n_sides);
   d.cap_slots = c->cap_slots;
   d.cand_pair = P(int*, cand_pair); d.cand_tri = P(int*, cand_tri); d.pair_cstart = P(int*, pair_cstart);
   d.n_cand = P(int*, n_cand); d.cap_cand = c->cap_cand;
@@ -494,12 +496,14 @@ int size_workspace(rbh_ctx* c) {
   const size_t nb = std::max(c->nb, 1), ns = std::max(c->ns, 1);
   c->cap_units = (int)std::min<long long>((long long)(c->cap_contacts + c->unit_T - 1) / c->unit_T + c->ns + 1, 1 << 20);
   const size_t nu = c->cap_units, nl = (size_t)c->nb + c->ns + 2;
-  int maxcap = std::max({c->cap_pairs, c->cap_slots, c->cap_cand, c->cap_contacts, c->nb + c->ns + 2, c->ns, c->cap_units});
+  int maxcap = std::max({2 * c->cap_pairs + 2, c->cap_slots, c->cap_cand, c->cap_contacts, c->nb + c->ns + 2, c->ns,
+                         c->cap_units});
   struct E { DBuf* b; size_t bytes; } el[] = {
       {&c->row_cnt, nb * 4}, {&c->row_off, (nb + 1) * 4},
       {&c->pairs, cp * 8}, {&c->pair_sz, cp * 4}, {&c->slot_off, (cp + 1) * 4}, {&c->n_pairs, 16},
       {&c->pair_items, cp * 4}, {&c->item_off, (cp + 1) * 4},
-      {&c->slot_flag, cs + 16}, {&c->slot_scan, (cs + 1) * 4}, {&c->n_slots, 16},
+      {&c->slot_flag, cs + 16}, {&c->n_slots, 16},
+      {&c->side_cnt, (2 * cp + 2) * 4}, {&c->side_off, (2 * cp + 2) * 4}, {&c->n_sides, 16},
       {&c->cand_pair, cc * 4}, {&c->cand_tri, cc * 4}, {&c->pair_cstart, (cp + 1) * 4}, {&c->n_cand, 16},
       {&c->cand_p, cc * V}, {&c->cand_n, cc * V}, {&c->cand_keep, cc + 16}, {&c->keep_scan, (cc + 1) * 4},
       {&c->kidx, cc * 4}, {&c->kpos, cc * V}, {&c->n_kept, 16}, {&c->kept_flag, cc + 16}, {&c->kept_scan, (cc + 1) * 4},
@@ -650,8 +654,8 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   scan<int>(c, d.pair_sz, d.slot_off, d.n_pairs, c->cap_pairs, d.n_slots);
   scan<int>(c, d.pair_items, d.item_off, d.n_pairs, c->cap_pairs, nullptr);
   hipLaunchKernelGGL(k_cand_flags<R>, dim3(grid_for(c->cap_pairs, 1, 16384)), dim3(CAND_BLOCK), 0, s, d);
-  scan<unsigned char>(c, d.slot_flag, d.slot_scan, d.n_slots, c->cap_slots, nullptr);
-  hipLaunchKernelGGL(k_cand_write<R>, dim3(grid_for(c->cap_slots, 256)), dim3(256), 0, s, d);
+  scan<int>(c, d.side_cnt, d.side_off, d.n_sides, 2 * c->cap_pairs, d.n_cand);
+  hipLaunchKernelGGL(k_cand_write<R>, dim3(grid_for(2L * c->cap_pairs * 64, 256)), dim3(256), 0, s, d);
   mark(ST_BROAD);
   hipLaunchKernelGGL(k_frank_wolfe<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
   scan<unsigned char>(c, d.cand_keep, d.keep_scan, d.n_cand, c->cap_cand, nullptr);

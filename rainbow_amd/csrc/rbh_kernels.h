@@ -82,7 +82,8 @@ template <typename R> struct Dev {
   int* row_cnt; int* row_off;          // [nb], [nb+1]
   int2* pairs; int* pair_sz; int* slot_off; int* n_pairs; int cap_pairs;
   int* pair_items; int* item_off;      // candidate work items (triangle blocks of both sides) per pair, scan
-  unsigned char* slot_flag; int* slot_scan; int* n_slots; int cap_slots;
+  unsigned char* slot_flag; int* n_slots; int cap_slots;
+  int* side_cnt; int* side_off; int* n_sides;   // candidates per (pair, side) 2 p + side, their scan, 2 np
   int* cand_pair; int* cand_tri; int* pair_cstart; int* n_cand; int cap_cand;
   V4<R>* cand_p; V4<R>* cand_n; unsigned char* cand_keep; int* keep_scan;
   int* kidx; V4<R>* kpos; int* n_kept;  // candidates kept by Frank-Wolfe, compacted (candidate order)
@@ -315,6 +316,7 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
   if (threadIdx.x == 0 && f == d.nb - 1) {
     int np = d.row_off[d.nb];
     if (np > d.cap_pairs) { atomicOr(d.err, ERR_CAP_PAIRS); atomicMax(d.err + 1, np); }
+    *d.n_sides = 2 * np;
   }
   if (d.row_off[d.nb] > d.cap_pairs) return;
   V4<R> alo = d.b_lo[b], ahi = d.b_hi[b];
@@ -329,6 +331,8 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
       d.pairs[pos] = make_int2(b, j);
       d.pair_sz[pos] = ntA + ntB;
       d.pair_items[pos] = n_blocks(ntA) + n_blocks(ntB);
+      d.side_cnt[2 * pos] = 0;
+      d.side_cnt[2 * pos + 1] = 0;
     }
     base += __popcll(m);
   }
@@ -512,8 +516,15 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
     const int blk = side0 ? k : k - n_blocks(ntA);
     const int x0 = blk * CAND_BLK, x1 = min(x0 + CAND_BLK, side0 ? ntA : ntB);
     unsigned long long ov = 0;
-    cand_block(d, X, Y, d.slot_off[p] + (side0 ? 0 : ntA), x0, x1, count_all && side0, lo_s[wave], hi_s[wave],
+    const int slot_base = d.slot_off[p] + (side0 ? 0 : ntA);
+    cand_block(d, X, Y, slot_base, x0, x1, count_all && side0, lo_s[wave], hi_s[wave],
                clo_s[wave], chi_s[wave], crange_s[wave], ov);
+    // candidates of this block (each lane reads back the flags it wrote last)
+    const int tXo = d.shapes[d.b_shape[X]].t_off;
+    int nset = 0;
+    for (int pp = x0 + (threadIdx.x & 63); pp < x1; pp += 64) nset += d.slot_flag[slot_base + d.sT[tXo + pp].w] ? 1 : 0;
+    nset = wave_sum(nset);
+    if ((threadIdx.x & 63) == 0 && nset) atomicAdd(&d.side_cnt[2 * p + (side0 ? 0 : 1)], nset);
     if (count_all && side0) {
       const unsigned long long tot = wave_sum_u64(ov);
       if ((threadIdx.x & 63) == 0 && tot) atomicAdd(&d.ss[d.b_slot[ab.x]].n_overlaps, tot);
@@ -521,35 +532,38 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
   }
 }
 
-// slot s -> (pair, direction, triangle) by binary search over slot_off
-template <typename R> __global__ void k_cand_write(Dev<R> d) {
+// Candidates in slot order (pair, side, triangle index): one wavefront per
+// (pair, side) compacts that side's flags from the offset the scan of the
+// per-side counts gives it.
+template <typename R> __global__ void __launch_bounds__(256) k_cand_write(Dev<R> d) {
   const int np = *d.n_pairs;
   if (np > d.cap_pairs || aborted(d.err)) return;
-  const int ns = d.slot_off[np];
-  const int nc = d.slot_scan[ns];
+  const int nc = *d.n_cand;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *d.n_cand = nc;
     if (nc > d.cap_cand) { atomicOr(d.err, ERR_CAP_CAND); atomicMax(d.err + 3, nc); }
     d.pair_cstart[np] = nc;
   }
   if (nc > d.cap_cand) return;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
-    int lo = 0, hi = np - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (d.slot_off[mid] <= s) lo = mid; else hi = mid - 1;
-    }
-    int p = lo;
-    int local = s - d.slot_off[p];
-    if (local == 0) d.pair_cstart[p] = d.slot_scan[s];
-    if (d.slot_flag[s]) {
-      int c = d.slot_scan[s];
-      int2 ab = d.pairs[p];
-      int ntA = d.shapes[d.b_shape[ab.x]].nt;
-      int dir = local >= ntA ? 1 : 0;
-      int tri = dir ? local - ntA : local;
-      d.cand_pair[c] = p;
-      d.cand_tri[c] = tri | (dir << 31);
+  const int lane = threadIdx.x & 63;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int sd = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; sd < 2 * np; sd += nw) {
+    const int p = sd >> 1, side = sd & 1;
+    const int2 ab = d.pairs[p];
+    const int ntA = d.shapes[d.b_shape[ab.x]].nt;
+    const int nt = side ? d.shapes[d.b_shape[ab.y]].nt : ntA;
+    const int base = d.slot_off[p] + (side ? ntA : 0);
+    int off = d.side_off[sd];
+    if (side == 0 && lane == 0) d.pair_cstart[p] = off;
+    for (int t0 = 0; t0 < nt; t0 += 64) {
+      const int t = t0 + lane;
+      const bool f = t < nt && d.slot_flag[base + t];
+      const unsigned long long m = __ballot(f);
+      if (f) {
+        const int c = off + __popcll(m & ((1ull << lane) - 1ull));
+        d.cand_pair[c] = p;
+        d.cand_tri[c] = t | (side << 31);
+      }
+      off += __popcll(m);
     }
   }
 }

@@ -84,14 +84,19 @@ def test_chain_fp64(bank, tol):
         assert stops & {1, 2}, stops
 
 
-@pytest.mark.parametrize("scheme,flags,pre", [
-    ("parallel_gauss_seidel", native.STEP_FORCE_MULTI, False),
-    ("parallel_gauss_seidel", native.STEP_FORCE_SINGLE, True),
-    ("gauss_seidel", 0, False),
+@pytest.mark.parametrize("scheme,flags,pre,post", [
+    ("parallel_gauss_seidel", native.STEP_FORCE_MULTI, False, False),
+    ("parallel_gauss_seidel", native.STEP_FORCE_SINGLE, True, False),
+    ("gauss_seidel", 0, False, False),
+    ("parallel_gauss_seidel", native.STEP_FORCE_SINGLE, False, True),   # post-stabilisation: contacts only
 ])
-def test_hinged_pair_on_ground_fp64(bank, scheme, flags, pre):
+def test_hinged_pair_on_ground_fp64(bank, scheme, flags, pre, post):
     eng = hinged_pair(bank, dict(precision="fp64", proximal_solver=scheme, use_pre_stabilization=pre,
-                                 absolute_tolerance=1e-7, relative_tolerance=1e-4))
+                                 use_post_stabilization=post, absolute_tolerance=1e-7, relative_tolerance=1e-4))
+    if post:
+        for b in eng.bodies.values():
+            if not b.is_fixed:
+                b.r = b.r - np.array([0.0, 0.03, 0.0])    # sunk below min_gap_value: the post solve runs
     sc = OS.from_engine(eng)
     seen = 0
     for _ in range(4):
@@ -107,6 +112,9 @@ def test_hinged_pair_on_ground_fp64(bank, scheme, flags, pre):
         eng.step_index += 1
         assert out["contacts"].K > 0
         assert_fp64_parity(c, (r, q, v, w), lg, out, sc)
+        if post and "post" in out:
+            assert lg["post_iterations"] == out["log"]["post_stabilization_iterations"]
+            assert lg["post_exitcode"] == out["log"]["post_stabilization_exitcode"]
         seen += 1
         resync(eng, sc)
     assert seen == 4

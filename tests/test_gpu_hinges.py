@@ -89,6 +89,8 @@ def test_chain_fp64(bank, tol):
     ("parallel_gauss_seidel", native.STEP_FORCE_SINGLE, True, False),
     ("gauss_seidel", 0, False, False),
     ("parallel_gauss_seidel", native.STEP_FORCE_SINGLE, False, True),   # post-stabilisation: contacts only
+    ("parallel_jacobi", 0, False, False),          # Jacobi contacts beside the serial hinge sweep
+    ("parallel_jacobi_hybrid", 0, True, False),
 ])
 def test_hinged_pair_on_ground_fp64(bank, scheme, flags, pre, post):
     eng = hinged_pair(bank, dict(precision="fp64", proximal_solver=scheme, use_pre_stabilization=pre,

@@ -115,3 +115,10 @@ def test_axis_sort_permutes_consistently():
 def test_substeps_match_reference_loop():
     assert API.substeps(0.025, 0.01) == pytest.approx([0.01, 0.01, 0.005])
     assert len(API.substeps(1.0, 0.005)) >= 200
+
+
+def test_smoke_scene_has_contacts():
+    """The scene smoke() steps on the GPU gives the oracle contacts to compare."""
+    import __graft_entry__ as G
+    eng, ref, out = G.smoke_scene()
+    assert out["contacts"].K > 0

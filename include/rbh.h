@@ -168,7 +168,7 @@ int rbh_scene_set_params(rbh_ctx* ctx, int scene_id, const rbh_params* params);
  * indices, arm_p / arm_c (3n) the joint origin and axis_p / axis_c (3n) the
  * joint axis in the parent's and the child's body frames.  Each step then
  * solves the hinge problem of constraints/hinges.py:16-218 together with the
- * contacts (gauss_seidel_solver.py:14-101).  A scene holds at most 1536
+ * contacts (gauss_seidel_solver.py:14-101).  A scene holds at most 1024
  * hinges joining at most 1024 bodies; a hinge needs two distinct bodies. */
 int rbh_scene_set_hinges(rbh_ctx* ctx, int scene_id, int n, const int* parent, const int* child, const double* arm_p,
                          const double* arm_c, const double* axis_p, const double* axis_c);

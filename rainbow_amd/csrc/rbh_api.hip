@@ -105,7 +105,7 @@ struct rbh_ctx {
   DBuf W0, W1, du0, du1, lo, hi, ws0, ws1, ww0, ww1, hd0, hd1;
   DBuf tlo, thi, clo, chi;
   // hinges of all live scenes, packed by scene slot (rebuilt when they change)
-  DBuf h_ids, h_geo, hb_body, hb_inc_off, hb_inc, h_J, h_WJ, h_b, h_r0, h_sol, d_hmer;
+  DBuf h_ids, h_geo, hb_body, hb_inc_off, hb_inc, h_J, h_WJ, h_b, h_r0, d_hmer;
   bool hinges_dirty = false;
   bool any_hinge = false;   // a scene of the batch has hinges (set per run_steps call)
   size_t hinge_lds = 0;     // dynamic LDS of the hinge solve for the batch (its largest scene)
@@ -210,7 +210,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->u_nbody, &c->u_boff, &c->f_loc, &c->f_unit, &c->u_blist, &c->s_loc, &c->s_W, &c->l_cnt,
           &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs, &c->s_JL, &c->s_JA, &c->s_JB,
           &c->xr0, &c->xr1, &c->hd0, &c->hd1, &c->h_ids, &c->h_geo, &c->hb_body, &c->hb_inc_off, &c->hb_inc,
-          &c->h_J, &c->h_WJ, &c->h_b, &c->h_r0, &c->h_sol, &c->d_hmer};
+          &c->h_J, &c->h_WJ, &c->h_b, &c->h_r0, &c->d_hmer};
 }
 
 size_t total_bytes(rbh_ctx* c) {
@@ -390,7 +390,6 @@ template <typename R> int sync_hinges(rbh_ctx* c) {
   if (!rc) rc = ensure(c, c->h_WJ, 20 * nh * V);
   if (!rc) rc = ensure(c, c->h_b, 5 * nh * c->prec);
   if (!rc) rc = ensure(c, c->h_r0, 5 * nh * c->prec);
-  if (!rc) rc = ensure(c, c->h_sol, 5 * nh * c->prec);
   if (!rc) rc = ensure(c, c->d_hmer, (size_t)std::max(c->res.cap, 1) * 8, true);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -479,7 +478,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.h_ids = P(const int4*, h_ids); d.h_geo = P(const V4<R>*, h_geo); d.hb_body = P(const int*, hb_body);
   d.hb_inc_off = P(const int*, hb_inc_off); d.hb_inc = P(const int*, hb_inc);
   d.h_J = P(V4<R>*, h_J); d.h_WJ = P(V4<R>*, h_WJ); d.h_b = P(R*, h_b); d.h_r0 = P(R*, h_r0);
-  d.h_sol = P(R*, h_sol); d.h_mer = P(double*, d_hmer);
+  d.h_mer = P(double*, d_hmer);
   d.hd0 = P(V4<R>*, hd0); d.hd1 = P(V4<R>*, hd1);
   d.err = P(int*, errbuf);
   d.dt = dt;

@@ -24,7 +24,7 @@
 #pragma once
 
 constexpr int HINGE_BODY_MAX = 1024;   // bodies joined by hinges in one scene (LDS: two V4 each)
-constexpr int HINGE_MAX = 1536;        // hinges of one scene (LDS: 5 iterates each; FP64 total 124 KB)
+constexpr int HINGE_MAX = 1024;        // hinges of one scene (LDS: 5 iterates and 5 accepted values each; FP64 total 144 KB)
 
 // One J row (parent lin, parent ang, child lin, child ang) times body-space
 // vectors, term by term in the csr column order of hinges.py:184 (the body of
@@ -104,7 +104,6 @@ template <typename R> __global__ void __launch_bounds__(64) k_hinge_rows(Dev<R> 
       }
       d.h_b[5 * h + i] = (ju + jd) + g[i];
       d.h_r0[5 * h + i] = R(1) / (dg == R(0) ? R(1) : dg);
-      d.h_sol[5 * h + i] = R(0);
     }
   }
 }
@@ -197,15 +196,15 @@ __device__ __forceinline__ void hinge_update(V4<R>* wl, V4<R>* wa, R* x, R* s_de
 }
 
 // Bytes of dynamic LDS the hinge solve of a scene needs: w (two V4 per hinge
-// body) and the iterate x (5 per hinge).
+// body), the iterate x and the accepted solution sol (5 each per hinge).
 template <typename R> __host__ __device__ constexpr size_t hinge_lds_bytes(int hb, int nh) {
-  return (2 * (size_t)hb * sizeof(V4<R>) + 5 * (size_t)nh * sizeof(R) + 15) & ~(size_t)15;
+  return (2 * (size_t)hb * sizeof(V4<R>) + 10 * (size_t)nh * sizeof(R) + 15) & ~(size_t)15;
 }
 
 // One wavefront per batch scene.  pass 0 records the hinge merit
 // contributions for the contact solve; pass 1 runs to the joint stop and
-// writes hd0/hd1 (see the top of this file).  w and x live in LDS; the
-// accepted solution sol in h_sol.
+// writes hd0/hd1 (see the top of this file).  w, x and the accepted
+// solution sol live in LDS.
 template <typename R> __global__ void __launch_bounds__(64) k_hinge_solve(Dev<R> d, int pass) {
   extern __shared__ __align__(16) unsigned char hinge_smem[];
   __shared__ R s_delta[5];
@@ -223,7 +222,7 @@ template <typename R> __global__ void __launch_bounds__(64) k_hinge_solve(Dev<R>
   const int lane = threadIdx.x;
   const int h0 = sp.h_off, h1 = sp.h_off + sp.h_cnt, nrow = 5 * sp.h_cnt;
   R* x = reinterpret_cast<R*>(wa + sp.hb_cnt);   // row 5 (h - h0) + i
-  R* sol = d.h_sol + 5 * h0;
+  R* sol = x + nrow;
   double* res = d.residuals + sp.res_off;
   double* hm = d.h_mer + sp.res_off;
   for (int i = lane; i < nrow; i += 64) { x[i] = R(0); sol[i] = R(0); }

@@ -129,7 +129,7 @@ template <typename R> struct Dev {
   const int* hb_body;                  // hinge-body slot -> arena body
   const int* hb_inc_off; const int* hb_inc;   // hinge-body slot -> 2 h + side (0 parent, 1 child), hinge order
   V4<R>* h_J; V4<R>* h_WJ;             // 20 per hinge: row i at 4 i: parent lin, parent ang, child lin, child ang
-  R* h_b; R* h_r0; R* h_sol;           // 5 per hinge
+  R* h_b; R* h_r0;                     // 5 per hinge
   double* h_mer;                       // per sweep: the hinge problem's merit contribution (at res_off)
   V4<R>* hd0; V4<R>* hd1;              // per arena body: hinge W J^T sol (linear, angular)
   int* err;                           // [0] flags, [1..] needed capacities

@@ -160,7 +160,7 @@ def test_set_hinges_rejects_bad_input(bank):
             ctx.set_hinges(sid, [1], [1], z, z, z, z)          # parent and child are one body
         with pytest.raises(native.RbhError):
             ctx.set_hinges(sid, [0], [7], z, z, z, z)          # no body 7 in a 3-body scene
-        n = 1537
+        n = 1025
         zz = np.zeros((n, 3))
         with pytest.raises(native.RbhError):
             ctx.set_hinges(sid, [0] * n, [1] * n, zz, zz, zz, zz)   # more hinges than a scene holds

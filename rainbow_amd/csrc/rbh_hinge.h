@@ -256,7 +256,7 @@ template <typename R> __global__ void __launch_bounds__(64) k_hinge_solve(Dev<R>
     if (own) {
       dec = ctl.decide(sp, m, lane == 0 ? res : nullptr, it);
     } else {
-      dec = ctl.judge(sp, m);
+      dec = ctl.accept_or_reject(sp, m);
       if (pass == 0 && lane == 0) hm[it] = ctl.gm;
     }
     for (int i = lane; i < nrow; i += 64) {

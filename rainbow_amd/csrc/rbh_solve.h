@@ -974,7 +974,7 @@ struct SolveCtl {
   int acc = 0, rejects = 0, exitcode = 0, iters = 0, sweeps = 0;
   // accept (bit0) or reject, r-factor update; gm = this problem's contribution
   // to the global merit (its merit, or its last accepted merit on a reject)
-  __device__ int judge(const SceneParams& sp, double merit) {
+  __device__ int accept_or_reject(const SceneParams& sp, double merit) {
     int dec = 0;
     if (merit > last_merit) {
       rscale *= sp.nu_reduce;
@@ -991,7 +991,7 @@ struct SolveCtl {
   // returns bit0 accept, bit1 stop; `other` is the contribution of the
   // scene's other problem (the hinges) to the global merit of this sweep
   __device__ int decide(const SceneParams& sp, double merit, double* res, int it, double other = 0.0) {
-    int dec = judge(sp, merit);
+    int dec = accept_or_reject(sp, merit);
     gm = fmax(gm, other);
     if (res) res[it] = gm;
     if (gm < sp.abs_tol) { dec |= 2; exitcode = 1; }

@@ -17,9 +17,10 @@
 //          W J^T sol per body for the velocity update.
 // A hinge sweep is serial over the hinges (Gauss-Seidel); inside one hinge
 // the five rows update from the same w (hinges.py:88), one lane per row, and
-// the twelve body entries of W J^T delta one lane each.  w and the iterate
-// stay in LDS; each hinge's rows are loaded while the previous hinge
-// updates, so a hinge costs two wave barriers and LDS traffic.
+// the twelve body entries of W J^T delta one lane each.  w, the iterate and
+// the accepted solution stay in LDS; the rows of the next HINGE_AHEAD hinges
+// are loaded while one updates, so a hinge costs two wave syncs and LDS
+// traffic.
 // (included inside namespace rbh by rbh_kernels.h)
 #pragma once
 
@@ -139,7 +140,7 @@ __device__ __forceinline__ double wave_max_d(double v) {
 
 // Per-hinge constants a lane needs for one Gauss-Seidel update: lanes 0-4
 // hold row `lane` (J, b, r0), lanes 0-11 the W J^T entries of body entry
-// `lane` for rows 0-4.  Loaded one hinge ahead of their use.
+// `lane` for rows 0-4.  Loaded HINGE_AHEAD hinges ahead of their use.
 template <typename R> struct HingeRegs {
   int4 id;
   V4<R> J[4];

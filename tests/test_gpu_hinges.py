@@ -149,6 +149,19 @@ def test_reference_chain_fp32(bank):
     assert np.abs(got - sc.r).max() < 0.05, np.abs(got - sc.r).max()
 
 
+def test_hinge_only_profile_keys(bank):
+    """steppers.py:78-83 and gauss_seidel_solver.py:75-101: a step with hinges
+    and no contacts still logs the solve (residuals, iterations, exitcode)."""
+    eng = scenes.cube_hinge_chain(bank, n=3, edge=1.0, params=dict(precision="fp32", max_iterations=30))
+    API.simulate(eng, 0.02, profiling_on=True)
+    lg = API.get_log(eng)
+    assert len(lg) == 2
+    for d in lg:
+        assert d["contact_points"] == 0
+        assert {"residuals", "iterations", "exitcode"} <= set(d)
+        assert len(d["residuals"]) == 30 and np.isfinite(d["residuals"][0])
+
+
 def test_set_hinges_rejects_bad_input(bank):
     eng = scenes.cube_hinge_chain(bank, n=3, edge=1.0, params=dict(precision="fp32"))
     ctx = native.Context(0, "fp32")

@@ -68,6 +68,7 @@ class Server:
         self.budget = int(budget)
         self.graph = bool(graph)
         self.max_batch = max_batch
+        self._active = {}          # during run(): scene id -> [request, steps taken]
         if tuning:
             self.ctx.set_tuning(**tuning)
 
@@ -80,24 +81,33 @@ class Server:
         return sid
 
     def checkpoint(self, req, xml_filename):
-        """Write a finished request's scene, at its final state and step count,
-        as a lossless scene file; ``xml_io.read_xml`` of it is a request that
-        continues bitwise where this one ended."""
-        if req.state is None:
-            raise RuntimeError("checkpoint(): request %d has not finished" % req.rid)
+        """Write a request's scene as a lossless scene file: a finished request
+        at its final state, or, between ticks of ``run`` (from ``on_tick``), an
+        admitted one at its current state.  ``xml_io.read_xml`` of the file is
+        a scene that continues bitwise from that state (its step count is in
+        the file); returns the number of steps the request had taken."""
+        done = None
+        for sid, (r_, n_) in self._active.items():
+            if r_ is req:
+                state, done = self.ctx.get_state(sid, len(req.engine.bodies)), n_
+        if done is None:
+            if req.state is None:
+                raise RuntimeError("checkpoint(): request %d is neither running nor finished" % req.rid)
+            state, done = req.state, req.n_steps
         eng = req.engine
-        r, q, v, w = req.state
+        r, q, v, w = state
         saved = [(b.r, b.q, b.v, b.w) for b in eng.bodies.values()]
         step0 = getattr(eng, "step_index", 0)
         try:
             for k, b in enumerate(eng.bodies.values()):
                 b.r, b.q, b.v, b.w = r[k].copy(), q[k].copy(), v[k].copy(), w[k].copy()
-            eng.step_index = step0 + req.n_steps
+            eng.step_index = step0 + done
             xml_io.write_xml(eng, xml_filename, lossless=True)
         finally:
             for b, (r0, q0, v0, w0) in zip(eng.bodies.values(), saved):
                 b.r, b.q, b.v, b.w = r0, q0, v0, w0
             eng.step_index = step0
+        return done
 
     def prepare(self, req):
         """Build the device description of a request's scene (host work done ahead of arrival)."""
@@ -114,6 +124,7 @@ class Server:
         """Serve `requests` (prepared, sorted or not) until all finish; returns a report dict."""
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting, active = [], {}       # active: sid -> [request, steps done]
+        self._active = active
         used = 0
         lat = []                        # per-step latency samples (s), one per scene per tick
         ticks = 0
@@ -144,13 +155,13 @@ class Server:
             dt_tick = clock() - ts
             ticks += 1
             lat.extend([dt_tick] * len(ids))
-            if on_tick:
-                on_tick(ticks, len(ids), dt_tick)
             done = []
             for sid in ids:
                 active[sid][1] += 1
                 if active[sid][1] >= active[sid][0].n_steps:
                     done.append(sid)
+            if on_tick:   # every admitted request's step count includes this tick
+                on_tick(ticks, len(ids), dt_tick)
             for sid in done:
                 req = active.pop(sid)[0]
                 n = len(req.engine.bodies)
@@ -159,6 +170,7 @@ class Server:
                 self.ctx.scene_destroy(sid)
                 used -= req.footprint
         wall = clock() - t0
+        self._active = {}
         steps = sum(r.n_steps for r in requests)
         waits = [r.admitted - r.arrival for r in requests]
         return dict(requests=len(requests), scene_steps=steps, wall_s=wall, scene_steps_per_s=steps / wall,

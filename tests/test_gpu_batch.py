@@ -115,6 +115,33 @@ def test_tuning_rejects_bad_values():
         ctx.close()
 
 
+def test_serving_snapshot_of_a_running_request(bank, tmp_path):
+    """A request snapshot between ticks (from on_tick) and read back finishes
+    its remaining steps bitwise like the request left running."""
+    from rainbow_amd import xml_io
+    ctx = native.Context(0, "fp32")
+    try:
+        srv = serving.Server(ctx=ctx, budget=1 << 30)
+        a = scenes.serving_request(4400, bank, nmax=40, params=dict(precision="fp32"))
+        b = scenes.serving_request(4401, bank, nmax=40, params=dict(precision="fp32"))
+        reqs = [srv.prepare(serving.Request(0, a, 14, 0.0)), srv.prepare(serving.Request(1, b, 9, 0.0))]
+        fn = str(tmp_path / "mid.xml")
+        taken = {}
+
+        def on_tick(tick, n, dt):
+            if tick == 5:
+                taken["n"] = srv.checkpoint(reqs[0], fn)
+
+        srv.run(reqs, on_tick=on_tick)
+        assert taken["n"] == 5
+        rest = srv.prepare(serving.Request(2, xml_io.read_xml(fn), 14 - 5, 0.0))
+        srv.run([rest])
+        for x, y in zip(rest.state, reqs[0].state):
+            assert np.array_equal(x, y)
+    finally:
+        ctx.close()
+
+
 def test_large_unit_colouring_paths_bitwise(bank):
     """A pile of 1024 spheres (one island of more than 2048 contacts) is
     coloured the same with its colouring state in LDS and in global memory

@@ -456,11 +456,23 @@ __device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0,
       for (int k = 0; k < ncl && !(flag && !count_all); ++k) {
         if (!box_overlap(lo, hi, clo_s[k], chi_s[k])) continue;
         const int2 rg = crange_s[k];
-        for (int e = rg.x; e < rg.x + rg.y; ++e) {
-          if (box_overlap(lo, hi, lo_s[e], hi_s[e])) {
+        const int end = rg.x + rg.y;
+        // four staged boxes per pass, loaded together (indexes clamped into
+        // the cluster's run; a clamped repeat is not tested)
+        for (int e = rg.x; e < end; e += 4) {
+          V4<R> sl[4], sh[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int ej = min(e + j, end - 1);
+            sl[j] = lo_s[ej]; sh[j] = hi_s[ej];
+          }
+          unsigned hit = 0;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) hit |= (box_overlap(lo, hi, sl[j], sh[j]) & (e + j < end)) ? 1u << j : 0u;
+          if (hit) {
             flag = 1;
             if (!count_all) break;
-            ++cnt;
+            cnt += __popc(hit);
           }
         }
       }

@@ -101,6 +101,20 @@ template <typename R> __device__ __forceinline__ V3<R> sym_mul(const R W[6], V3<
                W[4] * a.x + W[5] * a.y + W[2] * a.z};
 }
 
+// c + a * b for the contact sweep: one fused multiply-add in FP32; in FP64
+// the product and the sum are rounded one by one, as the FP64 oracle does
+template <typename R> __device__ __forceinline__ R madd(R a, R b, R c) {
+  if constexpr (sizeof(R) == 4) return __builtin_fmaf(a, b, c);
+  else return c + a * b;
+}
+
+// sym_mul with madd: the same sums in the same order
+template <typename R> __device__ __forceinline__ V3<R> sym_madd(const R W[6], V3<R> a) {
+  return V3<R>{madd(W[4], a.z, madd(W[3], a.y, W[0] * a.x)),
+               madd(W[5], a.z, madd(W[1], a.y, W[3] * a.x)),
+               madd(W[2], a.z, madd(W[5], a.y, W[4] * a.x))};
+}
+
 // vector3.py:75-113: (t, b, n) with t guessed from the axis of smallest |n_i|
 // (the later axis wins ties) and b = unit(n x t), t = b x n.
 template <typename R> __device__ __forceinline__ void orthonormal_frame(V3<R> nin, V3<R>& t, V3<R>& b, V3<R>& n) {

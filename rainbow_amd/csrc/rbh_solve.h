@@ -906,18 +906,38 @@ template <typename R> __host__ __device__ constexpr int lds_body_cap() {
   return SOLVE_LDS_BYTES / (4 * (int)sizeof(V4<R>));
 }
 
+// One row of J w (lin, angA, angB: the row's blocks).  FP64 keeps the
+// oracle's sum of four dot products; FP32 sums side A and side B in two
+// chains of fused multiply-adds.
+template <typename R>
+__device__ __forceinline__ R row_jw(V3<R> lin, V3<R> angA, V3<R> angB, V3<R> wlA, V3<R> waA, V3<R> wlB, V3<R> waB) {
+  if constexpr (sizeof(R) == 8) {
+    return -dot(lin, wlA) - dot(angA, waA) + dot(lin, wlB) + dot(angB, waB);
+  } else {
+    R a = lin.x * wlA.x;
+    a = madd(lin.y, wlA.y, a); a = madd(lin.z, wlA.z, a);
+    a = madd(angA.x, waA.x, a); a = madd(angA.y, waA.y, a); a = madd(angA.z, waA.z, a);
+    R b = lin.x * wlB.x;
+    b = madd(lin.y, wlB.y, b); b = madd(lin.z, wlB.z, b);
+    b = madd(angB.x, waB.x, b); b = madd(angB.y, waB.y, b); b = madd(angB.z, waB.z, b);
+    return b - a;
+  }
+}
+
+// z = x - r (J w + b) for one row: xo - r (Jw + b)
+template <typename R> __device__ __forceinline__ R row_z(R xo, R r, R jw, R b) { return madd(-r, jw + b, xo); }
+
 // contacts.py:77-95: z = x - r (J w + b); x_n = max(0, z_n); friction prox
 template <typename R>
 __device__ __forceinline__ V4<R> contact_prox(const SceneParams& sp, const ContactRow<R>& J, V4<R> xo, V4<R> bb,
                                               V4<R> rr, V4<R> mu, V3<R> wlA, V3<R> waA, V3<R> wlB, V3<R> waB,
                                               bool use_old_xn) {
   R Jw[4];
-  for (int i = 0; i < 4; ++i)
-    Jw[i] = -dot(J.lin[i], wlA) - dot(J.angA[i], waA) + dot(J.lin[i], wlB) + dot(J.angB[i], waB);
-  R z0 = xo.x - rr.x * (Jw[0] + bb.x);
-  R z1 = xo.y - rr.y * (Jw[1] + bb.y);
-  R z2 = xo.z - rr.z * (Jw[2] + bb.z);
-  R z3 = xo.w - rr.w * (Jw[3] + bb.w);
+  for (int i = 0; i < 4; ++i) Jw[i] = row_jw(J.lin[i], J.angA[i], J.angB[i], wlA, waA, wlB, waB);
+  R z0 = row_z(xo.x, rr.x, Jw[0], bb.x);
+  R z1 = row_z(xo.y, rr.y, Jw[1], bb.y);
+  R z2 = row_z(xo.z, rr.z, Jw[2], bb.z);
+  R z3 = row_z(xo.w, rr.w, Jw[3], bb.w);
   R xn = fmax(R(0), z0);
   R xr = use_old_xn ? xo.x : xn;
   R os, ot, orr;
@@ -944,11 +964,11 @@ __device__ __forceinline__ void dw_body(V4<R> W0, V4<R> W1, R sign, const Contac
   V3<R> a = {sum_first(s0 * ang[0].x), sum_first(s0 * ang[0].y), sum_first(s0 * ang[0].z)};
   for (int i = 1; i < 4; ++i) {
     const R s = sign * dd[i];
-    dl[0] += J.lin[i].x * s; dl[1] += J.lin[i].y * s; dl[2] += J.lin[i].z * s;
-    a = a + s * ang[i];
+    dl[0] = madd(J.lin[i].x, s, dl[0]); dl[1] = madd(J.lin[i].y, s, dl[1]); dl[2] = madd(J.lin[i].z, s, dl[2]);
+    a = V3<R>{madd(s, ang[i].x, a.x), madd(s, ang[i].y, a.y), madd(s, ang[i].z, a.z)};
   }
-  const V3<R> da = sym_mul(W, a);
-  l = V4<R>{l.x + im * dl[0], l.y + im * dl[1], l.z + im * dl[2], R(0)};
+  const V3<R> da = sym_madd(W, a);
+  l = V4<R>{madd(im, dl[0], l.x), madd(im, dl[1], l.y), madd(im, dl[2], l.z), R(0)};
   g = V4<R>{g.x + da.x, g.y + da.y, g.z + da.z, R(0)};
 }
 
@@ -1341,8 +1361,8 @@ __device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow
   const V4<R> jl = it.jl, ja = it.ja, jb = it.jb;
   const R xo = it.xo;
   // row q of z = x - r (J w + b)  (contacts.py:77-92)
-  const R Jw = -dot(xyz(jl), xyz(lA)) - dot(xyz(ja), xyz(aA)) + dot(xyz(jl), xyz(lB)) + dot(xyz(jb), xyz(aB));
-  const R z = xo - (ja.w * rs) * (Jw + jl.w);
+  const R Jw = row_jw(xyz(jl), xyz(ja), xyz(jb), xyz(lA), xyz(aA), xyz(lB), xyz(aB));
+  const R z = row_z(xo, ja.w * rs, Jw, jl.w);
   const R z0 = quad_bcast<0>(z), z1 = quad_bcast<1>(z), z2 = quad_bcast<2>(z), z3 = quad_bcast<3>(z);
   const R xo0 = quad_bcast<0>(xo);
   const R mu0 = quad_bcast<1>(jb.w), mu1 = quad_bcast<2>(jb.w), mu2 = quad_bcast<3>(jb.w);
@@ -1365,17 +1385,18 @@ __device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow
     const R sign = side_b ? R(1) : R(-1);
     const R s0 = sign * d0, s1 = sign * d1, s2 = sign * d2, s3 = sign * d3;
     V3<R> acc = {sum_first(v0.x * s0), sum_first(v0.y * s0), sum_first(v0.z * s0)};
-    acc.x = acc.x + v1.x * s1; acc.y = acc.y + v1.y * s1; acc.z = acc.z + v1.z * s1;
-    acc.x = acc.x + v2.x * s2; acc.y = acc.y + v2.y * s2; acc.z = acc.z + v2.z * s2;
-    acc.x = acc.x + v3.x * s3; acc.y = acc.y + v3.y * s3; acc.z = acc.z + v3.z * s3;
-    V3<R> t;
+    acc.x = madd(v1.x, s1, acc.x); acc.y = madd(v1.y, s1, acc.y); acc.z = madd(v1.z, s1, acc.z);
+    acc.x = madd(v2.x, s2, acc.x); acc.y = madd(v2.y, s2, acc.y); acc.z = madd(v2.z, s2, acc.z);
+    acc.x = madd(v3.x, s3, acc.x); acc.y = madd(v3.y, s3, acc.y); acc.z = madd(v3.z, s3, acc.z);
+    V4<R> nw;
     if (ang) {
       const R W[6] = {W0.x, W0.y, W0.z, W0.w, W1.x, W1.y};
-      t = sym_mul(W, acc);
+      const V3<R> t = sym_madd(W, acc);
+      nw = V4<R>{old.x + t.x, old.y + t.y, old.z + t.z, R(0)};
     } else {
-      t = V3<R>{W1.z * acc.x, W1.z * acc.y, W1.z * acc.z};
+      nw = V4<R>{madd(W1.z, acc.x, old.x), madd(W1.z, acc.y, old.y), madd(W1.z, acc.z, old.z), R(0)};
     }
-    dst[slot] = V4<R>{old.x + t.x, old.y + t.y, old.z + t.z, R(0)};
+    dst[slot] = nw;
   }
   return fabs((double)xq - (double)xo);
 }

@@ -127,7 +127,7 @@ struct rbh_ctx {
   DBuf kidx, kpos, n_kept, kept_flag, kept_scan;
   DBuf c_ia, c_ib, c_tri, c_pair, c_p, c_n, n_contacts, c_rA, c_rB, c_b, c_r0, c_mu, c_imp, c_color;
   DBuf f_bs, uf, isl_cnt, isl_off, isl_fill, isl_max, c_lab, uperm, u_cnt, u_base, n_units, u_start, u_end, u_ncol,
-      u_bs, body_claim, body_loc, bc_cnt, bc_off, bc_fill, bc_list, pal, g_pc, g_done, col_off, merit, bar;
+      u_bs, body_claim, body_loc, bc_cnt, bc_off, bc_fill, bc_list, pal, g_pc, g_done, col_off, merit;
   DBuf u_T, isl_unit, u_nbody, u_boff, f_loc, f_unit, u_blist, s_loc, s_W;
   DBuf l_cnt, l_base, n_lanes, l_start, l_end, l_ncol, l_bs;
   DBuf s_JL, s_JA, s_JB, xr0, xr1;
@@ -205,7 +205,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->c_imp, &c->c_color, &c->f_bs, &c->uf, &c->isl_cnt, &c->isl_off, &c->isl_fill, &c->isl_max,
           &c->c_lab, &c->uperm, &c->u_cnt, &c->u_base, &c->n_units, &c->u_start, &c->u_end, &c->u_ncol,
           &c->u_bs, &c->body_claim, &c->body_loc, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->pal,
-          &c->g_pc, &c->g_done, &c->col_off, &c->merit, &c->bar, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
+          &c->g_pc, &c->g_done, &c->col_off, &c->merit, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
           &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part, &c->u_T, &c->isl_unit,
           &c->u_nbody, &c->u_boff, &c->f_loc, &c->f_unit, &c->u_blist, &c->s_loc, &c->s_W, &c->l_cnt,
           &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs, &c->s_JL, &c->s_JA, &c->s_JB,
@@ -470,8 +470,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.u_bs = P(int*, u_bs); d.body_claim = P(int*, body_claim); d.body_loc = P(int2*, body_loc);
   d.bc_cnt = P(int*, bc_cnt); d.bc_off = P(int*, bc_off); d.bc_fill = P(int*, bc_fill);
   d.bc_list = P(int*, bc_list); d.pal = P(unsigned*, pal); d.g_pc = P(int*, g_pc);
-  d.g_done = P(unsigned char*, g_done); d.col_off = P(int*, col_off); d.merit = P(double*, merit);
-  d.bar = P(unsigned*, bar);
+  d.g_done = P(unsigned char*, g_done); d.col_off = P(int*, col_off); d.merit = P(unsigned long long*, merit);
   d.perm = P(int*, perm); d.s_id = P(int2*, s_id); d.s_n = P(V4<R>*, s_n); d.s_rA = P(V4<R>*, s_rA);
   d.s_rB = P(V4<R>*, s_rB); d.s_b = P(V4<R>*, s_b); d.s_r0 = P(V4<R>*, s_r0); d.x0 = P(V4<R>*, x0);
   d.x1 = P(V4<R>*, x1);
@@ -522,7 +521,6 @@ int size_workspace(rbh_ctx* c) {
       {&c->l_start, nu * 4}, {&c->l_end, nu * 4}, {&c->l_ncol, nu * 4}, {&c->l_bs, nu * 4},
       {&c->s_JL, 4 * ck * V}, {&c->s_JA, 4 * ck * V}, {&c->s_JB, 4 * ck * V}, {&c->xr0, 4 * ck * c->prec},
       {&c->xr1, 4 * ck * c->prec},
-      {&c->bar, 64},
       {&c->perm, ck * 4}, {&c->s_id, ck * 8}, {&c->s_n, ck * V}, {&c->s_rA, ck * V}, {&c->s_rB, ck * V},
       {&c->s_b, ck * V}, {&c->s_r0, ck * V}, {&c->x0, ck * V}, {&c->x1, ck * V}, {&c->errbuf, 64},
       {&c->scan_part, (size_t)scan_parts(maxcap) * 4}};
@@ -638,6 +636,7 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   Dev<R> d = make_dev<R>(c, dt, flags, multi);
   hipStream_t s = c->stream;
   const int nb = c->nb, ns = c->ns;
+  const long merit_slots = (long)std::max(c->max_iter_batch, 1) * multi_grid;
   auto mark = [&](int st) { if (ev) (void)hipEventRecord(ev[st], s); };
   mark(ST_BEGIN);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, d.err, 16, 0);
@@ -688,8 +687,8 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_color_units<R>, dim3(c->cap_units), dim3(COLOR_UNIT_BLOCK), 0, s, d);
   hipLaunchKernelGGL(k_order_lanes<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
   if (multi) {
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(d.merit), 2 * c->max_iter_batch, 0);
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(d.bar), 16, 0);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(2L * merit_slots, 256)), dim3(256), 0, s, reinterpret_cast<int*>(d.merit),
+                       2 * merit_slots, -1);
     mark(ST_COLOR);
     hinge_pass<R>(c, d, 0);
     void* args[] = {&d};
@@ -714,8 +713,8 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
     Dev<R> dp = d;
     dp.post = 1;
     if (multi) {
-      hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(d.merit), 2 * c->max_iter_batch, 0);
-      hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(d.bar), 16, 0);
+      hipLaunchKernelGGL(k_fill, dim3(grid_for(2L * merit_slots, 256)), dim3(256), 0, s,
+                         reinterpret_cast<int*>(d.merit), 2 * merit_slots, -1);
       void* args[] = {&dp};
       hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<void*>(k_solve_multi<R>), dim3(multi_grid),
                                                 dim3(SOLVE_BLOCK), args, c->solve_lds, s);
@@ -796,6 +795,11 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
     HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_multi<R>),
                                                            SOLVE_BLOCK, c->solve_lds));
     c->coop_max = std::max(1, per_cu) * c->prop.multiProcessorCount;
+  }
+  if (c->multi_ok) {
+    // merit slots of the cooperative solve: one per sweep and workgroup
+    int rc = ensure(c, c->merit, (size_t)std::max(c->max_iter_batch, 1) * c->coop_max * 8);
+    if (rc) return rc;
   }
   bool multi = c->multi_ok && !use_graph;
   if (flags & RBH_STEP_FORCE_SINGLE) multi = false;

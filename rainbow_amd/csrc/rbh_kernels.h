@@ -114,8 +114,9 @@ template <typename R> struct Dev {
   unsigned* pal;                       // PALW words per unit-order position
   int* g_pc; unsigned char* g_done;    // colouring state of units too large for LDS
   int* col_off;                        // colour offsets of lane l at 2 * (l_start[l] + l)
-  double* merit;                       // per-sweep merit of a scene solved by several workgroups
-  unsigned* bar;                       // grid barrier counters
+  // a scene solved by several workgroups: per sweep and workgroup, the bits
+  // of that workgroup's merit (all ones until it publishes them)
+  unsigned long long* merit;
   // solver arrays (colour order)
   int* perm; int2* s_id; V4<R>* s_n; V4<R>* s_rA; V4<R>* s_rB; V4<R>* s_b; V4<R>* s_r0; V4<R>* x0; V4<R>* x1;
   int2* s_loc;                         // body slots (u_blist index) of the two bodies (-1: fixed)

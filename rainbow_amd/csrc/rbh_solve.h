@@ -1043,29 +1043,44 @@ template <typename R> __device__ void write_solve_state(const Dev<R>& d, SceneSt
   st.final_merit = c.last_merit;
 }
 
-// grid-wide barrier for a cooperative launch; a bounded wait that flags an
-// error instead of spinning forever
-__device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nblocks, int* err, int* s_ok) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    unsigned g = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned arrived = atomicAdd(&bar[0], 1u) + 1u;
-    if (arrived == nblocks) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      unsigned long long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1ull << 26)) { atomicOr(err, ERR_SOLVER); ok = 0; break; }
+// The scene-wide merit of sweep `it` in a cooperative launch, which is also
+// the sweep's grid barrier.  Workgroup b publishes its merit in slot
+// it * G + b: the bits of the non-negative double, or 0 for a merit that is
+// not > 0 (a maximum that starts from 0 ignores it).  The slots start as all
+// ones, which no published value is.  Wavefront 0 then waits until the
+// sweep's G slots are all published and takes their maximum.  Workgroups
+// share no other data during the solve, so the slots need no fences.  The
+// wait is bounded: it flags ERR_SOLVER and returns false instead of hanging.
+__device__ __forceinline__ bool grid_merit(unsigned long long* slots, int it, double& merit, int* err, int* s_ok,
+                                           double* s_m) {
+  const unsigned G = gridDim.x;
+  unsigned long long* row = slots + (size_t)it * G;
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&row[blockIdx.x], merit > 0.0 ? (unsigned long long)__double_as_longlong(merit) : 0ull,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 64) {
+    unsigned long long m = 0;
+    bool ok = true;
+    for (unsigned j = threadIdx.x; j < G && ok; j += 64) {
+      unsigned long long v = __hip_atomic_load(&row[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (unsigned spins = 0; v == ~0ull;) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 26)) { ok = false; break; }
+        v = __hip_atomic_load(&row[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      if (ok) m = max(m, v);
     }
-    __threadfence();
-    *s_ok = ok;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, o, 64));
+    const bool all_ok = __ballot(!ok) == 0;
+    if (threadIdx.x == 0) {
+      if (!all_ok) atomicOr(err, ERR_SOLVER);
+      *s_ok = all_ok ? 1 : 0;
+      *s_m = __longlong_as_double((long long)m);
+    }
   }
   __syncthreads();
+  merit = *s_m;
   return *s_ok != 0;
 }
 
@@ -1240,11 +1255,7 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
                                                   jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     double merit = block_max(m, sh_red);
-    if (COOP) {
-      if (threadIdx.x == 0 && merit > 0.0) atomic_max_nonneg(&d.merit[it], merit);
-      if (!grid_barrier(d.bar, gridDim.x, d.err, s_ok)) return;
-      merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (COOP && !grid_merit(d.merit, it, merit, d.err, s_ok, sh_red)) return;
     const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it,
                                hinge_merit(d, sp, it));
     // accepted sweep: ws <- ww, else ww <- ws
@@ -1548,11 +1559,7 @@ __device__ __forceinline__ int sweep_decide(const Dev<R>& d, const SceneParams& 
                                             int it, bool lead, double* sh_red, int* s_ok, const BodyLds<R>& B, int nb,
                                             bool& abort) {
   double merit = block_max(m, sh_red);
-  if (COOP) {
-    if (threadIdx.x == 0 && merit > 0.0) atomic_max_nonneg(&d.merit[it], merit);
-    if (!grid_barrier(d.bar, gridDim.x, d.err, s_ok)) { abort = true; return 0; }
-    merit = __hip_atomic_load(&d.merit[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (COOP && !grid_merit(d.merit, it, merit, d.err, s_ok, sh_red)) { abort = true; return 0; }
   double* res = d.residuals + sp.res_off;
   const int dec = ctl.decide(sp, merit, (lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it,
                              hinge_merit(d, sp, it));

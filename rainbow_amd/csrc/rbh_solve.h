@@ -906,6 +906,16 @@ template <typename R> __host__ __device__ constexpr int lds_body_cap() {
   return SOLVE_LDS_BYTES / (4 * (int)sizeof(V4<R>));
 }
 
+// The scene parameters the friction projection reads, taken into registers
+// once per solve: a reference into the scene table would be read again from
+// memory after every colour phase's fence.
+struct ProxParams {
+  double ell_tol, ell_expansion;
+  int friction, ell_max_iter;
+  __device__ explicit ProxParams(const SceneParams& sp)
+      : ell_tol(sp.ell_tol), ell_expansion(sp.ell_expansion), friction(sp.friction), ell_max_iter(sp.ell_max_iter) {}
+};
+
 // One row of J w (lin, angA, angB: the row's blocks).  FP64 keeps the
 // oracle's sum of four dot products; FP32 sums side A and side B in two
 // chains of fused multiply-adds.
@@ -929,7 +939,7 @@ template <typename R> __device__ __forceinline__ R row_z(R xo, R r, R jw, R b) {
 
 // contacts.py:77-95: z = x - r (J w + b); x_n = max(0, z_n); friction prox
 template <typename R>
-__device__ __forceinline__ V4<R> contact_prox(const SceneParams& sp, const ContactRow<R>& J, V4<R> xo, V4<R> bb,
+__device__ __forceinline__ V4<R> contact_prox(const ProxParams& pp, const ContactRow<R>& J, V4<R> xo, V4<R> bb,
                                               V4<R> rr, V4<R> mu, V3<R> wlA, V3<R> waA, V3<R> wlB, V3<R> waB,
                                               bool use_old_xn) {
   R Jw[4];
@@ -941,9 +951,9 @@ __device__ __forceinline__ V4<R> contact_prox(const SceneParams& sp, const Conta
   R xn = fmax(R(0), z0);
   R xr = use_old_xn ? xo.x : xn;
   R os, ot, orr;
-  if (sp.friction == 1) {
+  if (pp.friction == 1) {
     R o[3];
-    prox_ellipsoid_call<R>(z1, z2, z3, mu.x, mu.y, mu.z, xr, sp.ell_tol, sp.ell_expansion, sp.ell_max_iter, o);
+    prox_ellipsoid_call<R>(z1, z2, z3, mu.x, mu.y, mu.z, xr, pp.ell_tol, pp.ell_expansion, pp.ell_max_iter, o);
     os = o[0]; ot = o[1]; orr = o[2];
   } else {
     prox_sphere<R>(z1, z2, z3, mu.x, xr, os, ot, orr);
@@ -1116,7 +1126,7 @@ template <typename R> __device__ __forceinline__ void load_item(const Dev<R>& d,
 // accepted ones and adds its change to the working ones.  Returns
 // max |x_new - x_old| in FP64.
 template <typename R, bool LDS>
-__device__ __forceinline__ double update_contact(const SceneParams& sp, const Item<R>& it, VelStore<R, LDS>& w,
+__device__ __forceinline__ double update_contact(const ProxParams& pp, const Item<R>& it, VelStore<R, LDS>& w,
                                                  V4<R>* xn, int kk, R rs, bool jacobi, bool hybrid) {
   ContactRow<R> J;
   contact_rows(it.n, it.rA, it.rB, J);
@@ -1129,7 +1139,7 @@ __device__ __forceinline__ double update_contact(const SceneParams& sp, const It
   const V4<R>* ra = jacobi ? w.sa : w.wa;
   const V4<R> lA = fa ? rl[sa] : z, aA = fa ? ra[sa] : z;
   const V4<R> lB = fb ? rl[sb] : z, aB = fb ? ra[sb] : z;
-  const V4<R> x = contact_prox(sp, J, it.xo, it.b, rr, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), jacobi && !hybrid);
+  const V4<R> x = contact_prox(pp, J, it.xo, it.b, rr, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), jacobi && !hybrid);
   xn[kk] = x;
   const V4<R> dx = {x.x - it.xo.x, x.y - it.xo.y, x.z - it.xo.z, x.w - it.xo.w};
   if (fa) {
@@ -1163,7 +1173,7 @@ template <int STRIDE> __device__ __forceinline__ void phase_sync() {
 // passes of STRIDE; colours are separated by a sync.  coff: colour offsets
 // (serial: colour c is position c).  Returns this thread's max |dx|.
 template <typename R, bool LDS, int STRIDE>
-__device__ double sweep_lane(const Dev<R>& d, const SceneParams& sp, VelStore<R, LDS>& w, int p0, int nc,
+__device__ double sweep_lane(const Dev<R>& d, const ProxParams& pp, VelStore<R, LDS>& w, int p0, int nc,
                              const int* coff, bool serial, bool jacobi, bool hybrid, const V4<R>* xa, V4<R>* xn,
                              R rs, int lane) {
   double m = 0.0;
@@ -1189,7 +1199,7 @@ __device__ double sweep_lane(const Dev<R>& d, const SceneParams& sp, VelStore<R,
     }
     fhave = c1 < nc && pb1 + lane < hi1;
     if (fhave) load_item(d, xa, p0 + pb1 + lane, fill);
-    if (have) m = fmax(m, update_contact(sp, use, w, xn, p0 + pb + lane, rs, jacobi, hybrid));
+    if (have) m = fmax(m, update_contact(pp, use, w, xn, p0 + pb + lane, rs, jacobi, hybrid));
     if (!same) phase_sync<STRIDE>();
     c = c1; pb = pb1; hi = hi1; hn = hn1;
   };
@@ -1231,6 +1241,7 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
   const bool serial = sp.scheme == SCHEME_GS;
   const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
+  const ProxParams pp(sp);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
   double* res = d.residuals + sp.res_off;
   SolveCtl ctl;
@@ -1244,14 +1255,14 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
       for (int l = gr.l_lo + wave; l < gr.l_hi; l += nwave) {
         const int p0 = d.l_start[l], K = d.l_end[l] - p0;
         if (K <= 0 || (K > WAVE_LANE_MAX && !serial)) continue;
-        m = fmax(m, sweep_lane<R, LDS, 64>(d, sp, w, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial, jacobi,
+        m = fmax(m, sweep_lane<R, LDS, 64>(d, pp, w, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial, jacobi,
                                            hybrid, xa, xn, rs, lane));
       }
     }
     for (int l = gr.l_lo; l < gr.l_hi; ++l) {
       const int p0 = d.l_start[l], K = d.l_end[l] - p0;
       if (K <= 0 || (LDS && (K <= WAVE_LANE_MAX || serial))) continue;
-      m = fmax(m, sweep_lane<R, LDS, NT>(d, sp, w, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
+      m = fmax(m, sweep_lane<R, LDS, NT>(d, pp, w, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
                                                   jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     double merit = block_max(m, sh_red);
@@ -1287,7 +1298,7 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
       const int sa = w.at(c.loc.x, c.id.x), sb = w.at(c.loc.y, c.id.y);
       const V4<R> lA = fa ? w.sl[sa] : z, aA = fa ? w.sa[sa] : z;
       const V4<R> lB = fb ? w.sl[sb] : z, aB = fb ? w.sa[sb] : z;
-      const V4<R> y = contact_prox(sp, J, x, c.b, c.r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
+      const V4<R> y = contact_prox(pp, J, x, c.b, c.r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
       rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
                                 (double)x.w - (double)y.w));
       xmax = fmax(xmax, absmax4(x.x, x.y, x.z, x.w));
@@ -1345,7 +1356,7 @@ template <typename R> struct BodyLds {
 // JL/JA/JB + 4 i: the contact's rows (LDS or global); this lane's velocity
 // block is built from rows 0..3 of one of them, loaded first.
 template <typename R>
-__device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow<R>& it, int q, R rs, bool jacobi,
+__device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<R>& it, int q, R rs, bool jacobi,
                                             bool hybrid, const BodyLds<R>& B, int zs, const V4<R>* JL,
                                             const V4<R>* JA, const V4<R>* JB, size_t i4, R& xq) {
   const V4<R>* srcv = (q & 1) ? (q >= 2 ? JB : JA) : JL;
@@ -1381,9 +1392,9 @@ __device__ __forceinline__ double quad_core(const SceneParams& sp, const QuadRow
   const R xnrm = fmax(R(0), z0);
   const R xr = (jacobi && !hybrid) ? xo0 : xnrm;
   R os, ot, orr;
-  if (sp.friction == 1) {
+  if (pp.friction == 1) {
     R o[3];
-    prox_ellipsoid_call<R>(z1, z2, z3, mu0, mu1, mu2, xr, sp.ell_tol, sp.ell_expansion, sp.ell_max_iter, o);
+    prox_ellipsoid_call<R>(z1, z2, z3, mu0, mu1, mu2, xr, pp.ell_tol, pp.ell_expansion, pp.ell_max_iter, o);
     os = o[0]; ot = o[1]; orr = o[2];
   } else {
     prox_sphere<R>(z1, z2, z3, mu0, xr, os, ot, orr);
@@ -1426,7 +1437,7 @@ __device__ __forceinline__ void load_quad(const V4<R>* JL, const V4<R>* JA, cons
 // natural-map residual parts of contact i (FP64) from its rows and the
 // accepted velocity changes; the impulses' final values
 template <typename R>
-__device__ __forceinline__ void quad_finish_contact(const SceneParams& sp, const V4<R>* JL, const V4<R>* JA,
+__device__ __forceinline__ void quad_finish_contact(const ProxParams& pp, const V4<R>* JL, const V4<R>* JA,
                                                     const V4<R>* JB, const R* xs, int i, const V4<R>& lA,
                                                     const V4<R>& aA, const V4<R>& lB, const V4<R>& aB, V4<R>& x,
                                                     double& rloc, double& xmax) {
@@ -1441,7 +1452,7 @@ __device__ __forceinline__ void quad_finish_contact(const SceneParams& sp, const
     bp[r] = a.w; rp[r] = e.w;
   }
   const V4<R> mu = {JB[4 * (size_t)i + 1].w, JB[4 * (size_t)i + 2].w, JB[4 * (size_t)i + 3].w, R(0)};
-  const V4<R> y = contact_prox(sp, J, x, bb, r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
+  const V4<R> y = contact_prox(pp, J, x, bb, r0, mu, xyz(lA), xyz(aA), xyz(lB), xyz(aB), false);
   rloc = fmax(rloc, absmax4((double)x.x - (double)y.x, (double)x.y - (double)y.y, (double)x.z - (double)y.z,
                             (double)x.w - (double)y.w));
   xmax = fmax(xmax, absmax4(x.x, x.y, x.z, x.w));
@@ -1529,7 +1540,7 @@ template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d
 // one sweep of one resident lane by STRIDE threads (a wavefront or the
 // workgroup): STRIDE / 4 contacts per pass
 template <typename R, int STRIDE>
-__device__ double res_sweep(const SceneParams& sp, ResLds<R>& L, int4 lt, bool serial, bool jacobi, bool hybrid,
+__device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool serial, bool jacobi, bool hybrid,
                             const R* xa, R* xn, R rs, int tid) {
   const int c0 = lt.x, nc = serial ? lt.y : lt.z;
   const int* co = L.coff + lt.w;
@@ -1545,7 +1556,7 @@ __device__ double res_sweep(const SceneParams& sp, ResLds<R>& L, int4 lt, bool s
       it.xo = xa[4 * i + q];
       it.lc = L.loc[i];
       R xq;
-      m = fmax(m, quad_core(sp, it, q, rs, jacobi, hybrid, L.B, L.zs, L.JL, L.JA, L.JB, 4 * (size_t)i, xq));
+      m = fmax(m, quad_core(pp, it, q, rs, jacobi, hybrid, L.B, L.zs, L.JL, L.JA, L.JB, 4 * (size_t)i, xq));
       xn[4 * i + q] = xq;
     }
     phase_sync<STRIDE>();
@@ -1614,6 +1625,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   const bool serial = sp.scheme == SCHEME_GS;
   const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
+  const ProxParams pp(sp);
   ResLds<R> L(smem);
   const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
   // stage: lane table, colour offsets, contact rows and iterates, bodies
@@ -1660,12 +1672,12 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     for (int t = wave; t < gs.nl; t += nwave) {
       const int4 lt = L.lt[t];
       if (lt.y > WAVE_LANE_MAX && !serial) continue;
-      m = fmax(m, res_sweep<R, 64>(sp, L, lt, serial, jacobi, hybrid, xa, xn, rs, lane));
+      m = fmax(m, res_sweep<R, 64>(pp, L, lt, serial, jacobi, hybrid, xa, xn, rs, lane));
     }
     for (int t = 0; t < gs.nl; ++t) {
       const int4 lt = L.lt[t];
       if (lt.y <= WAVE_LANE_MAX || serial) continue;
-      m = fmax(m, res_sweep<R, NT>(sp, L, lt, serial, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
+      m = fmax(m, res_sweep<R, NT>(pp, L, lt, serial, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
     if (abort) return;
@@ -1677,7 +1689,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
     const int2 lc = L.loc[i];
     V4<R> x;
-    quad_finish_contact(sp, L.JL, L.JA, L.JB, xs, i, L.B.sl[lc.x], L.B.sa[lc.x], L.B.sl[lc.y], L.B.sa[lc.y], x,
+    quad_finish_contact(pp, L.JL, L.JA, L.JB, xs, i, L.B.sl[lc.x], L.B.sa[lc.x], L.B.sl[lc.y], L.B.sa[lc.y], x,
                         rloc, xmax);
     if (!d.post) d.c_imp[d.perm[gs.p0 + i]] = x;
   }
@@ -1688,7 +1700,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
 // global memory, the next pass prefetched into the other of two buffers
 // while the current one is updated.
 template <typename R, int STRIDE>
-__device__ double stream_sweep(const Dev<R>& d, const SceneParams& sp, const BodyLds<R>& B, int zs, int b0, int p0,
+__device__ double stream_sweep(const Dev<R>& d, const ProxParams& pp, const BodyLds<R>& B, int zs, int b0, int p0,
                                int nc, const int* coff, bool serial, bool jacobi, bool hybrid, const R* xa, R* xn,
                                R rs, int tid) {
   constexpr int PER = STRIDE / 4;
@@ -1715,7 +1727,7 @@ __device__ double stream_sweep(const Dev<R>& d, const SceneParams& sp, const Bod
     if (fhave) load_quad(d.s_JL, d.s_JA, d.s_JB, xa, d.s_loc, p0 + pb1 + j, q, b0, zs, fill);
     if (have) {
       R xq;
-      m = fmax(m, quad_core(sp, use, q, rs, jacobi, hybrid, B, zs, d.s_JL, d.s_JA, d.s_JB,
+      m = fmax(m, quad_core(pp, use, q, rs, jacobi, hybrid, B, zs, d.s_JL, d.s_JA, d.s_JB,
                             4 * (size_t)(p0 + pb + j), xq));
       xn[4 * (size_t)(p0 + pb + j) + q] = xq;
     }
@@ -1740,6 +1752,7 @@ __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group&
   const bool serial = sp.scheme == SCHEME_GS;
   const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
+  const ProxParams pp(sp);
   StreamLds<R> L(smem);
   const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
   stage_bodies(d, L.B, b0, nb, L.zs);
@@ -1756,13 +1769,13 @@ __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group&
     for (int l = gr.l_lo + wave; l < gr.l_hi; l += nwave) {
       const int p0 = d.l_start[l], K = d.l_end[l] - p0;
       if (K <= 0 || (K > WAVE_LANE_MAX && !serial)) continue;
-      m = fmax(m, stream_sweep<R, 64>(d, sp, L.B, L.zs, b0, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
+      m = fmax(m, stream_sweep<R, 64>(d, pp, L.B, L.zs, b0, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
                                       jacobi, hybrid, xa, xn, rs, lane));
     }
     for (int l = gr.l_lo; l < gr.l_hi; ++l) {
       const int p0 = d.l_start[l], K = d.l_end[l] - p0;
       if (K <= WAVE_LANE_MAX || serial) continue;
-      m = fmax(m, stream_sweep<R, NT>(d, sp, L.B, L.zs, b0, p0, d.l_ncol[l], lane_coff(d, l), serial,
+      m = fmax(m, stream_sweep<R, NT>(d, pp, L.B, L.zs, b0, p0, d.l_ncol[l], lane_coff(d, l), serial,
                                                jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
@@ -1778,7 +1791,7 @@ __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group&
       const int2 lc = d.s_loc[i];
       const int sa = lc.x >= 0 ? lc.x - b0 : L.zs, sb = lc.y >= 0 ? lc.y - b0 : L.zs;
       V4<R> x;
-      quad_finish_contact(sp, d.s_JL, d.s_JA, d.s_JB, xs, i, L.B.sl[sa], L.B.sa[sa], L.B.sl[sb], L.B.sa[sb], x,
+      quad_finish_contact(pp, d.s_JL, d.s_JA, d.s_JB, xs, i, L.B.sl[sa], L.B.sa[sa], L.B.sl[sb], L.B.sa[sb], x,
                           rloc, xmax);
       if (!d.post) d.c_imp[d.perm[i]] = x;
     }

@@ -519,7 +519,16 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
   const int nitems = d.item_off[np];
   const bool count_all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   const int wave = threadIdx.x >> 6;
-  for (int item = blockIdx.x * CAND_WAVES + wave; item < nitems; item += gridDim.x * CAND_WAVES) {
+  // Workgroups go to the 8 XCDs in turn.  When one pass of the grid covers
+  // the items, workgroup w takes logical block (w % 8) * per + w / 8, so each
+  // XCD walks a contiguous run of pairs: the pairs of one body follow each
+  // other and then share its triangle boxes in that XCD's L2.
+  const int per = (nitems + 8 * CAND_WAVES - 1) / (8 * CAND_WAVES);
+  const bool remap = (int)gridDim.x >= 8 * per;
+  if (remap && (int)blockIdx.x >= 8 * per) return;
+  const int first = remap ? ((int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8)) * CAND_WAVES + wave
+                          : (int)blockIdx.x * CAND_WAVES + wave;
+  for (int item = first; item < nitems; item += remap ? nitems : gridDim.x * CAND_WAVES) {
     const int p = item_pair(d.item_off, np, item);
     const int2 ab = d.pairs[p];
     const int ntA = d.shapes[d.b_shape[ab.x]].nt, ntB = d.shapes[d.b_shape[ab.y]].nt;

@@ -1370,16 +1370,9 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
   // fixed bodies read the zero slot (the serial update uses zeros for them)
   const V4<R> lA = rl[it.lc.x], aA = ra[it.lc.x], lB = rl[it.lc.y], aB = ra[it.lc.y];
   const V4<R> W0 = B.W0[slot], W1 = B.W1[slot];
-  // the block this lane updates: already loaded in Gauss-Seidel (w is read in place)
-  // (component-wise selects: a select of whole structs by a per-lane flag goes through scratch)
-  V4<R> old;
-  if (jacobi) {
-    old = dst[slot];
-  } else {
-    const R ox0 = ang ? aA.x : lA.x, oy0 = ang ? aA.y : lA.y, oz0 = ang ? aA.z : lA.z;
-    const R ox1 = ang ? aB.x : lB.x, oy1 = ang ? aB.y : lB.y, oz1 = ang ? aB.z : lB.z;
-    old = V4<R>{side_b ? ox1 : ox0, side_b ? oy1 : oy0, side_b ? oz1 : oz0, R(0)};
-  }
+  // the block this lane updates, read beside the four blocks above (in
+  // Gauss-Seidel it is one of them: no other quad of the phase writes it)
+  const V4<R> old = dst[slot];
   const V4<R> jl = it.jl, ja = it.ja, jb = it.jb;
   const R xo = it.xo;
   // row q of z = x - r (J w + b)  (contacts.py:77-92)
@@ -1399,7 +1392,9 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
   } else {
     prox_sphere<R>(z1, z2, z3, mu0, xr, os, ot, orr);
   }
-  xq = q == 0 ? xnrm : (q == 1 ? os : (q == 2 ? ot : orr));
+  // row q's new value, by two selects on the bits of q
+  const R x01 = ang ? os : xnrm, x23 = ang ? orr : ot;
+  xq = side_b ? x23 : x01;
   const R dq = xq - xo;
   const R d0 = quad_bcast<0>(dq), d1 = quad_bcast<1>(dq), d2 = quad_bcast<2>(dq), d3 = quad_bcast<3>(dq);
   // w += W J^T dx (contacts.py:107), this lane's block

@@ -1355,12 +1355,10 @@ template <typename R> struct BodyLds {
 // block; returns the new iterate in xq and |xq - xo| in FP64.
 // JL/JA/JB + 4 i: the contact's rows (LDS or global); this lane's velocity
 // block is built from rows 0..3 of one of them, loaded first.
-template <typename R>
+template <typename R, typename F>
 __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<R>& it, int q, R rs, bool jacobi,
                                             bool hybrid, const BodyLds<R>& B, int zs, const V4<R>* JL,
-                                            const V4<R>* JA, const V4<R>* JB, size_t i4, R& xq) {
-  const V4<R>* srcv = (q & 1) ? (q >= 2 ? JB : JA) : JL;
-  const V4<R> v0 = srcv[i4], v1 = srcv[i4 + 1], v2 = srcv[i4 + 2], v3 = srcv[i4 + 3];
+                                            const V4<R>* JA, const V4<R>* JB, size_t i4, R& xq, F&& after_loads) {
   const V4<R>* rl = jacobi ? B.sl : B.wl;
   const V4<R>* ra = jacobi ? B.sa : B.wa;
   const bool side_b = q >= 2, ang = q & 1;   // lane 0: linear A, 1: angular A, 2: linear B, 3: angular B
@@ -1373,6 +1371,10 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
   // the block this lane updates, read beside the four blocks above (in
   // Gauss-Seidel it is one of them: no other quad of the phase writes it)
   const V4<R> old = dst[slot];
+  // the caller's loads (the next phase's contact), issued behind this phase's
+  after_loads();
+  const V4<R>* srcv = (q & 1) ? (q >= 2 ? JB : JA) : JL;
+  const V4<R> v0 = srcv[i4], v1 = srcv[i4 + 1], v2 = srcv[i4 + 2], v3 = srcv[i4 + 3];
   const V4<R> jl = it.jl, ja = it.ja, jb = it.jb;
   const R xo = it.xo;
   // row q of z = x - r (J w + b)  (contacts.py:77-92)
@@ -1542,19 +1544,58 @@ __device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool se
   const int q = tid & 3, j = tid >> 2;
   constexpr int PER = STRIDE / 4;
   double m = 0.0;
+  if (nc <= 0) return m;
+  // rows, iterate and body slots of contact i (none of them change during a sweep)
+  auto load = [&](int i, QuadRow<R>& it) {
+    it.jl = L.JL[4 * i + q]; it.ja = L.JA[4 * i + q]; it.jb = L.JB[4 * i + q];
+    it.xo = xa[4 * i + q];
+    it.lc = L.loc[i];
+  };
+  // A quad with no contact in a colour takes a null row (zero rows and
+  // iterate, both bodies in the zero slot): its update is zeros and writes
+  // nothing, so the first pass of every phase runs on the whole wavefront.
+  auto load_or_null = [&](int pos, int end, QuadRow<R>& it) {
+    if (pos < end) {
+      load(c0 + pos, it);
+    } else {
+      const V4<R> z = {R(0), R(0), R(0), R(0)};
+      it.jl = z; it.ja = z; it.jb = z;
+      it.xo = R(0);
+      it.lc = make_int2(L.zs, L.zs);
+    }
+  };
+  // The first contact of this quad in each colour, and the colour's end, are
+  // read during the previous colour's phase, behind that phase's velocity
+  // loads: a phase then waits on one LDS round trip, not two.
+  int lo = serial ? 0 : co[0], hi = serial ? 1 : co[1];
+  QuadRow<R> cur, nxt;
+  load_or_null(lo + j, hi, cur);
   for (int c = 0; c < nc; ++c) {
-    const int lo = serial ? c : co[c], hi = serial ? c + 1 : co[c + 1];
-    for (int pos = lo + j; pos < hi; pos += PER) {
+    const int lo1 = hi;
+    int hi1 = hi;
+    auto next = [&] {
+      if (c + 1 < nc) {
+        hi1 = serial ? c + 2 : co[c + 2];
+        load_or_null(lo1 + j, hi1, nxt);
+      }
+    };
+    const int p0 = lo + j;
+    const bool mine = p0 < hi;
+    // a null row's column rows are read at the lane's first contact (times zero)
+    const int i0 = c0 + (mine ? p0 : 0);
+    R xq;
+    m = fmax(m, quad_core(pp, cur, q, rs, jacobi, hybrid, L.B, L.zs, L.JL, L.JA, L.JB, 4 * (size_t)i0, xq, next));
+    if (mine) xn[4 * i0 + q] = xq;
+    // further contacts of this quad in the colour (a colour wider than the quads)
+    for (int pos = p0 + PER; pos < hi; pos += PER) {
       const int i = c0 + pos;
       QuadRow<R> it;
-      it.jl = L.JL[4 * i + q]; it.ja = L.JA[4 * i + q]; it.jb = L.JB[4 * i + q];
-      it.xo = xa[4 * i + q];
-      it.lc = L.loc[i];
-      R xq;
-      m = fmax(m, quad_core(pp, it, q, rs, jacobi, hybrid, L.B, L.zs, L.JL, L.JA, L.JB, 4 * (size_t)i, xq));
+      load(i, it);
+      m = fmax(m, quad_core(pp, it, q, rs, jacobi, hybrid, L.B, L.zs, L.JL, L.JA, L.JB, 4 * (size_t)i, xq, [] {}));
       xn[4 * i + q] = xq;
     }
     phase_sync<STRIDE>();
+    lo = lo1; hi = hi1; cur = nxt;
   }
   return m;
 }
@@ -1723,7 +1764,7 @@ __device__ double stream_sweep(const Dev<R>& d, const ProxParams& pp, const Body
     if (have) {
       R xq;
       m = fmax(m, quad_core(pp, use, q, rs, jacobi, hybrid, B, zs, d.s_JL, d.s_JA, d.s_JB,
-                            4 * (size_t)(p0 + pb + j), xq));
+                            4 * (size_t)(p0 + pb + j), xq, [] {}));
       xn[4 * (size_t)(p0 + pb + j) + q] = xq;
     }
     if (!same) phase_sync<STRIDE>();

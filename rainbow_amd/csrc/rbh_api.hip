@@ -598,6 +598,10 @@ int grid_for(long n, int block, int maxg = 8192) {
 }
 
 template <typename T> void scan(rbh_ctx* c, const T* in, int* out, const int* n_dev, int cap, int* total) {
+  if (cap < SCAN_SINGLE_TILES * SCAN_TILE) {
+    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, out, total);
+    return;
+  }
   int nb = std::max(1, (cap + SCAN_TILE - 1) / SCAN_TILE + 1);
   int* part = reinterpret_cast<int*>(c->scan_part.p);
   hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, part);

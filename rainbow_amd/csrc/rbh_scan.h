@@ -4,7 +4,7 @@
 // past the live length return at once.
 //
 // Three kernels: per-block totals, one-block scan of the totals, per-block
-// rescan with the block offset.  out[i] = sum(in[0..i)), out[n] = total,
+// rescan with the block offset (one kernel for arrays of at most eight tiles).  out[i] = sum(in[0..i)), out[n] = total,
 // and *total (if given) = total.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -100,6 +100,39 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down(const T* __restrict__ 
     if (idx <= n) out[idx] = run;   // out[n] = total
     run += vals[i];
   }
+}
+
+// Short arrays (bodies, units, scenes): one workgroup scans the tiles in
+// turn with a running carry, in one launch instead of three.  Same outputs as
+// the three kernels above.
+constexpr int SCAN_SINGLE_TILES = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_single(const T* __restrict__ in, const int* __restrict__ n_dev,
+                                                           int cap, int* __restrict__ out, int* __restrict__ total_out) {
+  __shared__ int sh[SCAN_BLOCK / 64];
+  const int n = min(max(*n_dev, 0), cap);
+  int carry = 0;
+  // base <= n: the tile that holds out[n] runs even when n is a multiple of the tile
+  for (long base = 0; base <= n; base += SCAN_TILE) {
+    int vals[SCAN_ITEMS];
+    int s = 0;
+    const long t0 = base + (long)threadIdx.x * SCAN_ITEMS;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+      const long idx = t0 + i;
+      vals[i] = idx < n ? (int)in[idx] : 0;
+      s += vals[i];
+    }
+    int tot;
+    int run = carry + block_exclusive_scan(s, sh, tot);
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+      const long idx = t0 + i;
+      if (idx <= n) out[idx] = run;   // out[n] = total
+      run += vals[i];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
 }  // namespace rbh

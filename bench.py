@@ -4,8 +4,11 @@
 Workload: 4096 spheres (create_sphere(0.5, 16, 16), 32^3 SDF) dropped from
 [-9.5, 9.5] x [1, 40] x [-9.5, 9.5] onto a fixed ground box, mu = 0.3,
 dt = 0.005, 50 coloured parallel Gauss-Seidel sweeps per step (tolerances 0),
-FP32 state, FP64 merit/residual.  The reference's run is 200 steps; the
-default --warmup 20 --steps 180 times steps 21..200 of that run.
+FP32 state, FP64 merit/residual.  The run is 200 steps long; the timed window
+is its last --steps steps (never starting before step --warmup + 1): the
+steps before the window run untimed, so under any --steps/--warmup the window
+covers the run's contact regime (the spheres dropped from low heights have
+landed and piled up by then).
 
 metric: contact-iterations per second over the whole step (collision
 detection + assembly + colouring + solve + integration), inputs resident in
@@ -16,14 +19,20 @@ roofline: the solve kernel (the largest single kernel of the step) against
 HBM peak, with 288 algorithmic bytes per contact-iteration; its duration is
 the mean over the timed steps of HIP events recorded on the library's stream.
 traffic: HBM bytes per solve launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-of this same command, read from the committed summary under profiles/.
+of this same command, read from the committed summary under profiles/.  The
+timed window is bracketed by roctxProfilerResume/Pause, so a profile taken
+with rocprofv3 --selected-regions covers exactly the timed steps.
 roofline.latency: the solve is bound by its dependent colour phases (each
 sweep visits the colours of its longest sweep lane one after another); one
 step after the timed region gives colours x sweeps and the microseconds per
 phase.
 
-cpu_baseline: the FP64 NumPy restatement of the step, single-threaded, timed
-on one step of the same scene from the GPU's end state.
+cpu_baseline: the FP64 NumPy restatement of the step, single-threaded, on one
+step of the same scene from the GPU's state at the end of the timed window
+("port", O(K) per sweep); beside it ("faithful") the reference's own sweep on
+its scipy.sparse structures (a full J.dot(w) and WJT.tocsr() per contact,
+contacts.py:89, 107) on the same contact problem for a bounded time on one
+core.  The host's CPU model and core count are recorded.
 
 batched (on by default, --no-batched to skip): 1024 independent scenes of
 8..512 boxes and spheres (log-uniform counts, mu ~ U[0.1, 0.8], dt = 0.01,
@@ -57,6 +66,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -64,6 +75,7 @@ BYTES_PER_CONTACT_ITER = 288
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_solve_multi_spheres4096_pmc.txt")
 WORKLOAD = "sphere_rain_4096"
+RUN_STEPS = 200      # length of the 4096-sphere run; the timed window is its last steps
 
 
 def log(*a):
@@ -94,27 +106,90 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(eng, stepper, steps_done):
-    """One oracle step of the same scene from the GPU's current state."""
+def host_info():
+    """CPU model and core counts of this host."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "usable_cores": usable}
+
+
+class ProfilerRegion:
+    """roctxProfilerResume(0) / roctxProfilerPause(0) of the ROCm marker
+    library: under `rocprofv3 --selected-regions` only what runs between them
+    is traced or counted.  Without the profiler (or the library) both are
+    no-ops."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            for fn in ("roctxProfilerResume", "roctxProfilerPause"):
+                getattr(self.lib, fn).argtypes = [ctypes.c_uint64]
+                getattr(self.lib, fn).restype = ctypes.c_int
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+
+def cpu_baseline(eng, state, steps_done, faithful_seconds=10.0):
+    """One oracle step of the scene from ``state`` (the GPU's state after
+    ``steps_done`` steps), and the reference's own sweep on that step's
+    contact problem for ``faithful_seconds``; both on one thread."""
     from threadpoolctl import threadpool_limits
-    from oracle import scene as OS, stepper as OST
-    stepper._finish(eng)
+    from oracle import faithful, problem as OP, scene as OS, stepper as OST
+    r, q, v, w = state
+    for k, b in enumerate(eng.bodies.values()):
+        if not b.is_fixed:
+            b.r, b.q, b.v, b.w = r[k].copy(), q[k].copy(), v[k].copy(), w[k].copy()
     sc = OS.from_engine(eng)
     sc.step_index = steps_done
+    Winv = OP.body_mass_blocks(sc.copy())[0]
     dt = eng.params.time_step
     with threadpool_limits(1):
         t0 = time.perf_counter()
         out = OST.step(sc, dt)
         t = time.perf_counter() - t0
-    lg = out["log"]
-    K = int(lg["contact_points"])
-    sweeps = int(lg.get("iterations", 0)) + (1 if lg.get("exitcode", 0) else 0) if K else 0
-    return dict(value=K * sweeps / t, unit="contact-iters/s", cores=1, kind="port",
+        lg = out["log"]
+        K = int(lg["contact_points"])
+        sweeps = int(lg.get("iterations", 0)) + (1 if lg.get("exitcode", 0) else 0) if K else 0
+        fa = None
+        if K:
+            order = np.concatenate(out["groups"])
+            rate, done, tf, tbuild = faithful.time_sweep(out["problem"], Winv, order, faithful_seconds)
+            fa = {"value": rate, "unit": "contact-iters/s", "cores": 1, "kind": "port",
+                  "sample": "the reference's sweep (contacts.py:66-107: scipy.sparse J and bsr W J^T, a full "
+                            "J.dot(w) and WJT.tocsr() per contact) on the same step's contact problem (K=%d) in "
+                            "colour order from x = 0, %d contact updates in %.1f s, 1 thread; a whole step's %d "
+                            "sweeps would take %.0f s" % (K, done, tf, sweeps, K * sweeps / rate if rate else 0),
+                  "seconds": tf, "contact_updates": done, "matrix_build_s": tbuild}
+    return dict(value=K * sweeps / t if t > 0 else 0.0, unit="contact-iters/s", cores=1, kind="port",
                 sample="one full time step (collision detection, assembly, colouring, %d GS sweeps) of the "
-                       "same 4096-sphere scene from the GPU state after step %d, FP64 NumPy restatement, "
-                       "1 thread: K=%d contacts, %.2f s" % (sweeps, steps_done, K, t),
-                seconds=t, contacts=K, sweeps=sweeps,
-                natural_residual=float(lg.get("natural_residual", 0.0)))
+                       "same 4096-sphere scene from the GPU state after step %d, FP64 NumPy restatement "
+                       "(O(K) vectorised sweep), 1 thread: K=%d contacts, %.2f s" % (sweeps, steps_done, K, t),
+                seconds=t, contacts=K, sweeps=sweeps, natural_residual=float(lg.get("natural_residual", 0.0)),
+                faithful=fa, host=host_info())
 
 
 def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
@@ -428,20 +503,28 @@ def main():
     dt = eng.params.time_step
     log("scene built: %d bodies in %.1f s" % (len(eng.bodies), time.perf_counter() - t0))
 
-    if args.warmup:
-        ctx.step([sid], args.warmup, dt, 0)
+    # untimed: the warm-up steps and the run up to the timed window, which is
+    # the last args.steps steps of the RUN_STEPS-step run (or, for a longer
+    # window, the args.steps steps after the warm-up)
+    start = max(args.warmup, RUN_STEPS - args.steps)
+    if start:
+        ctx.step([sid], start, dt, 0)
     ctx.sync()
     ctx.scene_totals(sid, clear=True)
-    log("warmup done: %d steps" % args.warmup)
+    log("untimed: %d steps (warm-up %d)" % (start, args.warmup))
 
+    region = ProfilerRegion()
+    region.resume()
     t0 = time.perf_counter()
     ctx.step([sid], args.steps, dt, native.STEP_TIMING)
     ctx.sync()
     sec = time.perf_counter() - t0
+    region.pause()
     tot = ctx.scene_totals(sid)
     tm = ctx.get_timings()
     lg = ctx.get_log(sid, eng.params.max_iterations)
-    log("timed %d steps in %.3f s; totals %s" % (args.steps, sec, tot))
+    end_state = ctx.get_state(sid, len(eng.bodies))
+    log("timed steps %d..%d in %.3f s; totals %s" % (start + 1, start + args.steps, sec, tot))
     assert tot["steps"] == args.steps, tot
 
     # one more step, outside the timed region, for the solve's latency figures:
@@ -475,7 +558,8 @@ def main():
         "data": "synthetic (seeded scene of the reference's own shapes)",
         "config": {"workload": WORKLOAD, "bodies": len(eng.bodies), "dt": dt,
                    "gs_sweeps": eng.params.max_iterations, "mu": 0.3, "sdf_resolution": 32,
-                   "steps_timed": "%d..%d" % (args.warmup + 1, args.warmup + args.steps)},
+                   "run_steps": max(RUN_STEPS, start + args.steps),
+                   "steps_timed": "%d..%d" % (start + 1, start + args.steps)},
         "roofline": {"kernel": "k_solve_multi", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
                      "bytes_per_launch": bytes_launch, "launch_ms": solve_ms,
@@ -494,9 +578,10 @@ def main():
     }
     if not args.no_cpu:
         log("cpu baseline: one oracle step ...")
-        cb = cpu_baseline(eng, st, args.warmup + args.steps + 1)
+        cb = cpu_baseline(eng, end_state, start + args.steps)
         result["cpu_baseline"] = cb
-        log("cpu baseline: %.3g contact-iters/s (%.1f s)" % (cb["value"], cb["seconds"]))
+        log("cpu baseline: %.3g contact-iters/s (%.1f s); faithful %s" % (
+            cb["value"], cb["seconds"], cb["faithful"] and "%.3g" % cb["faithful"]["value"]))
     else:
         result["cpu_baseline"] = None
     if not args.no_batched:

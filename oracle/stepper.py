@@ -102,16 +102,21 @@ def post_stabilize(scene, P, groups, prm):
     return res
 
 
-def step(scene, dt, margins=None):
+def step(scene, dt, margins=None, contacts=None):
     """Advance ``scene`` in place by dt.  Returns a dict with the contacts, the
     solved problem, the colouring and the per-step log.  ``margins`` (dict)
     receives the decision margins of the candidate triangles, keyed by
-    (SDF owner, triangle owner, triangle)."""
+    (SDF owner, triangle owner, triangle).  ``contacts`` (a collision.Contacts
+    in the reference's order) replaces collision detection: the rest of the
+    step runs on that contact set."""
     prm = scene.params
     Winv, f_ext, du_ext = problem.body_mass_blocks(scene)
     delta_u = dt * du_ext
     stats = {}
-    contacts = collision.detect(scene, stats, margins)
+    if contacts is None:
+        contacts = collision.detect(scene, stats, margins)
+    else:
+        stats["number_of_overlaps"] = None
     out = dict(contacts=contacts, problem=None, colors=None, log={})
     log = out["log"]
     log["number_of_overlaps"] = stats["number_of_overlaps"]

@@ -4,8 +4,10 @@ The library is built in-tree (``rainbow_amd/librbh.so``) by ``build.py``.  There
 is no fallback: when the library or a GPU is missing, ``load()`` and
 ``Context()`` raise, so no step can silently run anywhere but on the device.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -65,6 +67,19 @@ class DeviceInfo(C.Structure):
 
 
 _lib = None
+_LIVE = weakref.WeakSet()      # open contexts, closed by close_all() at interpreter exit
+
+
+def close_all():
+    """Destroy every open context.  Registered with atexit, so device buffers,
+    graphs, events and streams are released while the HIP runtime is still
+    up, not from object finalizers during interpreter teardown (which run
+    after the runtime's and a profiler's own exit handlers)."""
+    for ctx in list(_LIVE):
+        ctx.close()
+
+
+atexit.register(close_all)
 
 # (name, argtypes) of every entry point declared in include/rbh.h
 _PROTOS = {
@@ -149,11 +164,17 @@ class Context:
             raise RbhError(rc, "rbh_ctx_create failed (code %d): no usable HIP device %d" % (rc, device))
         self.h = h
         self._shape_cache = {}
+        _LIVE.add(self)
 
     def close(self):
         if getattr(self, "h", None):
             self.lib.rbh_ctx_destroy(self.h)
             self.h = None
+        _LIVE.discard(self)
+
+    @property
+    def closed(self):
+        return not getattr(self, "h", None)
 
     def __del__(self):
         try:

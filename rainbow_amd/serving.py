@@ -69,6 +69,7 @@ class Server:
         self.graph = bool(graph)
         self.max_batch = max_batch
         self._active = {}          # during run(): scene id -> [request, steps taken]
+        self.used_bytes = 0        # during run(): footprint of the admitted requests
         if tuning:
             self.ctx.set_tuning(**tuning)
 
@@ -125,7 +126,7 @@ class Server:
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting, active = [], {}       # active: sid -> [request, steps done]
         self._active = active
-        used = 0
+        used = peak = max_active = 0
         lat = []                        # per-step latency samples (s), one per scene per tick
         ticks = 0
         flags = native.STEP_GRAPH if self.graph else 0
@@ -142,6 +143,9 @@ class Server:
                 used += req.footprint
                 req.admitted = clock() - t0
                 active[sid] = [req, 0]
+            self.used_bytes = used
+            peak = max(peak, used)
+            max_active = max(max_active, len(active))
             if not active:
                 if pending:
                     wait = pending[0].arrival - (clock() - t0)
@@ -169,6 +173,7 @@ class Server:
                 req.finished = clock() - t0
                 self.ctx.scene_destroy(sid)
                 used -= req.footprint
+            self.used_bytes = used
         wall = clock() - t0
         self._active = {}
         steps = sum(r.n_steps for r in requests)
@@ -176,7 +181,7 @@ class Server:
         return dict(requests=len(requests), scene_steps=steps, wall_s=wall, scene_steps_per_s=steps / wall,
                     ticks=ticks, p50_step_ms=1e3 * _percentile(lat, 50), p99_step_ms=1e3 * _percentile(lat, 99),
                     p50_wait_ms=1e3 * _percentile(waits, 50), p99_wait_ms=1e3 * _percentile(waits, 99),
-                    budget_bytes=self.budget)
+                    budget_bytes=self.budget, peak_used_bytes=peak, max_active=max_active)
 
     def run_alone(self, req):
         """The request by itself from its initial state; returns its final state (r, q, v, w)."""

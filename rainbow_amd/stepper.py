@@ -25,7 +25,7 @@ _CONTEXTS = {}
 def shared_context(device=0, precision="fp32"):
     """One librbh context per (device, precision) for the process."""
     key = (int(device), precision)
-    if key not in _CONTEXTS:
+    if key not in _CONTEXTS or _CONTEXTS[key].closed:
         _CONTEXTS[key] = native.Context(device, precision)
     return _CONTEXTS[key]
 
@@ -81,11 +81,19 @@ def material_table(engine, bodies):
     return mats, tab
 
 
+def check_masses(bodies):
+    """state_storage.py:169-171: the step refuses a free body without mass."""
+    for b in bodies:
+        if not b.is_fixed and b.mass == 0:
+            raise ValueError("Body {} has no mass.".format(b.name))
+
+
 def engine_desc(engine, ctx):
     """rbh_scene_desc of an engine (state_storage.py:108-185 static inputs)."""
     bodies = list(engine.bodies.values())
     if not bodies:
         raise ValueError("the engine has no bodies")
+    check_masses(bodies)
     shape_ids = {}
     for b in bodies:
         if b.shape is None:
@@ -193,6 +201,7 @@ class HipStepper:
 
     # -- device scene management
     def _prepare(self, engine):
+        check_masses(engine.bodies.values())
         prec = getattr(engine.params, "precision", "fp32")
         ctx = shared_context(getattr(engine.params, "device", 0), prec)
         key = (id(engine), getattr(engine, "_version", 0), params_key(engine.params), prec)

@@ -103,3 +103,76 @@ def test_reduction_equals_quadratic_loop():
         if not any({c.ia[k], c.ib[k]} == {c.ia[j], c.ib[j]} and np.all(c.p[k] == c.p[j]) for j in keep):
             keep.append(k)
     assert list(r.tri) == keep
+
+
+# ------------------------------------------------------------ Frank-Wolfe hand cases
+# Body 1 owns a tabulated field on a 9^3 grid over [-4, 4]^3 (spacing 1); body
+# 0 owns one triangle whose corners are interior grid nodes.  At a node that
+# is the lower corner of its enclosing cell the local coordinates are 0, so
+# the gradient is the forward difference of the node values (whatever the
+# local coordinates are divided by).  The base field is linear, phi = y - c;
+# a few nodes are moved to set the gradient at chosen corners.  Each case has
+# an endpoint worked out by hand (collision_detection.py:134-165) and names
+# the plausible misreading that would give another endpoint.
+
+def fw_case(P, base, edits):
+    from oracle.grid import Grid, nodes_array
+    g = Grid([-4.0, -4.0, -4.0], [4.0, 4.0, 4.0], 9, 9, 9)
+    nodes = nodes_array(g)
+    vals = nodes[:, 1] - base
+    for (i, j, k), v in edits.items():
+        vals[(i + 4) + 9 * ((j + 4) + 9 * (k + 4))] = v
+    g.values = vals
+    sc = OS.Scene()
+    V = np.array(P, dtype=float)
+    sc.shapes = [OS.ShapeArrays(V, np.array([[0, 1, 2]]), None), OS.ShapeArrays(V, np.array([[0, 1, 2]]), g)]
+    sc.shape = np.array([0, 1])
+    sc.r = np.zeros((2, 3))
+    sc.q = np.tile([1.0, 0.0, 0.0, 0.0], (2, 1))
+    sc.fixed = np.array([False, True])
+    return sc
+
+
+def fw_endpoint(sc):
+    p, n, gap, tri = CD.frank_wolfe_contacts(sc, 0, 1, np.array([0]))
+    s = CD.fw_scalar(sc, 0, 1, 0)
+    assert len(p) == 1 and s is not None
+    assert np.array_equal(p[0], s[0]) and np.array_equal(n[0], s[1]) and gap[0] == s[2]
+    return p[0], n[0], gap[0]
+
+
+def test_fw_step_before_break_and_first_index_ties():
+    # phi = y - 1 except at p2 = (-1, 1, 0), whose gradient is (1, 0, 0): all
+    # three corner gradients have norm 1, so the start is corner 0 (first index);
+    # its gradient (0, 1, 0) gives objectives s.g = (3, 1, 1), a tie won by
+    # corner 1 (first index); the step (alpha = 1) moves x onto corner 1 before
+    # the break test (1 > 0) ends the loop.  Break before the step: (0, 3, 0);
+    # last index on the objective tie: (-1, 1, 0); last index on the start tie:
+    # start at p2, objectives (0, 1, -1), endpoint (-1, 1, 0).
+    sc = fw_case([(0, 3, 0), (1, 1, 0), (-1, 1, 0)], 1.0, {(0, 1, 0): 1.0, (-1, 2, 0): 0.0})
+    p, n, gap = fw_endpoint(sc)
+    assert p.tolist() == [1.0, 1.0, 0.0] and n.tolist() == [0.0, 1.0, 0.0] and gap == 0.0
+
+
+def test_fw_start_at_smallest_gradient_norm():
+    # phi = y - 2; at p2 = (-2, 2, 0) the gradient is (0.2, 0.4, 0) (norm 0.45,
+    # the others 1): the start is p2, objectives (1.2, 0.8, 0.4) pick p2 itself
+    # and 0.4 > 0 ends the loop: the contact is at p2 with gap 0.  Starting at
+    # corner 0 instead: objectives (3, 1, 2) pick p1 = (2, 1, 0), gap -1.
+    sc = fw_case([(0, 3, 0), (2, 1, 0), (-2, 2, 0)], 2.0, {(-1, 2, 0): 0.2, (-2, 3, 0): 0.4})
+    p, n, gap = fw_endpoint(sc)
+    assert p.tolist() == [-2.0, 2.0, 0.0] and gap == 0.0
+    assert np.allclose(n, np.array([0.2, 0.4, 0.0]) / np.sqrt(0.2))
+
+
+def test_fw_objective_is_s_dot_gradient():
+    # phi = y - 1; gradient (0, 0.5, 0) at p2 = (-2, 2, 0) (the start: smallest
+    # norm) and (1, 1, 0) at p1 = (2, 1, 0).  Objectives s.g at p2 are
+    # (1.5, 0.5, 1.0): x steps onto p1 and 0.5 > 0 ends the loop, so the
+    # contact is at p1 with the normal (1, 1, 0)/sqrt(2).  With the objective
+    # (s - x).g the minimum is never positive, the loop goes on, and the
+    # gradient at p1 drags x towards p2 (to (-2/3, 5/3, 0) at the next step).
+    sc = fw_case([(0, 3, 0), (2, 1, 0), (-2, 2, 0)], 1.0, {(-2, 3, 0): 1.5, (3, 1, 0): 1.0})
+    p, n, gap = fw_endpoint(sc)
+    assert p.tolist() == [2.0, 1.0, 0.0] and gap == 0.0
+    assert np.allclose(n, [np.sqrt(0.5), np.sqrt(0.5), 0.0], atol=1e-15)

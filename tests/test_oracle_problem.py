@@ -160,3 +160,120 @@ def test_absolute_and_relative_stopping():
     # sweep 0: x_n = 1 (merit 1); sweep 1: x_n stays 1 (merit 0 < 1e-3) -> exit code 1 at iteration 1
     assert res["exitcode"] == 1 and res["iterations"] == 1
     assert P.sol[0, 0] == pytest.approx(1.0)
+
+
+# ------------------------------------------------------------ natural-map residual
+def one_contact(b, mu, r_scale=1.0):
+    """One contact between a fixed body 0 and a free body 1 (mass 1, inertia
+    2 I, arm 0) with normal +y: A = J W J^T = diag(1, 1, 1, 1/2)."""
+    n = np.array([0.0, 1.0, 0.0])
+    s, t, _ = Q.make_orthonormal_vectors(n)
+    P = PB.Problem()
+    P.K, P.N = 1, 2
+    P.ia, P.ib = np.array([0]), np.array([1])
+    P.JA = np.zeros((1, 4, 6))
+    P.JB = np.zeros((1, 4, 6))
+    P.JB[0, 0, :3], P.JB[0, 1, :3], P.JB[0, 2, :3], P.JB[0, 3, 3:] = n, s, t, n
+    W = np.diag([1.0, 1.0, 1.0, 0.5, 0.5, 0.5])
+    P.WJA = np.zeros((1, 6, 4))
+    P.WJB = (W @ P.JB[0].T)[None]
+    P.b = np.array([b], dtype=float)
+    P.r0 = np.array([[1.0, 1.0, 1.0, 2.0]])
+    P.r = P.r0 * r_scale          # the solve's current r-factors (not used by the residual)
+    P.mu = np.array([mu])
+    P.mu3 = np.array([[mu] * 3])
+    return P
+
+
+def test_natural_residual_zero_at_closed_form_fixed_point():
+    # x_n = -b_n / A_nn = 1.5 with zero friction rows is a fixed point of the prox map
+    P = one_contact([-1.5, 0.0, 0.0, 0.0], 0.2, r_scale=3.0)
+    assert PB.natural_residual(P, PARAM_DEFAULTS, x=np.array([[1.5, 0.0, 0.0, 0.0]])) == 0.0
+
+
+def test_natural_residual_hand_value():
+    # x = (2, 0.9, 0, 0), b = (-1.5, 0.5, 0, 0), mu = 0.2, r0 = 1/diag(A):
+    # z = x - r0 (A x + b) = (1.5, -0.5, 0, 0); prox: y_n = 1.5, friction ball of
+    # radius 0.3 -> y_s = -0.3; |x - y|_inf = 1.2 (row s); / max(1, |x|_inf = 2) = 0.6.
+    # (with the solve's r = 3 r0 instead: 0.75; without the cone: 0.7; unscaled: 1.2)
+    P = one_contact([-1.5, 0.5, 0.0, 0.0], 0.2, r_scale=3.0)
+    x = np.array([[2.0, 0.9, 0.0, 0.0]])
+    assert PB.natural_residual(P, PARAM_DEFAULTS, x=x) == pytest.approx(0.6, abs=1e-15)
+    # the torsion row has A = 1/2, so r0 = 2: z_tau = 0.4 - 2 (0.2 + 0.1) = -0.2, inside the cone
+    P = one_contact([-1.5, 0.0, 0.0, 0.1], 0.5)
+    x = np.array([[2.0, 0.0, 0.0, 0.4]])
+    assert PB.natural_residual(P, PARAM_DEFAULTS, x=x) == pytest.approx(0.3, abs=1e-15)
+
+
+def test_natural_residual_equals_dense_prox_map():
+    # two contacts among three free bodies with random Jacobian blocks and SPD
+    # inverse masses: the residual equals the prox fixed-point map on the dense
+    # A = J W J^T, and vanishes at the converged Gauss-Seidel solution
+    rng = np.random.default_rng(11)
+    P = PB.Problem()
+    P.K, P.N = 2, 3
+    P.ia, P.ib = np.array([0, 1]), np.array([1, 2])
+    P.JA, P.JB = rng.normal(size=(2, 4, 6)), rng.normal(size=(2, 4, 6))
+    Ws = []
+    for _ in range(3):
+        M = rng.normal(size=(6, 6))
+        Ws.append(M @ M.T + 6 * np.eye(6))
+    P.WJA = np.einsum("kij,klj->kil", np.array(Ws)[P.ia], P.JA)
+    P.WJB = np.einsum("kij,klj->kil", np.array(Ws)[P.ib], P.JB)
+    J = np.zeros((8, 18))
+    for k in range(2):
+        J[4 * k:4 * k + 4, 6 * P.ia[k]:6 * P.ia[k] + 6] = P.JA[k]
+        J[4 * k:4 * k + 4, 6 * P.ib[k]:6 * P.ib[k] + 6] = P.JB[k]
+    Wd = np.zeros((18, 18))
+    for i in range(3):
+        Wd[6 * i:6 * i + 6, 6 * i:6 * i + 6] = Ws[i]
+    A = J @ Wd @ J.T
+    P.b = rng.normal(size=(2, 4))
+    P.b[:, 0] = -np.abs(P.b[:, 0]) - 1.0
+    P.r0 = (1.0 / np.diag(A)).reshape(2, 4)
+    P.r = P.r0.copy()
+    P.mu = np.array([0.3, 0.7])
+    P.mu3 = np.repeat(P.mu[:, None], 3, axis=1)
+
+    def dense(x):
+        z = x.ravel() - P.r0.ravel() * (A @ x.ravel() + P.b.ravel())
+        y = np.empty(8)
+        for k in range(2):
+            yn = max(0.0, z[4 * k])
+            zf = z[4 * k + 1:4 * k + 4]
+            nf = np.linalg.norm(zf)
+            y[4 * k] = yn
+            y[4 * k + 1:4 * k + 4] = zf if nf <= P.mu[k] * yn else zf * (P.mu[k] * yn / nf)
+        return np.abs(x.ravel() - y).max() / max(1.0, np.abs(x).max())
+
+    for _ in range(5):
+        x = rng.normal(size=(2, 4))
+        x[:, 0] = np.abs(x[:, 0])
+        assert PB.natural_residual(P, PARAM_DEFAULTS, x=x) == pytest.approx(dense(x), rel=1e-12, abs=1e-15)
+    P.x, P.sol = np.zeros((2, 4)), np.zeros((2, 4))
+    prm = dict(PARAM_DEFAULTS, max_iterations=2000, absolute_tolerance=0.0, relative_tolerance=0.0,
+               too_small_merit_change=0.0)
+    PB.solve(P, COL.serial_groups(2), prm)
+    assert PB.natural_residual(P, prm) < 1e-10 and dense(P.sol) < 1e-10
+
+
+def test_faithful_sweep_equals_blocked_sweep(oracle_bank32):
+    # contacts.py:66-107 on the reference's scipy.sparse structures (a full
+    # J.dot(w) and WJT.tocsr() per contact) against the blocked sweep, serial
+    # and coloured orders, three sweeps each: equal up to summation order
+    from oracle import faithful
+    sc = scene_with_contacts(oracle_bank32, seed=3)
+    c = CD.detect(sc)
+    assert c.K > 4
+    Winv, f_ext, du = PB.body_mass_blocks(sc)
+    prm = dict(sc.params)
+    for groups in (COL.serial_groups(c.K), COL.groups_from_colors(COL.color(c.ia, c.ib, sc.fixed, 5, 0))):
+        P = PB.assemble(sc, c, Winv, du, 0.01)
+        Jm, W, WJT = faithful.sparse_problem(P, Winv)
+        assert np.allclose(Jm.toarray() @ W.toarray(), WJT.toarray().T, atol=1e-14)
+        x = np.zeros(4 * c.K)
+        order = np.concatenate(groups)
+        for _ in range(3):
+            PB.sweep_gs(P, groups, prm)
+            assert faithful.sweep(Jm, WJT, x, P.r.ravel(), P.b.ravel(), P.mu, order) == c.K
+            assert np.allclose(x, P.x.ravel(), rtol=1e-10, atol=1e-12)

@@ -116,3 +116,39 @@ def test_post_stabilization_skips_shallow_contacts(oracle_bank32):
     r0 = sc.r.copy()
     assert OST.post_stabilize(sc, P, groups, sc.params) is None
     assert np.array_equal(sc.r, r0)
+
+
+def test_energies_closed_forms():
+    # state_storage.py:188-237: KE = sum over free bodies of 1/2 (m v.v + w^T I_w w)
+    # with I_w = R diag(I) R^T, PE = sum of m g (up . r) for the summed gravity
+    # vector; fixed bodies count for nothing.  The rotational term is checked in
+    # body coordinates, 1/2 sum_i I_i (R^T w)_i^2, and for a spin about a
+    # principal axis, 1/2 I_1 |w|^2.
+    from oracle import quat as Q
+    rng = np.random.default_rng(5)
+    sc = OS.Scene()
+    sc.r = np.array([[0.0, -3.0, 0.0], [1.0, 2.5, -2.0], [0.5, 7.0, 3.0]])
+    qa = rng.normal(size=4)
+    qa /= np.linalg.norm(qa)
+    sc.q = np.array([[1.0, 0.0, 0.0, 0.0], qa, [np.cos(0.2), 0.0, np.sin(0.2), 0.0]])
+    sc.mass = np.array([50.0, 2.0, 3.0])
+    sc.inertia = np.array([[1.0, 1.0, 1.0], [0.9, 0.5, 0.2], [0.3, 0.3, 0.3]])
+    sc.fixed = np.array([True, False, False])
+    Ra = Q.to_matrix(qa)
+    sc.v = np.array([[5.0, 5.0, 5.0], [0.3, -1.2, 0.7], [0.0, 0.0, 0.0]])
+    w_body = np.array([0.4, -2.0, 1.1])
+    sc.w = np.array([[9.0, 9.0, 9.0], Ra @ w_body, [0.0, 0.0, 0.0]])
+    sc.gravity = [(9.81, np.array([0.0, 1.0, 0.0])), (1.0, np.array([1.0, 0.0, 0.0]))]
+    kin, pot = OST.energies(sc)
+    ke = 0.5 * 2.0 * (0.3 ** 2 + 1.2 ** 2 + 0.7 ** 2) + 0.5 * np.sum(sc.inertia[1] * w_body ** 2)
+    G = np.array([1.0, 9.81, 0.0])
+    pe = 2.0 * G.dot(sc.r[1]) + 3.0 * G.dot(sc.r[2])
+    assert kin == pytest.approx(ke, rel=1e-13) and pot == pytest.approx(pe, rel=1e-13)
+    # spin about the first principal axis only
+    sc.v[1] = 0.0
+    sc.w[1] = Ra[:, 0] * 3.0
+    kin, _ = OST.energies(sc)
+    assert kin == pytest.approx(0.5 * 0.9 * 9.0, rel=1e-13)
+    # no gravity force: no potential energy
+    sc.gravity = []
+    assert OST.energies(sc)[1] == 0.0

@@ -152,6 +152,18 @@ int rbh_memory_usage(rbh_ctx* ctx, unsigned long long* bytes);
 int rbh_sdf_build(rbh_ctx* ctx, const double* V, int nv, const int* T, int nt, const double* min_coord,
                   const double* max_coord, int I, int J, int K, double* values);
 
+/* rbh_sdf_build without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot call hipMalloc (serving loops and harnesses
+ * that forbid it).  Same arguments and the same values, bit for bit; the
+ * inputs and the grid are staged in a fixed 32 MiB device array per GPU
+ * (held until the process ends) instead of buffers allocated per call.  The
+ * staging needs 24 nv + 12 nt + 72 nt + 8 I J K bytes, each part rounded up
+ * to 256; a larger call fails with RBH_E_CAPACITY (e.g. a 128^3 grid does not
+ * fit, a 96^3 grid of 10^4 triangles does).  Every context on one GPU shares
+ * that array: calls that overlap on one GPU (two contexts, two threads) race. */
+int rbh_sdf_build_noalloc(rbh_ctx* ctx, const double* V, int nv, const int* T, int nt, const double* min_coord,
+                          const double* max_coord, int I, int J, int K, double* values);
+
 /* Upload one shape (types.py:82-108): body-frame mesh (at least 2 triangles)
  * and its SDF grid. */
 int rbh_shape_upload(rbh_ctx* ctx, const double* V, int nv, const int* T, int nt, const double* sdf_values, int I,

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,7 +27,33 @@
 
 using namespace rbh;
 
+// Staging array of rbh_sdf_build_noalloc: one copy per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel or a
+// copy (on the host the name is only a stand-in); it uses the device address
+// hipGetSymbolAddress returns, looked up once per device (scratch_address).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `device`'s scratch array; the device must be current.
+hipError_t scratch_address(int device, unsigned char** out) {
+  *out = nullptr;
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 struct DBuf {
   void* p = nullptr;
@@ -154,6 +181,10 @@ struct rbh_ctx {
   double timings[ST_N]{};
   int timed_steps = 0;
   long long counters[4]{};
+  // device address of this device's no-allocation staging array (rbh_sdf_build_noalloc),
+  // looked up once per device when the first context on it is created
+  unsigned char* scratch = nullptr;
+  hipError_t scratch_err = hipSuccess;
 };
 
 namespace {
@@ -1066,6 +1097,9 @@ int rbh_ctx_create(int device, int precision_bytes, unsigned long long hbm_budge
   c->bodies.init(0);
   c->tris.init(0);
   c->res.init(0);
+  // the no-allocation entry points' staging array on this device (device current)
+  c->scratch_err = scratch_address(device, &c->scratch);
+  (void)hipGetLastError();
   *out = c;
   return 0;
 }
@@ -1126,6 +1160,42 @@ int rbh_sdf_build(rbh_ctx* c, const double* V, int nv, const int* T, int nt, con
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(dtri);
   (void)hipFree(dout);
+  if (e != hipSuccess) return fail(c, RBH_E_HIP, std::string("SDF kernel: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int rbh_sdf_build_noalloc(rbh_ctx* c, const double* V, int nv, const int* T, int nt, const double* mn,
+                          const double* mx, int I, int J, int K, double* values) {
+  if (!c || !V || !T || !mn || !mx || !values || nv <= 0 || nt <= 0) return fail(c, RBH_E_ARG, "bad SDF arguments");
+  if (I < 2 || J < 2 || K < 2) return fail(c, RBH_E_ARG, "grid needs at least 2 nodes per axis");
+  for (int i = 0; i < 3; ++i) if (mn[i] > mx[i]) return fail(c, RBH_E_ARG, "min_coord > max_coord");
+  for (int i = 0; i < 3 * nt; ++i) if (T[i] < 0 || T[i] >= nv) return fail(c, RBH_E_ARG, "triangle index out of range");
+  if (!c->scratch)
+    return fail(c, RBH_E_HIP, std::string("no-allocation scratch unavailable: ") + hipGetErrorString(c->scratch_err));
+  // staging layout, each part 256-byte aligned: vertices (3 nv doubles),
+  // triangles (3 nt ints), gathered triangle corners (9 nt doubles), grid (I J K doubles)
+  auto up = [](unsigned long long b) { return (b + 255ull) & ~255ull; };
+  unsigned long long n = (unsigned long long)I * (unsigned long long)J;   // < 2^62
+  n = n > NOALLOC_SCRATCH_BYTES ? NOALLOC_SCRATCH_BYTES : n * (unsigned long long)K;
+  const unsigned long long oV = 0, oT = oV + up(24ull * (unsigned)nv), oTri = oT + up(12ull * (unsigned)nt),
+                           oOut = oTri + up(72ull * (unsigned)nt), need = oOut + up(8ull * n);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(c, RBH_E_CAPACITY, "SDF build needs " + std::to_string(need) + " bytes of staging; the no-allocation "
+                "entry point holds " + std::to_string(NOALLOC_SCRATCH_BYTES) + " (use rbh_sdf_build)");
+  HIPCHK(c, hipSetDevice(c->device));
+  double* dV = reinterpret_cast<double*>(c->scratch + oV);
+  int* dT = reinterpret_cast<int*>(c->scratch + oT);
+  double* dtri = reinterpret_cast<double*>(c->scratch + oTri);
+  double* dout = reinterpret_cast<double*>(c->scratch + oOut);
+  HIPCHK(c, hipMemcpyAsync(dV, V, 24ull * (unsigned)nv, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dT, T, 12ull * (unsigned)nt, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_sdf_gather, dim3(grid_for(9L * nt, 256)), dim3(256), 0, c->stream, dV, dT, nt, dtri);
+  double h[3] = {(mx[0] - mn[0]) / (I - 1), (mx[1] - mn[1]) / (J - 1), (mx[2] - mn[2]) / (K - 1)};
+  hipLaunchKernelGGL(k_sdf, dim3((unsigned)((n + SDF_BLOCK - 1) / SDF_BLOCK)), dim3(SDF_BLOCK), 0, c->stream, dtri, nt,
+                     I, J, K, mn[0], mn[1], mn[2], h[0], h[1], h[2], dout);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(values, dout, 8ull * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, RBH_E_HIP, std::string("SDF kernel: ") + hipGetErrorString(e));
   return 0;
 }

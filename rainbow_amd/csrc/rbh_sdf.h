@@ -70,6 +70,18 @@ __device__ __forceinline__ double solid_angle(double px, double py, double pz, c
   return 2.0 * atan2(det, den);
 }
 
+// tri[9 t + 3 k + a] = V[3 T[3 t + k] + a]: the corners of each triangle in
+// one record (triangle indices checked on the host)
+__global__ void k_sdf_gather(const double* __restrict__ V, const int* __restrict__ T, int nt,
+                             double* __restrict__ tri) {
+  const long n = 9L * nt;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long t = e / 9;
+    const int r = (int)(e - 9 * t);
+    tri[e] = V[3L * T[3 * t + r / 3] + r % 3];
+  }
+}
+
 // tri: 9 doubles per triangle; out[i + I (j + J k)] = signed distance of node (i, j, k)
 __global__ void __launch_bounds__(SDF_BLOCK) k_sdf(const double* __restrict__ tri, int nt, int I, int J, int K,
                                                    double mnx, double mny, double mnz, double hx, double hy, double hz,

@@ -91,6 +91,8 @@ _PROTOS = {
     "rbh_memory_usage": [C.c_void_p, C.POINTER(C.c_ulonglong)],
     "rbh_sdf_build": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                       C.c_int, C.c_int, C.c_int, C.c_void_p],
+    "rbh_sdf_build_noalloc": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                              C.c_int, C.c_int, C.c_int, C.c_void_p],
     "rbh_shape_upload": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
                          C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)],
     "rbh_scene_create": [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_int)],
@@ -209,6 +211,16 @@ class Context:
         out = np.empty(int(I) * int(J) * int(K), dtype=np.float64)
         self._check(self.lib.rbh_sdf_build(self.h, _ptr(V), len(V), _ptr(T), len(T), _ptr(_f64(min_coord, 3)),
                                            _ptr(_f64(max_coord, 3)), int(I), int(J), int(K), _ptr(out)))
+        return out
+
+    def sdf_build_noalloc(self, V, T, min_coord, max_coord, I, J, K):
+        """sdf_build staged in the library's fixed per-GPU array (no device allocation)."""
+        V = _f64(V)
+        T = _i32(T)
+        out = np.empty(int(I) * int(J) * int(K), dtype=np.float64)
+        self._check(self.lib.rbh_sdf_build_noalloc(self.h, _ptr(V), len(V), _ptr(T), len(T),
+                                                   _ptr(_f64(min_coord, 3)), _ptr(_f64(max_coord, 3)),
+                                                   int(I), int(J), int(K), _ptr(out)))
         return out
 
     def shape_upload(self, V, T, values, I, J, K, min_coord, max_coord):

@@ -36,17 +36,21 @@ core.  The host's CPU model and core count are recorded.
 
 batched (on by default, --no-batched to skip): 1024 independent scenes of
 8..512 boxes and spheres (log-uniform counts, mu ~ U[0.1, 0.8], dt = 0.01,
-30 GS sweeps) advanced together by one hipGraph replay per step; scene-steps/s
-and per-step p50/p99 beside the plain path (one scene per launch) on a sample
-of the same scenes, whose end states must equal the batched ones.
+30 GS sweeps) advanced together by one hipGraph replay per step, timed after
+the drops have landed (steps 156..175); scene-steps/s and per-step p50/p99
+beside the plain path (one scene per launch) on a sample of the same scenes
+at the same steps, whose end states must equal the batched ones; the FP64
+oracle on the host's cores (one scene per process) as the CPU baseline.
 
 serving (on by default, --no-serving to skip): Poisson arrivals of
 sphere-drop requests (8..1024 bodies, 50..500 steps) admitted against an HBM
 budget and advanced by continuous batching, 64 requests at 10, 50 and 200
-requests/s, and 256 requests all waiting at t = 0 (saturation); launch shapes
-from the autotuner; p50/p99 per-step latency, completed scene-steps/s,
-admission wait, the plain path (one request per launch) on a sample of the
-requests, and the alone-vs-mixed difference of a few requests.
+requests/s, 256 requests all waiting at t = 0 (saturation), and the 64
+requests waiting under a quarter of their footprint (admission queues);
+launch shapes from the autotuner; p50/p99 per-step latency, completed
+scene-steps/s, admission wait, the plain path (one request per launch) on a
+sample of the requests over their whole lifetimes, and the alone-vs-mixed
+difference of the checked requests.
 
 stress (on by default, --no-stress to skip): 128 noisy icospheres and 128
 torus knots (about 10^4 triangles each, 64^3 SDFs) poured into a fixed funnel,
@@ -192,8 +196,13 @@ def cpu_baseline(eng, state, steps_done, faithful_seconds=10.0):
                 faithful=fa, host=host_info())
 
 
-def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
-    """1024 mixed box/sphere scenes stepped as one batch vs one scene per launch."""
+def batched_leg(n_scenes, steps, settle, plain_sample, plain_steps, cpu=True):
+    """1024 mixed box/sphere scenes stepped as one batch vs one scene per
+    launch, both timed after the drops have landed: the batch first advances
+    `settle` steps untimed (1.5 s at dt = 0.01; the highest drop, 10 m, lands
+    after 1.4 s), then `plain_steps` steps whose end states the plain path
+    must reproduce, then the timed `steps` steps.  The plain path steps the
+    sample scenes from their states at step `settle`, one scene per launch."""
     import numpy as np
     from rainbow_amd import autotune, native, scenes
     from rainbow_amd.stepper import engine_desc, gather_state, shared_context
@@ -212,10 +221,15 @@ def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
     n_bodies = sum(len(e.bodies) for e in engs)
     log("batched: %d scenes, %d bodies built in %.1f s" % (n_scenes, n_bodies, time.perf_counter() - t0))
     sample = list(range(0, n_scenes, max(1, n_scenes // plain_sample)))[:plain_sample]
-    # batch: warm-up steps, then the sample's states for the plain comparison
-    ctx.step(sids, warmup, 0.0, native.STEP_GRAPH)
+    # untimed: the drops land
+    ctx.step(sids, settle, 0.0, native.STEP_GRAPH)
     ctx.sync()
-    ref = {i: ctx.get_state(sids[i], init[i][3]) for i in sample}
+    s_settle = {i: ctx.get_state(sids[i], init[i][3]) for i in sample}
+    ctx.step(sids, plain_steps, 0.0, native.STEP_GRAPH)
+    ctx.sync()
+    s_ref = {i: ctx.get_state(sids[i], init[i][3]) for i in sample}
+    at = settle + plain_steps
+    log("batched: %d steps untimed (landed)" % at)
     # launch shapes picked on this batch (the trials restore every scene's state)
     tuned = autotune.autotune(ctx, sids, [x[3] for x in init], steps=3, log=log)
     for sid in sids:
@@ -224,22 +238,28 @@ def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
     ctx.step(sids, steps, 0.0, native.STEP_GRAPH)
     ctx.sync()
     tb = time.perf_counter() - t1
+    tot = [ctx.scene_totals(sid) for sid in sids]
     lat = []
     for _ in range(10):
         t2 = time.perf_counter()
         ctx.step(sids, 1, 0.0, native.STEP_GRAPH)
         ctx.sync()
         lat.append(time.perf_counter() - t2)
-    citers = sum(ctx.scene_totals(sid)["contact_iters"] for sid in sids)
-    # plain path: the sample scenes again from their initial state, one scene per launch
+    citers = sum(t["contact_iters"] for t in tot)
+    contacts = sum(t["contacts"] for t in tot)
+    res = np.array([ctx.get_log(sid, 30)["natural_residual"] for sid in sids])
+    end_states = {i: ctx.get_state(sids[i], init[i][3]) for i in range(n_scenes)} if cpu else None
+    # plain path: the sample scenes from their states at step `settle`, one scene per launch
     dup = []
     for i in sample:
         desc, keep, st0, n = init[i]
         sid = ctx.scene_create(desc)
-        ctx.set_state(sid, *st0)
+        ctx.set_state(sid, *s_settle[i])
+        ctx.set_step_index(sid, settle)
+        ctx.step([sid], 1, 0.0, 0)           # first launch of each scene (workspace sizing), untimed
+        ctx.set_state(sid, *s_settle[i])
+        ctx.set_step_index(sid, settle)
         dup.append(sid)
-    for sid in dup:                      # first launch of each scene (workspace sizing) untimed
-        ctx.step([sid], 1, 0.0, 0)
     ctx.sync()
     t3 = time.perf_counter()
     for _ in range(plain_steps):
@@ -247,14 +267,10 @@ def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
             ctx.step([sid], 1, 0.0, 0)
     ctx.sync()
     tp = time.perf_counter() - t3
-    # equal results: the plain runs took 1 + plain_steps steps; compare at equal step counts
     diff = 0.0
-    if 1 + plain_steps == warmup:
-        for i, sid in zip(sample, dup):
-            got = ctx.get_state(sid, init[i][3])
-            diff = max(diff, max(float(np.abs(a - b).max()) for a, b in zip(got, ref[i])))
-    else:
-        diff = None
+    for i, sid in zip(sample, dup):
+        got = ctx.get_state(sid, init[i][3])
+        diff = max(diff, max(float(np.abs(a - b).max()) for a, b in zip(got, s_ref[i])))
     for sid in dup:
         ctx.scene_destroy(sid)
     for sid in sids:
@@ -262,15 +278,83 @@ def batched_leg(n_scenes, steps, warmup, plain_sample, plain_steps):
     ctx.set_tuning(**autotune.DEFAULT)
     batched = n_scenes * steps / tb
     plain = len(dup) * plain_steps / tp
-    return {"workload": "mixed_drop_%d_scenes" % n_scenes, "bodies": n_bodies, "steps": steps,
-            "scene_steps_per_s": batched, "ms_per_step": 1e3 * tb / steps,
-            "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
-            "contact_iters_per_s": citers / tb if tb > 0 else 0.0,
-            "plain": {"scene_steps_per_s": plain, "sample_scenes": len(dup), "steps": plain_steps,
-                      "ms_per_scene_step": 1e3 * tp / (len(dup) * plain_steps)},
-            "speedup_vs_plain": batched / plain if plain > 0 else None,
-            "max_abs_state_diff_vs_plain": diff,
-            "tuning": tuning_report(tuned)}
+    out = {"workload": "mixed_drop_%d_scenes" % n_scenes, "bodies": n_bodies, "steps": steps,
+           "steps_timed": "%d..%d" % (at + 1, at + steps),
+           "scene_steps_per_s": batched, "ms_per_step": 1e3 * tb / steps,
+           "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
+           "contact_iters_per_s": citers / tb if tb > 0 else 0.0,
+           "contacts_per_scene_step": contacts / (n_scenes * steps),
+           "natural_residual_last_step": {"median": float(np.median(res)), "max": float(res.max())},
+           "plain": {"scene_steps_per_s": plain, "sample_scenes": len(dup), "steps": plain_steps,
+                     "steps_timed": "%d..%d" % (settle + 1, at),
+                     "ms_per_scene_step": 1e3 * tp / (len(dup) * plain_steps)},
+           "speedup_vs_plain": batched / plain if plain > 0 else None,
+           "max_abs_state_diff_vs_plain": diff,
+           "tuning": tuning_report(tuned)}
+    if cpu:
+        log("batched: cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline_scenes(engs, end_states, at + steps + 10, "mixed_drop")
+    return out
+
+
+def _oracle_scene_steps(job):
+    """Worker: one oracle step of each (scene, dt) in job on one thread; seconds per scene."""
+    from threadpoolctl import threadpool_limits
+    from oracle import stepper as OST
+    out = []
+    with threadpool_limits(1):
+        for sc, dt in job:
+            t0 = time.perf_counter()
+            OST.step(sc, dt)
+            out.append(time.perf_counter() - t0)
+    return out
+
+
+def cpu_baseline_scenes(engs, states, step_index, what, seconds=15.0):
+    """The FP64 oracle's step (O(K) vectorised sweep) of independent scenes
+    from the GPU's states: on one core over the first scenes for about
+    `seconds`, then one scene per process on every usable core over about
+    `seconds` of work per core."""
+    import multiprocessing as mp
+    from oracle import scene as OS
+    hi = host_info()
+
+    def oracle_scene(i):
+        e = engs[i]
+        r, q, v, w = states[i]
+        for k, b in enumerate(e.bodies.values()):
+            if not b.is_fixed:
+                b.r, b.q, b.v, b.w = r[k].copy(), q[k].copy(), v[k].copy(), w[k].copy()
+        sc = OS.from_engine(e)
+        sc.step_index = step_index
+        return sc, e.params.time_step
+
+    # one core: scenes in order until `seconds` pass
+    one = []
+    t0 = time.perf_counter()
+    for i in range(len(engs)):
+        one += _oracle_scene_steps([oracle_scene(i)])
+        if time.perf_counter() - t0 > seconds:
+            break
+    per_scene = float(np.mean(one))
+    # all cores: enough scenes for about `seconds` per core, one process per core (fork: the workers
+    # only run NumPy and leave with os._exit, so they never touch the GPU)
+    cores = hi["usable_cores"]
+    n_all = min(len(engs), max(cores, int(cores * seconds / max(per_scene, 1e-3))))
+    jobs = [oracle_scene(i) for i in range(n_all)]
+    chunks = [jobs[k::cores] for k in range(cores) if jobs[k::cores]]
+    t1 = time.perf_counter()
+    with mp.get_context("fork").Pool(len(chunks)) as pool:
+        res = pool.map(_oracle_scene_steps, chunks)
+    wall = time.perf_counter() - t1
+    done = sum(len(r) for r in res)
+    return {"value": done / wall, "unit": "scene-steps/s", "cores": len(chunks), "kind": "port",
+            "sample": "one FP64 oracle step (NumPy, O(K) sweep) of each of %d %s scenes from the GPU states after "
+                      "step %d, one scene per process on %d cores, %.1f s wall" % (done, what, step_index, len(chunks),
+                                                                               wall),
+            "single_core": {"value": 1.0 / per_scene, "unit": "scene-steps/s", "cores": 1,
+                            "sample": "%d scenes in order, %.1f s" % (len(one), sum(one))},
+            "host": hi}
 
 
 def tuning_report(res):
@@ -279,11 +363,13 @@ def tuning_report(res):
             "trials": [[keep(t), round(ms, 4)] for t, ms in res["trials"]]}
 
 
-def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, plain_steps=10, sat_requests=256):
+def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, sat_requests=256):
     """Poisson arrivals served by continuous batching, one run per arrival rate
     on the same n_req requests, and a saturation run (every one of
     sat_requests requests waiting at t = 0; the first n_req are the same
-    requests); the plain path (one request per launch, no graph) on a sample.
+    requests); the n_req requests again all waiting under a budget of a
+    quarter of their footprint, so admission queues; the plain path (one
+    request per launch, no graph) on a sample, each over its whole lifetime.
     Launch shapes come from the autotuner, run on the 32 largest requests
     after 300 steps (when their bodies have piled up)."""
     import numpy as np
@@ -326,30 +412,48 @@ def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, plain_st
     for r in reqs[:alone_checks]:
         alone = srv.run_alone(r)
         diffs.append(max(float(np.abs(a - b).max()) for a, b in zip(alone, r.state)))
-    # plain path: a sample of the requests, each stepped alone (one scene per launch, no graph)
+    # a budget that binds: the first n_req requests all waiting at t = 0 under a quarter of their footprint
+    sub = reqs[:n_req]
+    for r in sub:
+        r.arrival = 0.0
+    full_budget = srv.budget
+    srv.budget = max(max(r.footprint for r in sub), sum(r.footprint for r in sub) // 4)
+    rep = srv.run(sub)
+    rep["rate_per_s"] = "saturation, budget 1/4 of the requests' footprint"
+    srv.budget = full_budget
+    bound = rep
+    log("serving under a binding budget (%d of %d bytes): %.0f scene-steps/s, max %d active, wait p99 %.1f ms" % (
+        bound["budget_bytes"], sum(r.footprint for r in sub), bound["scene_steps_per_s"], bound["max_active"],
+        bound["p99_wait_ms"]))
+    # plain path: a sample of the requests, each alone over its whole lifetime (one scene per launch, no
+    # graph, one step per call as a serving loop without batching would take them)
     sample = reqs[:n_req][::max(1, n_req // plain_sample)][:plain_sample]
-    sids = []
+    tp, psteps = 0.0, 0
     for r in sample:
         sid = ctx.scene_create(r._desc)
         ctx.set_state(sid, *r._state0)
         ctx.step([sid], 1, 0.0, 0)               # workspace sizing, untimed
-        sids.append(sid)
-    ctx.sync()
-    t1 = time.perf_counter()
-    for _ in range(plain_steps):
-        for sid in sids:
+        ctx.set_state(sid, *r._state0)
+        ctx.set_step_index(sid, 0)
+        ctx.sync()
+        t1 = time.perf_counter()
+        for _ in range(r.n_steps):
             ctx.step([sid], 1, 0.0, 0)
-            ctx.sync()
-    tp = time.perf_counter() - t1
-    for sid in sids:
+        ctx.sync()
+        tp += time.perf_counter() - t1
+        psteps += r.n_steps
+        got = ctx.get_state(sid, len(r.engine.bodies))
+        diffs.append(max(float(np.abs(a - b).max()) for a, b in zip(got, r.state)))
         ctx.scene_destroy(sid)
     ctx.set_tuning(**autotune.DEFAULT)
-    plain = len(sids) * plain_steps / tp
+    plain = psteps / tp
     sat = runs[-1]
     return {"workload": "sphere_drop_requests", "requests": n_req, "saturation_requests": sat_requests,
             "budget_bytes": budget_gib << 30, "tuning": tuning_report(tuned), "runs": runs,
-            "plain": {"scene_steps_per_s": plain, "sample_requests": len(sids), "steps": plain_steps,
-                      "ms_per_scene_step": 1e3 * tp / (len(sids) * plain_steps)},
+            "binding_budget": bound,
+            "plain": {"scene_steps_per_s": plain, "sample_requests": len(sample), "scene_steps": psteps,
+                      "note": "each sampled request alone over its whole lifetime, one step per launch",
+                      "ms_per_scene_step": 1e3 * tp / psteps},
             "speedup_vs_plain_at_saturation": sat["scene_steps_per_s"] / plain if plain > 0 else None,
             "max_abs_diff_alone_vs_mixed": max(diffs) if diffs else None, "alone_checked": len(diffs)}
 
@@ -480,7 +584,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.only == "batched":
-        print(json.dumps(batched_leg(args.batched_scenes, 20, 3, 64, 2)), flush=True)
+        print(json.dumps(batched_leg(args.batched_scenes, 20, 150, 64, 5, cpu=not args.no_cpu)), flush=True)
         return
     rates = [float(x) for x in args.serving_rates.split(",") if x]
     if args.only == "serving":
@@ -586,7 +690,7 @@ def main():
         result["cpu_baseline"] = None
     if not args.no_batched:
         log("batched leg ...")
-        result["batched"] = batched_leg(args.batched_scenes, 20, 3, 64, 2)
+        result["batched"] = batched_leg(args.batched_scenes, 20, 150, 64, 5, cpu=not args.no_cpu)
         log("batched: %.0f scene-steps/s (plain %.0f)" % (result["batched"]["scene_steps_per_s"],
                                                           result["batched"]["plain"]["scene_steps_per_s"]))
     if not args.no_serving:

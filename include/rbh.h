@@ -205,7 +205,10 @@ int rbh_step(rbh_ctx* ctx, const int* scene_ids, int n_scenes, int n_steps, doub
 int rbh_sync(rbh_ctx* ctx);
 
 /* Log of the scene's last step; residuals (cap entries) receives the per-sweep
- * merit array of gauss_seidel_solver.py:31-33,76 (inf past the last sweep). */
+ * merit array of gauss_seidel_solver.py:31-33,76 (inf past the last sweep) in
+ * entries [0, max_iterations), and with cap >= 2 max_iterations the
+ * post-stabilisation solve's ('post_stabilization_residuals', steppers.py:33-34)
+ * in [max_iterations, 2 max_iterations) (inf when the step ran none). */
 int rbh_get_log(rbh_ctx* ctx, int scene_id, rbh_step_log* log, double* residuals, int cap);
 
 /* Contacts of the scene's last step in the reference's order
@@ -219,12 +222,16 @@ int rbh_get_contacts(rbh_ctx* ctx, int scene_id, int cap, int* count, int* ia, i
 /* GPU milliseconds per stage, averaged over the steps of the last rbh_step
  * call made with RBH_STEP_TIMING (its last 512 steps at most):
  * [0] body prep + refit (update_bvh), [1] broad phase + candidates (narrow_phase),
- * [2] Frank-Wolfe + reduction (contact_determination), [3] assembly,
+ * [2] Frank-Wolfe, whose kernel also moves the triangles into the SDF's body
+ * frame and writes the contact points (contact_determination), [3] assembly,
  * [4] colouring + order, [5] solve (solver_time), [6] integration,
- * [7] total (stepper_time).  With RBH_STEP_GRAPH only [7] is measured (per
- * replay).  The stage names in parentheses are the profile keys of
- * steppers.py:55-100 and collision_detection.py:10-80, 204-252.  Returns the
- * number of entries written. */
+ * [7] total (stepper_time), [8] exact-position reduction and contact order
+ * (contact_point_reduction), [9] post-stabilisation rows, solve and position
+ * update (post_stabilization_solver_time; 0 when none ran).  With
+ * RBH_STEP_GRAPH only [7] is measured (per replay).  The stage names in
+ * parentheses are the profile keys of steppers.py:55-100,
+ * collision_detection.py:10-83, 193-201, 204-290 and
+ * gauss_seidel_solver.py:98-101.  Returns the number of entries written. */
 int rbh_get_timings(rbh_ctx* ctx, double* ms, int cap);
 
 /* Batch-level counters of the last step: [0] body pairs, [1] candidate slots,

@@ -109,7 +109,13 @@ struct SceneHost {
   int batch_index = -1;    // index inside that batch
 };
 
-enum Stage { ST_BEGIN = 0, ST_PREP, ST_BROAD, ST_FW, ST_ASM, ST_COLOR, ST_SOLVE, ST_INT, ST_N };
+// Stage boundaries of a step (events recorded after each stage):
+// PREP body prep + refit, BROAD broad phase, FWK Frank-Wolfe, FW reduction
+// and contact order, ASM assembly, COLOR colouring and sweep order, SOLVE the
+// solve, INTG integration, POST post-stabilisation, INT scene records.
+enum Stage { ST_BEGIN = 0, ST_PREP, ST_BROAD, ST_FWK, ST_FW, ST_ASM, ST_COLOR, ST_SOLVE, ST_INTG, ST_POST, ST_INT, ST_N };
+// rbh_get_timings entries
+constexpr int TIMING_OUT = 10;
 
 }  // namespace
 
@@ -139,7 +145,8 @@ struct rbh_ctx {
   // scenes
   std::vector<SceneHost> scenes;
   std::vector<SceneParams> sp_host;
-  std::vector<double> mat_host;
+  std::vector<double> mat_host;   // interaction tables of the live scenes (4 per entry), ranges from `mats`
+  RangeAlloc mats;                // entries of mat_host: a scene's nmat^2 block, reused after destroy
   DBuf d_sp, d_ss, d_mat, d_res;
   bool scenes_dirty = true;
   long long arena_gen = 0;
@@ -178,7 +185,7 @@ struct rbh_ctx {
   // timing
   // timing: ST_N events per step, for up to TIMED_STEPS steps of one call
   std::vector<hipEvent_t> evp;
-  double timings[ST_N]{};
+  double timings[TIMING_OUT]{};
   int timed_steps = 0;
   long long counters[4]{};
   // device address of this device's no-allocation staging array (rbh_sdf_build_noalloc),
@@ -691,6 +698,7 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_cand_write<R>, dim3(grid_for(2L * c->cap_pairs * 64, 256)), dim3(256), 0, s, d);
   mark(ST_BROAD);
   hipLaunchKernelGGL(k_frank_wolfe<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
+  mark(ST_FWK);
   scan<unsigned char>(c, d.cand_keep, d.keep_scan, d.n_cand, c->cap_cand, nullptr);
   hipLaunchKernelGGL(k_kept_list<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_dedup<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
@@ -741,6 +749,7 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hinge_pass<R>(c, d, 1);
   mark(ST_SOLVE);
   hipLaunchKernelGGL(k_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
+  mark(ST_INTG);
   if (c->any_post) {
     // post-stabilisation (steppers.py:90-91): the same solve on the rewritten
     // rows, then a position update with dt = 1
@@ -761,6 +770,7 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
     }
     hipLaunchKernelGGL(k_post_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
   }
+  mark(ST_POST);
   hipLaunchKernelGGL(k_scene_finalize<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
   mark(ST_INT);
   return hipGetLastError();
@@ -925,20 +935,22 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
   for (int i = 0; i < 4; ++i) c->counters[i] = h[i];
   if (timing) {
     // average over the timed steps of the last attempt; a graph replay has no stage marks
-    for (int i = 0; i < ST_N; ++i) c->timings[i] = 0.0;
+    for (int i = 0; i < TIMING_OUT; ++i) c->timings[i] = 0.0;
+    // stage interval -> rbh_get_timings entry
+    static const int out_of[ST_N] = {-1, 0, 1, 2, 8, 3, 4, 5, 6, 9, 6};
     for (int k = 0; k < c->timed_steps; ++k) {
       hipEvent_t* ev = &c->evp[(size_t)k * ST_N];
       if (!use_graph)
         for (int i = 1; i < ST_N; ++i) {
           float ms = 0;
           HIPCHK(c, hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
-          c->timings[i - 1] += ms;
+          c->timings[out_of[i]] += ms;
         }
       float tot = 0;
       HIPCHK(c, hipEventElapsedTime(&tot, ev[ST_BEGIN], ev[ST_INT]));
-      c->timings[ST_N - 1] += tot;
+      c->timings[7] += tot;
     }
-    for (int i = 0; i < ST_N; ++i) c->timings[i] /= std::max(1, c->timed_steps);
+    for (int i = 0; i < TIMING_OUT; ++i) c->timings[i] /= std::max(1, c->timed_steps);
   }
   for (size_t i = 0; i < c->batch.size(); ++i) {
     c->scenes[c->batch[i]].ws_gen = c->ws_gen;
@@ -1097,6 +1109,7 @@ int rbh_ctx_create(int device, int precision_bytes, unsigned long long hbm_budge
   c->bodies.init(0);
   c->tris.init(0);
   c->res.init(0);
+  c->mats.init(0);
   // the no-allocation entry points' staging array on this device (device current)
   c->scratch_err = scratch_address(device, &c->scratch);
   (void)hipGetLastError();
@@ -1273,7 +1286,7 @@ int rbh_scene_footprint(rbh_ctx* c, const rbh_scene_desc* d, unsigned long long*
   long long b = (long long)d->n_bodies * (19 * V + 5 * 4 + c->prec) + nt * 2 * V;
   long long ws_contacts = 64LL * d->n_bodies;
   long long ws = ws_contacts * (16 * V + PALW * 4 + 64) + 8 * nt * 5 + 4 * nt * (2 * V + 13) + (long long)d->n_bodies * 64;
-  *bytes = (unsigned long long)(b + ws + 8LL * d->params.max_iterations);
+  *bytes = (unsigned long long)(b + ws + 16LL * std::max(1, d->params.max_iterations));
   return 0;
 }
 
@@ -1312,7 +1325,7 @@ int rbh_scene_create(rbh_ctx* c, const rbh_scene_desc* d, int* scene_id) {
     if ((rc = grow_tris(c, c->tris.used() + s.ntri))) return rc;
     s.tri_off = c->tris.alloc(s.ntri);
   }
-  s.res_n = std::max(1, d->params.max_iterations);
+  s.res_n = 2 * std::max(1, d->params.max_iterations);   // contact solve, then post-stabilisation solve
   s.res_off = c->res.alloc(s.res_n);
   if (s.res_off < 0) {
     if ((rc = grow_res(c, c->res.used() + s.res_n))) return rc;
@@ -1320,9 +1333,15 @@ int rbh_scene_create(rbh_ctx* c, const rbh_scene_desc* d, int* scene_id) {
   }
   if (s.body_off < 0 || s.tri_off < 0 || s.res_off < 0) return fail(c, RBH_E_MEMORY, "arena allocation failed");
   s.nmat = nmat;
-  s.mat_off = (int)(c->mat_host.size() / 4);
+  s.mat_off = c->mats.alloc(nmat * nmat);
+  if (s.mat_off < 0) {
+    c->mats.grow(std::max(c->mats.used() + nmat * nmat, 2 * c->mats.cap));
+    c->mat_host.resize((size_t)c->mats.cap * 4, 0.0);
+    s.mat_off = c->mats.alloc(nmat * nmat);
+  }
   for (int i = 0; i < nmat * nmat; ++i)
-    for (int k = 0; k < 4; ++k) c->mat_host.push_back(d->interactions ? d->interactions[4 * i + k] : (k < 3 ? 1.0 : 0.0));
+    for (int k = 0; k < 4; ++k)
+      c->mat_host[4 * ((size_t)s.mat_off + i) + k] = d->interactions ? d->interactions[4 * i + k] : (k < 3 ? 1.0 : 0.0);
   s.params = d->params;
   s.live = true;
   int slot = -1;
@@ -1355,6 +1374,7 @@ int rbh_scene_destroy(rbh_ctx* c, int id) {
   c->bodies.release(s.body_off, s.nbody);
   c->tris.release(s.tri_off, s.ntri);
   c->res.release(s.res_off, s.res_n);
+  c->mats.release(s.mat_off, s.nmat * s.nmat);
   s.live = false;
   s.h_parent.clear(); s.h_child.clear(); s.h_geo.clear();
   c->hinges_dirty = true;
@@ -1402,9 +1422,9 @@ int rbh_scene_set_params(rbh_ctx* c, int id, const rbh_params* p) {
   int rc = check_params(c, *p);
   if (rc) return rc;
   SceneHost& s = c->scenes[id];
-  if (std::max(1, p->max_iterations) > s.res_n) {
+  if (2 * std::max(1, p->max_iterations) > s.res_n) {
     c->res.release(s.res_off, s.res_n);
-    s.res_n = std::max(1, p->max_iterations);
+    s.res_n = 2 * std::max(1, p->max_iterations);
     s.res_off = c->res.alloc(s.res_n);
     if (s.res_off < 0) {
       if ((rc = grow_res(c, c->res.used() + s.res_n))) return rc;
@@ -1491,12 +1511,19 @@ int rbh_get_log(rbh_ctx* c, int id, rbh_step_log* log, double* residuals, int ca
     log->post_exitcode = st.post_active ? st.post_exit : -1;
   }
   if (residuals && cap > 0) {
-    int m = std::min(cap, s.params.max_iterations);
+    // [0, mi): the contact solve; [mi, 2 mi): the post-stabilisation solve of this step, if it ran
+    const int mi = std::max(1, s.params.max_iterations);
+    const int m = std::min(cap, s.params.max_iterations);
     if (m > 0)
       HIPCHK(c, hipMemcpyAsync(residuals, reinterpret_cast<double*>(c->d_res.p) + s.res_off, m * 8,
                                hipMemcpyDeviceToHost, c->stream));
+    const int mp = (st.post_active && s.params.use_post_stabilization) ? std::min(cap - mi, s.params.max_iterations) : 0;
+    if (mp > 0)
+      HIPCHK(c, hipMemcpyAsync(residuals + mi, reinterpret_cast<double*>(c->d_res.p) + s.res_off + mi, mp * 8,
+                               hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int i = m; i < cap; ++i) residuals[i] = INFINITY;
+    for (int i = m; i < std::min(cap, mi); ++i) residuals[i] = INFINITY;
+    for (int i = mi + std::max(mp, 0); i < cap; ++i) residuals[i] = INFINITY;
   }
   return 0;
 }
@@ -1516,7 +1543,7 @@ int rbh_get_contacts(rbh_ctx* c, int id, int cap, int* count, int* ia, int* ib, 
 
 int rbh_get_timings(rbh_ctx* c, double* ms, int cap) {
   if (!c || !ms) return RBH_E_ARG;
-  int m = std::min(cap, (int)ST_N);
+  int m = std::min(cap, TIMING_OUT);
   for (int i = 0; i < m; ++i) ms[i] = c->timings[i];
   return m;
 }

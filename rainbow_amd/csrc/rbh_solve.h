@@ -1039,6 +1039,13 @@ template <typename R> __device__ __forceinline__ double hinge_merit(const Dev<R>
   return (sp.h_cnt > 0 && !d.post) ? d.h_mer[sp.res_off + it] : 0.0;
 }
 
+// the scene's per-sweep merit record: the contact solve's 'residuals', then
+// the post-stabilisation solve's 'post_stabilization_residuals'
+// (gauss_seidel_solver.py:29-33, 76 with prefix; max_iter entries each)
+template <typename R> __device__ __forceinline__ double* residual_record(const Dev<R>& d, const SceneParams& sp) {
+  return d.residuals + sp.res_off + (d.post ? max(1, sp.max_iter) : 0);
+}
+
 template <typename R> __device__ void write_solve_state(const Dev<R>& d, SceneState& st, const SolveCtl& c) {
   if (d.post) {   // the post-stabilisation solve keeps the contact solve's record
     st.post_iters = c.iters;
@@ -1243,7 +1250,7 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
   const ProxParams pp(sp);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
-  double* res = d.residuals + sp.res_off;
+  double* res = residual_record(d, sp);
   SolveCtl ctl;
   ctl.iters = sp.max_iter;
   for (int it = 0; it < sp.max_iter; ++it) {
@@ -1267,7 +1274,7 @@ __device__ void solve_group(const Dev<R>& d, const int slot, const Group& gr, V4
     }
     double merit = block_max(m, sh_red);
     if (COOP && !grid_merit(d.merit, it, merit, d.err, s_ok, sh_red)) return;
-    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it,
+    const int dec = ctl.decide(sp, merit, (gr.lead && threadIdx.x == 0) ? res : nullptr, it,
                                hinge_merit(d, sp, it));
     // accepted sweep: ws <- ww, else ww <- ws
     for (int i = threadIdx.x; i < nbody; i += blockDim.x) {
@@ -1344,7 +1351,8 @@ template <typename R> struct QuadRow {
 };
 
 // Velocity changes and inverse masses of a group's bodies in LDS (slot
-// zs stays zero and stands for every fixed body).
+// zs stays zero and stands for every fixed body; slot ts = zs + 1 takes the
+// discarded velocity updates of fixed bodies, so no update branches on them).
 template <typename R> struct BodyLds {
   V4<R> *wl, *wa, *sl, *sa, *W0, *W1;
 };
@@ -1352,12 +1360,13 @@ template <typename R> struct BodyLds {
 // One contact by one quad; lane q holds row q.  Every lane repeats the serial
 // update's operations (update_contact) on the same operands in the same
 // order, so the results are bitwise the same.  Writes this lane's velocity
-// block; returns the new iterate in xq and |xq - xo| in FP64.
+// block (to slot ts for a fixed body); returns the new iterate in xq and
+// |xq - xo| in FP64.
 // JL/JA/JB + 4 i: the contact's rows (LDS or global); this lane's velocity
 // block is built from rows 0..3 of one of them, loaded first.
 template <typename R, typename F>
 __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<R>& it, int q, R rs, bool jacobi,
-                                            bool hybrid, const BodyLds<R>& B, int zs, const V4<R>* JL,
+                                            bool hybrid, const BodyLds<R>& B, int zs, int ts, const V4<R>* JL,
                                             const V4<R>* JA, const V4<R>* JB, size_t i4, R& xq, F&& after_loads) {
   const V4<R>* rl = jacobi ? B.sl : B.wl;
   const V4<R>* ra = jacobi ? B.sa : B.wa;
@@ -1399,8 +1408,9 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
   xq = side_b ? x23 : x01;
   const R dq = xq - xo;
   const R d0 = quad_bcast<0>(dq), d1 = quad_bcast<1>(dq), d2 = quad_bcast<2>(dq), d3 = quad_bcast<3>(dq);
-  // w += W J^T dx (contacts.py:107), this lane's block
-  if (own) {
+  // w += W J^T dx (contacts.py:107), this lane's block; a fixed body's
+  // (zero W, zero block) lands in the discard slot
+  {
     const R sign = side_b ? R(1) : R(-1);
     const R s0 = sign * d0, s1 = sign * d1, s2 = sign * d2, s3 = sign * d3;
     V3<R> acc = {sum_first(v0.x * s0), sum_first(v0.y * s0), sum_first(v0.z * s0)};
@@ -1415,7 +1425,7 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
     } else {
       nw = V4<R>{madd(W1.z, acc.x, old.x), madd(W1.z, acc.y, old.y), madd(W1.z, acc.z, old.z), R(0)};
     }
-    dst[slot] = nw;
+    dst[own ? slot : ts] = nw;
   }
   return fabs((double)xq - (double)xo);
 }
@@ -1460,11 +1470,14 @@ __device__ __forceinline__ void quad_finish_contact(const ProxParams& pp, const 
 // JA = (angA_r, r0_r), JB = (angB_r, mu_r), mu_1..3 the friction
 // coefficients) and the iterates X0/X1[4i + r] are staged in LDS once per
 // solve; so are the group's bodies.  A sweep issues no global memory access.
-// Body arrays hold cb + 1 entries: slot cb stays zero and stands for every
-// fixed body, so a contact's loads never branch on fixed bodies.
+// Body arrays hold cb + 2 entries: slot cb stays zero and stands for every
+// fixed body, slot cb + 1 takes the discarded updates of fixed bodies, so a
+// contact's loads and stores never branch on fixed bodies.  Contact entry cc
+// is the null contact (zero rows and iterates, both bodies in the zero slot):
+// a quad with no contact in a colour updates it, which changes nothing.
 template <typename R> __host__ __device__ constexpr size_t res_lds_bytes(int cc, int cb, int co, int cl) {
-  return (size_t)cc * (12 * sizeof(V4<R>) + 8 * sizeof(R) + sizeof(int2)) + (size_t)(cb + 1) * 6 * sizeof(V4<R>) +
-         (size_t)co * 4 + (size_t)cl * 16;
+  return (size_t)(cc + 1) * (12 * sizeof(V4<R>) + 8 * sizeof(R) + sizeof(int2)) +
+         (size_t)(cb + 2) * 6 * sizeof(V4<R>) + (size_t)co * 4 + (size_t)cl * 16;
 }
 
 constexpr int RES_LANES = 64;   // lanes of a resident group
@@ -1474,22 +1487,24 @@ constexpr int RES_LANES = 64;   // lanes of a resident group
 template <typename R> struct ResCap {
   static constexpr int C = sizeof(R) == 4 ? 384 : 192;   // contacts
   static constexpr int B = C;                             // bodies
-  static constexpr int O = 4 * C;                         // colour offsets
+  static constexpr int O = 4 * C;                         // colour offsets (nc + 3 per lane)
   static constexpr size_t BYTES = res_lds_bytes<R>(C, B, O, RES_LANES);
-  // the streaming layout keeps only bodies in the same LDS
-  static constexpr int VB = (int)(BYTES / (6 * sizeof(V4<R>))) - 1;
+  // the streaming layout keeps only bodies (and the zero and discard slots) in the same LDS
+  static constexpr int VB = (int)(BYTES / (6 * sizeof(V4<R>))) - 2;
 };
 
 template <typename R> struct ResLds {
   V4<R> *JL, *JA, *JB;                     // per contact row (4 per contact)
   R *x0, *x1;                              // per contact row
-  BodyLds<R> B;                            // per body (slot - b_lo), zero slot zs
+  BodyLds<R> B;                            // per body (slot - b_lo), zero slot zs, discard slot ts
   int2* loc;                               // a contact's two bodies (slot - b_lo; zs: fixed)
   int4* lt;                                // per lane: first contact, K, colours, first colour offset
   int* coff;
   static constexpr int zs = ResCap<R>::B;
+  static constexpr int ts = ResCap<R>::B + 1;
+  static constexpr int nul = ResCap<R>::C;
   __device__ explicit ResLds(unsigned char* p) {
-    constexpr int cc = ResCap<R>::C, nbs = ResCap<R>::B + 1;
+    constexpr int cc = ResCap<R>::C + 1, nbs = ResCap<R>::B + 2;
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
     JL = v; JA = JL + 4 * cc; JB = JA + 4 * cc;
     B.wl = JB + 4 * cc; B.wa = B.wl + nbs; B.sl = B.wa + nbs; B.sa = B.sl + nbs; B.W0 = B.sa + nbs; B.W1 = B.W0 + nbs;
@@ -1504,8 +1519,9 @@ template <typename R> struct ResLds {
 template <typename R> struct StreamLds {
   BodyLds<R> B;
   static constexpr int zs = ResCap<R>::VB;
+  static constexpr int ts = ResCap<R>::VB + 1;
   __device__ explicit StreamLds(unsigned char* p) {
-    constexpr int nbs = ResCap<R>::VB + 1;
+    constexpr int nbs = ResCap<R>::VB + 2;
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
     B.wl = v; B.wa = B.wl + nbs; B.sl = B.wa + nbs; B.sa = B.sl + nbs; B.W0 = B.sa + nbs; B.W1 = B.W0 + nbs;
   }
@@ -1522,7 +1538,7 @@ template <typename R> __device__ GroupSize group_size(const Dev<R>& d, const Gro
     if (e <= a) continue;
     g.p0 = min(g.p0, a);
     p1 = max(p1, e);
-    g.no += serial ? 0 : d.l_ncol[l] + 1;
+    g.no += serial ? 0 : d.l_ncol[l] + 3;
     ++g.nl;
   }
   g.nc = g.nl ? p1 - g.p0 : 0;
@@ -1534,69 +1550,69 @@ template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d
   return g.nc <= d.res_cap_c && gr.b_hi - gr.b_lo <= d.res_cap_b && g.no <= ResCap<R>::O && g.nl <= RES_LANES;
 }
 
-// one sweep of one resident lane by STRIDE threads (a wavefront or the
-// workgroup): STRIDE / 4 contacts per pass
-template <typename R, int STRIDE>
-__device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool serial, bool jacobi, bool hybrid,
-                            const R* xa, R* xn, R rs, int tid) {
-  const int c0 = lt.x, nc = serial ? lt.y : lt.z;
-  const int* co = L.coff + lt.w;
+// One sweep of one resident lane by STRIDE threads (a wavefront or the
+// workgroup): STRIDE / 4 contacts per pass.  Colour c spans positions
+// [co[c], co[c + 1]) of the lane (SERIAL: position c alone); the lane's
+// offsets end with two repeats of co[nc], so the colours after the last are
+// empty.  The merit, max |x_new - x_old| in FP64, is taken after the sweep
+// over the lane's rows (a sweep updates each of its contacts once).
+template <typename R, int STRIDE, bool SERIAL>
+__device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool jacobi, bool hybrid, const R* xa,
+                            R* xn, R rs, int tid) {
+  // the lane's table entry is the same for every thread: scalar loop control
+  const int c0 = __builtin_amdgcn_readfirstlane(lt.x), K = __builtin_amdgcn_readfirstlane(lt.y);
+  const int nc = SERIAL ? K : __builtin_amdgcn_readfirstlane(lt.z);
+  const int* co = L.coff + __builtin_amdgcn_readfirstlane(lt.w);
   const int q = tid & 3, j = tid >> 2;
   constexpr int PER = STRIDE / 4;
-  double m = 0.0;
-  if (nc <= 0) return m;
+  if (nc <= 0) return 0.0;
+  // end of colour c (a load; made scalar where it is used)
+  auto col_end = [&](int c) { return SERIAL ? min(c + 1, K) : co[c + 1]; };
+  // contact at position pos of a colour ending at end, or the null contact
+  auto pick = [&](int pos, int end) { return pos < end ? c0 + pos : ResLds<R>::nul; };
   // rows, iterate and body slots of contact i (none of them change during a sweep)
   auto load = [&](int i, QuadRow<R>& it) {
     it.jl = L.JL[4 * i + q]; it.ja = L.JA[4 * i + q]; it.jb = L.JB[4 * i + q];
     it.xo = xa[4 * i + q];
     it.lc = L.loc[i];
   };
-  // A quad with no contact in a colour takes a null row (zero rows and
-  // iterate, both bodies in the zero slot): its update is zeros and writes
-  // nothing, so the first pass of every phase runs on the whole wavefront.
-  auto load_or_null = [&](int pos, int end, QuadRow<R>& it) {
-    if (pos < end) {
-      load(c0 + pos, it);
-    } else {
-      const V4<R> z = {R(0), R(0), R(0), R(0)};
-      it.jl = z; it.ja = z; it.jb = z;
-      it.xo = R(0);
-      it.lc = make_int2(L.zs, L.zs);
-    }
-  };
-  // The first contact of this quad in each colour, and the colour's end, are
-  // read during the previous colour's phase, behind that phase's velocity
-  // loads: a phase then waits on one LDS round trip, not two.
-  int lo = serial ? 0 : co[0], hi = serial ? 1 : co[1];
-  QuadRow<R> cur, nxt;
-  load_or_null(lo + j, hi, cur);
-  for (int c = 0; c < nc; ++c) {
-    const int lo1 = hi;
-    int hi1 = hi;
-    auto next = [&] {
-      if (c + 1 < nc) {
-        hi1 = serial ? c + 2 : co[c + 2];
-        load_or_null(lo1 + j, hi1, nxt);
-      }
-    };
-    const int p0 = lo + j;
-    const bool mine = p0 < hi;
-    // a null row's column rows are read at the lane's first contact (times zero)
-    const int i0 = c0 + (mine ? p0 : 0);
+  // Phase c knows where colours c and c + 1 end.  Behind its velocity loads
+  // it reads this quad's first contact of colour c + 1 and where colour c + 2
+  // ends, so a phase waits on one LDS round trip, not two or three.  The two
+  // row buffers take turns, so nothing is copied between phases.
+  int lo = SERIAL ? 0 : __builtin_amdgcn_readfirstlane(co[0]);
+  int hi = __builtin_amdgcn_readfirstlane(col_end(0)), hn = __builtin_amdgcn_readfirstlane(col_end(1)), c = 0;
+  QuadRow<R> ra, rb;
+  load(pick(lo + j, hi), ra);
+  auto phase = [&](const QuadRow<R>& cur, QuadRow<R>& nxt) {
+    const int i0 = pick(lo + j, hi);
+    int h2 = 0;
     R xq;
-    m = fmax(m, quad_core(pp, cur, q, rs, jacobi, hybrid, L.B, L.zs, L.JL, L.JA, L.JB, 4 * (size_t)i0, xq, next));
-    if (mine) xn[4 * i0 + q] = xq;
+    quad_core(pp, cur, q, rs, jacobi, hybrid, L.B, L.zs, L.ts, L.JL, L.JA, L.JB, 4 * (size_t)i0, xq, [&] {
+      load(pick(hi + j, hn), nxt);
+      h2 = col_end(c + 2);
+    });
+    xn[4 * i0 + q] = xq;
     // further contacts of this quad in the colour (a colour wider than the quads)
-    for (int pos = p0 + PER; pos < hi; pos += PER) {
+    for (int pos = lo + j + PER; pos < hi; pos += PER) {
       const int i = c0 + pos;
       QuadRow<R> it;
       load(i, it);
-      m = fmax(m, quad_core(pp, it, q, rs, jacobi, hybrid, L.B, L.zs, L.JL, L.JA, L.JB, 4 * (size_t)i, xq, [] {}));
+      quad_core(pp, it, q, rs, jacobi, hybrid, L.B, L.zs, L.ts, L.JL, L.JA, L.JB, 4 * (size_t)i, xq, [] {});
       xn[4 * i + q] = xq;
     }
     phase_sync<STRIDE>();
-    lo = lo1; hi = hi1; cur = nxt;
+    lo = hi; hi = hn; hn = __builtin_amdgcn_readfirstlane(h2); ++c;
+  };
+  while (true) {
+    phase(ra, rb);
+    if (c >= nc) break;
+    phase(rb, ra);
+    if (c >= nc) break;
   }
+  double m = 0.0;
+  for (int r = tid; r < 4 * K; r += STRIDE)
+    m = fmax(m, fabs((double)xn[4 * c0 + r] - (double)xa[4 * c0 + r]));
   return m;
 }
 
@@ -1607,8 +1623,8 @@ __device__ __forceinline__ int sweep_decide(const Dev<R>& d, const SceneParams& 
                                             bool& abort) {
   double merit = block_max(m, sh_red);
   if (COOP && !grid_merit(d.merit, it, merit, d.err, s_ok, sh_red)) { abort = true; return 0; }
-  double* res = d.residuals + sp.res_off;
-  const int dec = ctl.decide(sp, merit, (lead && threadIdx.x == 0 && !d.post) ? res : nullptr, it,
+  double* res = residual_record(d, sp);
+  const int dec = ctl.decide(sp, merit, (lead && threadIdx.x == 0) ? res : nullptr, it,
                              hinge_merit(d, sp, it));
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     if (dec & 1) { B.sl[i] = B.wl[i]; B.sa[i] = B.wa[i]; }
@@ -1672,7 +1688,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
       if (e <= a) continue;
       const int nc = serial ? 0 : d.l_ncol[l];
       L.lt[t++] = make_int4(a - gs.p0, e - a, nc, o);
-      o += serial ? 0 : nc + 1;
+      o += serial ? 0 : nc + 3;
     }
   }
   __syncthreads();
@@ -1682,7 +1698,8 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
       if (d.l_end[l] <= d.l_start[l]) continue;
       const int4 lt = L.lt[t++];
       const int* src = lane_coff(d, l);
-      for (int c = threadIdx.x; c <= lt.z; c += blockDim.x) L.coff[lt.w + c] = src[c];
+      // offsets 0..nc, then co[nc] twice: the (empty) colours after the last
+      for (int c = threadIdx.x; c <= lt.z + 2; c += blockDim.x) L.coff[lt.w + c] = src[min(c, lt.z)];
     }
   }
   for (int r = threadIdx.x; r < 4 * gs.nc; r += blockDim.x) {
@@ -1693,6 +1710,14 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
     const int2 lc = d.s_loc[gs.p0 + i];
     L.loc[i] = make_int2(lc.x >= 0 ? lc.x - b0 : L.zs, lc.y >= 0 ? lc.y - b0 : L.zs);
+  }
+  // the null contact: zero rows and iterates, both bodies fixed
+  if (threadIdx.x < 4) {
+    const V4<R> z = {R(0), R(0), R(0), R(0)};
+    const int r = 4 * ResLds<R>::nul + threadIdx.x;
+    L.JL[r] = z; L.JA[r] = z; L.JB[r] = z;
+    L.x0[r] = R(0); L.x1[r] = R(0);
+    if (threadIdx.x == 0) L.loc[ResLds<R>::nul] = make_int2(L.zs, L.zs);
   }
   stage_bodies(d, L.B, b0, nb, L.zs);
   __syncthreads();
@@ -1708,12 +1733,13 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     for (int t = wave; t < gs.nl; t += nwave) {
       const int4 lt = L.lt[t];
       if (lt.y > WAVE_LANE_MAX && !serial) continue;
-      m = fmax(m, res_sweep<R, 64>(pp, L, lt, serial, jacobi, hybrid, xa, xn, rs, lane));
+      m = fmax(m, serial ? res_sweep<R, 64, true>(pp, L, lt, jacobi, hybrid, xa, xn, rs, lane)
+                         : res_sweep<R, 64, false>(pp, L, lt, jacobi, hybrid, xa, xn, rs, lane));
     }
     for (int t = 0; t < gs.nl; ++t) {
       const int4 lt = L.lt[t];
       if (lt.y <= WAVE_LANE_MAX || serial) continue;
-      m = fmax(m, res_sweep<R, NT>(pp, L, lt, serial, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
+      m = fmax(m, res_sweep<R, NT, false>(pp, L, lt, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
     if (abort) return;
@@ -1763,7 +1789,7 @@ __device__ double stream_sweep(const Dev<R>& d, const ProxParams& pp, const Body
     if (fhave) load_quad(d.s_JL, d.s_JA, d.s_JB, xa, d.s_loc, p0 + pb1 + j, q, b0, zs, fill);
     if (have) {
       R xq;
-      m = fmax(m, quad_core(pp, use, q, rs, jacobi, hybrid, B, zs, d.s_JL, d.s_JA, d.s_JB,
+      m = fmax(m, quad_core(pp, use, q, rs, jacobi, hybrid, B, zs, zs + 1, d.s_JL, d.s_JA, d.s_JB,
                             4 * (size_t)(p0 + pb + j), xq, [] {}));
       xn[4 * (size_t)(p0 + pb + j) + q] = xq;
     }
@@ -1860,8 +1886,7 @@ template <typename R, int NT> __global__ void __launch_bounds__(NT) k_solve(Dev<
   const SceneParams& sp = d.sp[slot];
   SceneState& st = d.ss[slot];
   if (d.post && !(sp.use_post_stab && st.post_active)) return;
-  double* res = d.residuals + sp.res_off;
-  if (!d.post)
+  double* res = residual_record(d, sp);
     for (int i = threadIdx.x; i < sp.max_iter; i += NT) res[i] = INFINITY;
   if (st.k_cnt == 0) {
     if (threadIdx.x == 0) no_contacts<R>(st);
@@ -1883,8 +1908,8 @@ template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve_mul
   const SceneParams& sp = d.sp[slot];
   SceneState& st = d.ss[slot];
   if (d.post && !(sp.use_post_stab && st.post_active)) return;   // uniform over the grid
-  double* res = d.residuals + sp.res_off;
-  if (blockIdx.x == 0 && !d.post)
+  double* res = residual_record(d, sp);
+  if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < sp.max_iter; i += SOLVE_BLOCK) res[i] = INFINITY;
   if (st.k_cnt == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) no_contacts<R>(st);

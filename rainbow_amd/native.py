@@ -20,7 +20,8 @@ SCHEMES = {"gauss_seidel": 0, "parallel_gauss_seidel": 1, "parallel_jacobi": 2,
 FRICTION = {"isotropic": 0, "anisotropic": 1}
 STEP_COUNT_OVERLAPS, STEP_TIMING, STEP_GRAPH, STEP_FORCE_SINGLE, STEP_FORCE_MULTI = 1, 2, 4, 8, 16
 TIMING_KEYS = ["update_bvh", "narrow_phase", "contact_determination", "contact_assembly",
-               "coloring", "solver_time", "integration", "stepper_time"]
+               "coloring", "solver_time", "integration", "stepper_time", "contact_point_reduction",
+               "post_stabilization_solver_time"]
 
 
 class RbhError(RuntimeError):
@@ -303,10 +304,12 @@ class Context:
 
     def get_log(self, sid, max_iterations):
         lg = StepLog()
-        res = np.empty(max(1, int(max_iterations)))
+        mi = max(1, int(max_iterations))
+        res = np.empty(2 * mi)
         self._check(self.lib.rbh_get_log(self.h, int(sid), C.byref(lg), _ptr(res), len(res)))
         d = {k: getattr(lg, k) for k, _ in StepLog._fields_}
         d["residuals"] = res[:int(max_iterations)]
+        d["post_residuals"] = res[mi:mi + int(max_iterations)]
         return d
 
     def get_contacts(self, sid):
@@ -323,8 +326,8 @@ class Context:
         return out
 
     def get_timings(self):
-        ms = np.zeros(8)
-        n = self._check(self.lib.rbh_get_timings(self.h, _ptr(ms), 8))
+        ms = np.zeros(len(TIMING_KEYS))
+        n = self._check(self.lib.rbh_get_timings(self.h, _ptr(ms), len(ms)))
         return dict(zip(TIMING_KEYS[:n], ms[:n] / 1e3))
 
     def get_counters(self):

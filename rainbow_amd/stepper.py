@@ -234,19 +234,27 @@ class HipStepper:
         t = self.ctx.get_timings()
         r, q, v, w = self.ctx.get_state(self.scene, len(self._bodies))
         kin, pot = energies(engine, self._bodies, r, q, v, w)
-        cd = t["update_bvh"] + t["narrow_phase"] + t["contact_determination"]
+        cd = t["update_bvh"] + t["narrow_phase"] + t["contact_determination"] + t["contact_point_reduction"]
+        # collision_detection.py:193-201: the reference times three parts of the
+        # contact determination.  One kernel does all three here (each lane moves
+        # its triangle into the SDF's frame, runs Frank-Wolfe and writes the
+        # point), so its whole time is contact_optimization and the other two
+        # parts, which have no kernel of their own, are 0.
         d = dict(update_bvh=t["update_bvh"], narrow_phase=t["narrow_phase"],
                  number_of_overlaps=int(lg["number_of_overlaps"]), model_space_update=0.0,
                  contact_optimization=t["contact_determination"], contact_point_generation=0.0,
-                 contact_determination=t["contact_determination"], contact_point_reduction=0.0,
-                 collision_detection_time=cd)
+                 contact_determination=t["contact_determination"],
+                 contact_point_reduction=t["contact_point_reduction"], collision_detection_time=cd)
         if lg["contact_points"] > 0 or engine.hinges:
             d.update(residuals=lg["residuals"].copy(), iterations=int(lg["iterations"]),
                      exitcode=int(lg["exitcode"]), solver_time=t["coloring"] + t["solver_time"])
         if lg["post_exitcode"] >= 0:
-            # steppers.py:33-34: the post-stabilisation solve's keys carry its prefix
-            d.update(post_stabilization_iterations=int(lg["post_iterations"]),
-                     post_stabilization_exitcode=int(lg["post_exitcode"]))
+            # steppers.py:33-34 with gauss_seidel_solver.py:29-34, 98-101: the
+            # post-stabilisation solve's keys carry its prefix
+            d.update(post_stabilization_residuals=lg["post_residuals"].copy(),
+                     post_stabilization_iterations=int(lg["post_iterations"]),
+                     post_stabilization_exitcode=int(lg["post_exitcode"]),
+                     post_stabilization_solver_time=t["post_stabilization_solver_time"])
         d.update(stepper_time=t["stepper_time"], dt=dt, contact_points=int(lg["contact_points"]),
                  kinetic_energy=kin, potential_energy=pot, max_penetration=float(lg["max_penetration"]),
                  natural_residual=float(lg["natural_residual"]), rejects=int(lg["rejects"]),

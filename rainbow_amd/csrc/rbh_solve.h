@@ -1350,36 +1350,61 @@ template <typename R> struct QuadRow {
   int2 lc;
 };
 
-// Velocity changes and inverse masses of a group's bodies in LDS (slot
-// zs stays zero and stands for every fixed body; slot ts = zs + 1 takes the
-// discarded velocity updates of fixed bodies, so no update branches on them).
+// One field of the bodies' LDS records: entry i of a record array with
+// BODY_REC vec4s per body.
+constexpr int BODY_REC = 6;
+enum BodyField { F_WL = 0, F_WA, F_SL, F_SA, F_W0, F_W1 };
+template <typename R> struct BodyCol {
+  V4<R>* p;
+  __device__ __forceinline__ V4<R>& operator[](int i) const { return p[BODY_REC * i]; }
+};
+// bytes of one body record
+template <typename R> __host__ __device__ constexpr int rec_bytes() { return BODY_REC * (int)sizeof(V4<R>); }
+
+// Velocity changes and inverse masses of a group's bodies in LDS, one record
+// per slot: working (wl, wa) and accepted (sl, sa) velocity changes, W0, W1.
+// A contact's loads of one body then share one address.  Slot zs stays zero
+// and stands for every fixed body; slot ts = zs + 1 takes the discarded
+// velocity updates of fixed bodies, so no update branches on them.
 template <typename R> struct BodyLds {
-  V4<R> *wl, *wa, *sl, *sa, *W0, *W1;
+  BodyCol<R> wl, wa, sl, sa, W0, W1;
+  unsigned char* base;
+  __device__ void at(V4<R>* b) {
+    base = reinterpret_cast<unsigned char*>(b);
+    wl.p = b; wa.p = b + 1; sl.p = b + 2; sa.p = b + 3; W0.p = b + 4; W1.p = b + 5;
+  }
+  // field f of the record at byte offset off (a slot times rec_bytes)
+  __device__ __forceinline__ V4<R>& rec(int off, int f) const {
+    return *reinterpret_cast<V4<R>*>(base + off + f * (int)sizeof(V4<R>));
+  }
 };
 
 // One contact by one quad; lane q holds row q.  Every lane repeats the serial
 // update's operations (update_contact) on the same operands in the same
-// order, so the results are bitwise the same.  Writes this lane's velocity
-// block (to slot ts for a fixed body); returns the new iterate in xq and
-// |xq - xo| in FP64.
+// order, so the results are bitwise the same.  it.lc holds the byte offsets
+// of the two bodies' records (zsb: the zero record of fixed bodies, tsb: the
+// discard record).  Writes this lane's velocity block (to the discard record
+// for a fixed body); returns the new iterate in xq and |xq - xo| in FP64.
+// JAC: Jacobi reads the accepted velocity changes (jacobi_solver.py:6-57),
+// Gauss-Seidel the working ones.
 // JL/JA/JB + 4 i: the contact's rows (LDS or global); this lane's velocity
 // block is built from rows 0..3 of one of them, loaded first.
-template <typename R, typename F>
-__device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<R>& it, int q, R rs, bool jacobi,
-                                            bool hybrid, const BodyLds<R>& B, int zs, int ts, const V4<R>* JL,
+template <typename R, bool JAC, typename F>
+__device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<R>& it, int q, R rs, bool hybrid,
+                                            const BodyLds<R>& B, int zsb, int tsb, const V4<R>* JL,
                                             const V4<R>* JA, const V4<R>* JB, size_t i4, R& xq, F&& after_loads) {
-  const V4<R>* rl = jacobi ? B.sl : B.wl;
-  const V4<R>* ra = jacobi ? B.sa : B.wa;
+  constexpr int FL = JAC ? F_SL : F_WL, FA = JAC ? F_SA : F_WA;
   const bool side_b = q >= 2, ang = q & 1;   // lane 0: linear A, 1: angular A, 2: linear B, 3: angular B
-  const int slot = side_b ? it.lc.y : it.lc.x;
-  V4<R>* dst = ang ? B.wa : B.wl;
-  const bool own = slot != zs;
-  // fixed bodies read the zero slot (the serial update uses zeros for them)
-  const V4<R> lA = rl[it.lc.x], aA = ra[it.lc.x], lB = rl[it.lc.y], aB = ra[it.lc.y];
-  const V4<R> W0 = B.W0[slot], W1 = B.W1[slot];
-  // the block this lane updates, read beside the four blocks above (in
+  const int offA = it.lc.x, offB = it.lc.y;
+  const int off = side_b ? offB : offA;
+  const bool own = off != zsb;
+  // fixed bodies read the zero record (the serial update uses zeros for them)
+  const V4<R> lA = B.rec(offA, FL), aA = B.rec(offA, FA), lB = B.rec(offB, FL), aB = B.rec(offB, FA);
+  const V4<R> W0 = B.rec(off, F_W0), W1 = B.rec(off, F_W1);
+  // the working block this lane updates, read beside the four blocks above (in
   // Gauss-Seidel it is one of them: no other quad of the phase writes it)
-  const V4<R> old = dst[slot];
+  const int fo = ang ? (int)sizeof(V4<R>) : 0;
+  const V4<R> old = B.rec(off + fo, F_WL);
   // the caller's loads (the next phase's contact), issued behind this phase's
   after_loads();
   const V4<R>* srcv = (q & 1) ? (q >= 2 ? JB : JA) : JL;
@@ -1394,7 +1419,7 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
   const R mu0 = quad_bcast<1>(jb.w), mu1 = quad_bcast<2>(jb.w), mu2 = quad_bcast<3>(jb.w);
   // projections (contacts.py:92-95), the same in every lane of the quad
   const R xnrm = fmax(R(0), z0);
-  const R xr = (jacobi && !hybrid) ? xo0 : xnrm;
+  const R xr = (JAC && !hybrid) ? xo0 : xnrm;
   R os, ot, orr;
   if (pp.friction == 1) {
     R o[3];
@@ -1409,7 +1434,7 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
   const R dq = xq - xo;
   const R d0 = quad_bcast<0>(dq), d1 = quad_bcast<1>(dq), d2 = quad_bcast<2>(dq), d3 = quad_bcast<3>(dq);
   // w += W J^T dx (contacts.py:107), this lane's block; a fixed body's
-  // (zero W, zero block) lands in the discard slot
+  // (zero W, zero block) lands in the discard record
   {
     const R sign = side_b ? R(1) : R(-1);
     const R s0 = sign * d0, s1 = sign * d1, s2 = sign * d2, s3 = sign * d3;
@@ -1425,12 +1450,13 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
     } else {
       nw = V4<R>{madd(W1.z, acc.x, old.x), madd(W1.z, acc.y, old.y), madd(W1.z, acc.z, old.z), R(0)};
     }
-    dst[own ? slot : ts] = nw;
+    B.rec((own ? off : tsb) + fo, F_WL) = nw;
   }
   return fabs((double)xq - (double)xo);
 }
 
-// lane q's share of contact i from row arrays (LDS or global); slots made local
+// lane q's share of contact i from row arrays (LDS or global); body slots
+// made local and turned into record byte offsets
 template <typename R>
 __device__ __forceinline__ void load_quad(const V4<R>* JL, const V4<R>* JA, const V4<R>* JB, const R* xa,
                                           const int2* loc, int i, int q, int b0, int zs, QuadRow<R>& it) {
@@ -1438,7 +1464,7 @@ __device__ __forceinline__ void load_quad(const V4<R>* JL, const V4<R>* JA, cons
   it.jl = JL[row]; it.ja = JA[row]; it.jb = JB[row];
   it.xo = xa[row];
   const int2 lc = loc[i];
-  it.lc = make_int2(lc.x >= 0 ? lc.x - b0 : zs, lc.y >= 0 ? lc.y - b0 : zs);
+  it.lc = make_int2((lc.x >= 0 ? lc.x - b0 : zs) * rec_bytes<R>(), (lc.y >= 0 ? lc.y - b0 : zs) * rec_bytes<R>());
 }
 
 // natural-map residual parts of contact i (FP64) from its rows and the
@@ -1502,13 +1528,14 @@ template <typename R> struct ResLds {
   int* coff;
   static constexpr int zs = ResCap<R>::B;
   static constexpr int ts = ResCap<R>::B + 1;
+  static constexpr int zsb = zs * rec_bytes<R>(), tsb = ts * rec_bytes<R>();   // their records' byte offsets
   static constexpr int nul = ResCap<R>::C;
   __device__ explicit ResLds(unsigned char* p) {
     constexpr int cc = ResCap<R>::C + 1, nbs = ResCap<R>::B + 2;
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
     JL = v; JA = JL + 4 * cc; JB = JA + 4 * cc;
-    B.wl = JB + 4 * cc; B.wa = B.wl + nbs; B.sl = B.wa + nbs; B.sa = B.sl + nbs; B.W0 = B.sa + nbs; B.W1 = B.W0 + nbs;
-    x0 = reinterpret_cast<R*>(B.W1 + nbs); x1 = x0 + 4 * cc;
+    B.at(JB + 4 * cc);
+    x0 = reinterpret_cast<R*>(JB + 4 * cc + BODY_REC * nbs); x1 = x0 + 4 * cc;
     loc = reinterpret_cast<int2*>(x1 + 4 * cc);
     lt = reinterpret_cast<int4*>(loc + cc);
     coff = reinterpret_cast<int*>(lt + RES_LANES);
@@ -1523,7 +1550,8 @@ template <typename R> struct StreamLds {
   __device__ explicit StreamLds(unsigned char* p) {
     constexpr int nbs = ResCap<R>::VB + 2;
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
-    B.wl = v; B.wa = B.wl + nbs; B.sl = B.wa + nbs; B.sa = B.sl + nbs; B.W0 = B.sa + nbs; B.W1 = B.W0 + nbs;
+    (void)nbs;
+    B.at(v);
   }
 };
 
@@ -1556,9 +1584,9 @@ template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d
 // offsets end with two repeats of co[nc], so the colours after the last are
 // empty.  The merit, max |x_new - x_old| in FP64, is taken after the sweep
 // over the lane's rows (a sweep updates each of its contacts once).
-template <typename R, int STRIDE, bool SERIAL>
-__device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool jacobi, bool hybrid, const R* xa,
-                            R* xn, R rs, int tid) {
+template <typename R, int STRIDE, bool SERIAL, bool JAC>
+__device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool hybrid, const R* xa, R* xn, R rs,
+                            int tid) {
   // the lane's table entry is the same for every thread: scalar loop control
   const int c0 = __builtin_amdgcn_readfirstlane(lt.x), K = __builtin_amdgcn_readfirstlane(lt.y);
   const int nc = SERIAL ? K : __builtin_amdgcn_readfirstlane(lt.z);
@@ -1588,7 +1616,7 @@ __device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool ja
     const int i0 = pick(lo + j, hi);
     int h2 = 0;
     R xq;
-    quad_core(pp, cur, q, rs, jacobi, hybrid, L.B, L.zs, L.ts, L.JL, L.JA, L.JB, 4 * (size_t)i0, xq, [&] {
+    quad_core<R, JAC>(pp, cur, q, rs, hybrid, L.B, L.zsb, L.tsb, L.JL, L.JA, L.JB, 4 * (size_t)i0, xq, [&] {
       load(pick(hi + j, hn), nxt);
       h2 = col_end(c + 2);
     });
@@ -1598,7 +1626,7 @@ __device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool ja
       const int i = c0 + pos;
       QuadRow<R> it;
       load(i, it);
-      quad_core(pp, it, q, rs, jacobi, hybrid, L.B, L.zs, L.ts, L.JL, L.JA, L.JB, 4 * (size_t)i, xq, [] {});
+      quad_core<R, JAC>(pp, it, q, rs, hybrid, L.B, L.zsb, L.tsb, L.JL, L.JA, L.JB, 4 * (size_t)i, xq, [] {});
       xn[4 * i + q] = xq;
     }
     phase_sync<STRIDE>();
@@ -1709,7 +1737,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   }
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
     const int2 lc = d.s_loc[gs.p0 + i];
-    L.loc[i] = make_int2(lc.x >= 0 ? lc.x - b0 : L.zs, lc.y >= 0 ? lc.y - b0 : L.zs);
+    L.loc[i] = make_int2((lc.x >= 0 ? lc.x - b0 : L.zs) * rec_bytes<R>(), (lc.y >= 0 ? lc.y - b0 : L.zs) * rec_bytes<R>());
   }
   // the null contact: zero rows and iterates, both bodies fixed
   if (threadIdx.x < 4) {
@@ -1717,7 +1745,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     const int r = 4 * ResLds<R>::nul + threadIdx.x;
     L.JL[r] = z; L.JA[r] = z; L.JB[r] = z;
     L.x0[r] = R(0); L.x1[r] = R(0);
-    if (threadIdx.x == 0) L.loc[ResLds<R>::nul] = make_int2(L.zs, L.zs);
+    if (threadIdx.x == 0) L.loc[ResLds<R>::nul] = make_int2(L.zsb, L.zsb);
   }
   stage_bodies(d, L.B, b0, nb, L.zs);
   __syncthreads();
@@ -1733,13 +1761,15 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     for (int t = wave; t < gs.nl; t += nwave) {
       const int4 lt = L.lt[t];
       if (lt.y > WAVE_LANE_MAX && !serial) continue;
-      m = fmax(m, serial ? res_sweep<R, 64, true>(pp, L, lt, jacobi, hybrid, xa, xn, rs, lane)
-                         : res_sweep<R, 64, false>(pp, L, lt, jacobi, hybrid, xa, xn, rs, lane));
+      m = fmax(m, serial   ? res_sweep<R, 64, true, false>(pp, L, lt, hybrid, xa, xn, rs, lane)
+                  : jacobi ? res_sweep<R, 64, false, true>(pp, L, lt, hybrid, xa, xn, rs, lane)
+                           : res_sweep<R, 64, false, false>(pp, L, lt, hybrid, xa, xn, rs, lane));
     }
     for (int t = 0; t < gs.nl; ++t) {
       const int4 lt = L.lt[t];
       if (lt.y <= WAVE_LANE_MAX || serial) continue;
-      m = fmax(m, res_sweep<R, NT, false>(pp, L, lt, jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
+      m = fmax(m, jacobi ? res_sweep<R, NT, false, true>(pp, L, lt, hybrid, xa, xn, rs, (int)threadIdx.x)
+                         : res_sweep<R, NT, false, false>(pp, L, lt, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
     if (abort) return;
@@ -1751,7 +1781,8 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   for (int i = threadIdx.x; i < gs.nc; i += blockDim.x) {
     const int2 lc = L.loc[i];
     V4<R> x;
-    quad_finish_contact(pp, L.JL, L.JA, L.JB, xs, i, L.B.sl[lc.x], L.B.sa[lc.x], L.B.sl[lc.y], L.B.sa[lc.y], x,
+    quad_finish_contact(pp, L.JL, L.JA, L.JB, xs, i, L.B.rec(lc.x, F_SL), L.B.rec(lc.x, F_SA), L.B.rec(lc.y, F_SL),
+                        L.B.rec(lc.y, F_SA), x,
                         rloc, xmax);
     if (!d.post) d.c_imp[d.perm[gs.p0 + i]] = x;
   }
@@ -1761,10 +1792,9 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
 // One sweep of one lane in the streaming layout: rows and iterates from
 // global memory, the next pass prefetched into the other of two buffers
 // while the current one is updated.
-template <typename R, int STRIDE>
+template <typename R, int STRIDE, bool JAC>
 __device__ double stream_sweep(const Dev<R>& d, const ProxParams& pp, const BodyLds<R>& B, int zs, int b0, int p0,
-                               int nc, const int* coff, bool serial, bool jacobi, bool hybrid, const R* xa, R* xn,
-                               R rs, int tid) {
+                               int nc, const int* coff, bool serial, bool hybrid, const R* xa, R* xn, R rs, int tid) {
   constexpr int PER = STRIDE / 4;
   const int q = tid & 3, j = tid >> 2;
   double m = 0.0;
@@ -1789,7 +1819,8 @@ __device__ double stream_sweep(const Dev<R>& d, const ProxParams& pp, const Body
     if (fhave) load_quad(d.s_JL, d.s_JA, d.s_JB, xa, d.s_loc, p0 + pb1 + j, q, b0, zs, fill);
     if (have) {
       R xq;
-      m = fmax(m, quad_core(pp, use, q, rs, jacobi, hybrid, B, zs, zs + 1, d.s_JL, d.s_JA, d.s_JB,
+      m = fmax(m, quad_core<R, JAC>(pp, use, q, rs, hybrid, B, zs * rec_bytes<R>(), (zs + 1) * rec_bytes<R>(),
+                                    d.s_JL, d.s_JA, d.s_JB,
                             4 * (size_t)(p0 + pb + j), xq, [] {}));
       xn[4 * (size_t)(p0 + pb + j) + q] = xq;
     }
@@ -1831,14 +1862,19 @@ __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group&
     for (int l = gr.l_lo + wave; l < gr.l_hi; l += nwave) {
       const int p0 = d.l_start[l], K = d.l_end[l] - p0;
       if (K <= 0 || (K > WAVE_LANE_MAX && !serial)) continue;
-      m = fmax(m, stream_sweep<R, 64>(d, pp, L.B, L.zs, b0, p0, serial ? K : d.l_ncol[l], lane_coff(d, l), serial,
-                                      jacobi, hybrid, xa, xn, rs, lane));
+      const int nc = serial ? K : d.l_ncol[l];
+      m = fmax(m, jacobi ? stream_sweep<R, 64, true>(d, pp, L.B, L.zs, b0, p0, nc, lane_coff(d, l), serial, hybrid,
+                                                     xa, xn, rs, lane)
+                         : stream_sweep<R, 64, false>(d, pp, L.B, L.zs, b0, p0, nc, lane_coff(d, l), serial, hybrid,
+                                                      xa, xn, rs, lane));
     }
     for (int l = gr.l_lo; l < gr.l_hi; ++l) {
       const int p0 = d.l_start[l], K = d.l_end[l] - p0;
       if (K <= WAVE_LANE_MAX || serial) continue;
-      m = fmax(m, stream_sweep<R, NT>(d, pp, L.B, L.zs, b0, p0, d.l_ncol[l], lane_coff(d, l), serial,
-                                               jacobi, hybrid, xa, xn, rs, (int)threadIdx.x));
+      m = fmax(m, jacobi ? stream_sweep<R, NT, true>(d, pp, L.B, L.zs, b0, p0, d.l_ncol[l], lane_coff(d, l), serial,
+                                                     hybrid, xa, xn, rs, (int)threadIdx.x)
+                         : stream_sweep<R, NT, false>(d, pp, L.B, L.zs, b0, p0, d.l_ncol[l], lane_coff(d, l), serial,
+                                                      hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
     if (abort) return;

@@ -1234,6 +1234,10 @@ int rbh_shape_upload(rbh_ctx* c, const double* V, int nv, const int* T, int nt, 
     s.mx[a] = mx[a];
     s.h[a] = (mx[a] - mn[a]) / (double)res[a];   // grid3.py:42-44
   }
+  s.hdiag = std::sqrt(s.h[0] * s.h[0] + s.h[1] * s.h[1] + s.h[2] * s.h[2]);
+  s.reach = 0.0;
+  for (int i = 0; i < nv; ++i)
+    s.reach = std::max(s.reach, std::sqrt(V[3 * i] * V[3 * i] + V[3 * i + 1] * V[3 * i + 1] + V[3 * i + 2] * V[3 * i + 2]));
   for (int i = 0; i < nv; ++i) {
     c->sV.push_back(V[3 * i]); c->sV.push_back(V[3 * i + 1]); c->sV.push_back(V[3 * i + 2]); c->sV.push_back(0.0);
   }

@@ -17,6 +17,9 @@ namespace rbh {
 struct ShapeInfo {
   int v_off, nv, t_off, nt, g_off, I, J, K;
   double mn[3], mx[3], h[3];
+  // reach: largest distance of a vertex from the body-frame origin; hdiag:
+  // the SDF cell diagonal (sqrt(hx^2 + hy^2 + hz^2))
+  double reach, hdiag;
 };
 
 enum Scheme { SCHEME_GS = 0, SCHEME_PGS = 1, SCHEME_JACOBI = 2, SCHEME_JACOBI_HYBRID = 3 };
@@ -295,6 +298,32 @@ template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
 // ========================================================== body pairs
 // itertools.combinations over the scene's bodies (collision_detection.py:70),
 // kept when the root volumes overlap (kdop_bvh.py:340).  One wave per row.
+//
+// A pair can only give a contact if its bodies come within reach of each
+// other.  A contact needs a point x on a triangle of A with phi_B(x) <=
+// envelope, x inside B's grid (collision_detection.py:172-176).  There phi_B
+// is the trilinear interpolant of exact signed distances at the corners of
+// x's cell, each within the cell diagonal hd_B of x, and the signed distance
+// is 1-Lipschitz, so phi_B(x) >= sd_B(x) - hd_B >= |x - r_B| - reach_B - hd_B
+// >= |r_A - r_B| - reach_A - reach_B - hd_B.  Pairs farther apart than
+// reach_A + reach_B + envelope + max(hd_A, hd_B) (plus a rounding margin)
+// therefore give no contact, in either direction, and are dropped before
+// their triangles are compared.  Counting triangle overlaps
+// (STEP_COUNT_OVERLAPS, the profile key number_of_overlaps) keeps every pair.
+template <typename R> __device__ __forceinline__ bool within_reach(const Dev<R>& d, int a, int b, R env) {
+  const ShapeInfo& sa = d.shapes[d.b_shape[a]];
+  const ShapeInfo& sb = d.shapes[d.b_shape[b]];
+  const R lim = (R(sa.reach) + R(sb.reach) + env + R(fmax(sa.hdiag, sb.hdiag))) * R(1.001) + R(1e-4);
+  const V4<R> ra = d.r[a], rb = d.r[b];
+  const R dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+  return dx * dx + dy * dy + dz * dz <= lim * lim;
+}
+
+template <typename R> __device__ __forceinline__ bool pair_kept(const Dev<R>& d, int a, int b, V4<R> alo, V4<R> ahi,
+                                                                R env, bool all) {
+  return box_overlap(alo, ahi, d.b_lo[b], d.b_hi[b]) && (all || within_reach(d, a, b, env));
+}
+
 template <typename R> __global__ void __launch_bounds__(64) k_pair_count(Dev<R> d) {
   int f = blockIdx.x;
   if (f >= d.nb) return;
@@ -302,8 +331,10 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_count(Dev<R> 
   int b = d.bb_body[f];
   int end = d.f_end[f];
   V4<R> alo = d.b_lo[b], ahi = d.b_hi[b];
+  const R env = R(d.sp[d.b_slot[b]].envelope);
+  const bool all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   int cnt = 0;
-  for (int j = b + 1 + threadIdx.x; j < end; j += 64) cnt += box_overlap(alo, ahi, d.b_lo[j], d.b_hi[j]) ? 1 : 0;
+  for (int j = b + 1 + threadIdx.x; j < end; j += 64) cnt += pair_kept(d, b, j, alo, ahi, env, all) ? 1 : 0;
   cnt = wave_sum(cnt);
   if (threadIdx.x == 0) d.row_cnt[f] = cnt;
 }
@@ -321,10 +352,12 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
   }
   if (d.row_off[d.nb] > d.cap_pairs) return;
   V4<R> alo = d.b_lo[b], ahi = d.b_hi[b];
+  const R env = R(d.sp[d.b_slot[b]].envelope);
+  const bool all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   const int ntA = d.shapes[d.b_shape[b]].nt;
   for (int j0 = b + 1; j0 < end; j0 += 64) {
     int j = j0 + threadIdx.x;
-    bool hit = j < end && box_overlap(alo, ahi, d.b_lo[j], d.b_hi[j]);
+    bool hit = j < end && pair_kept(d, b, j, alo, ahi, env, all);
     unsigned long long m = __ballot(hit);
     if (hit) {
       int pos = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));

@@ -312,7 +312,8 @@ def _oracle_scene_steps(job):
 
 def cpu_baseline_scenes(engs, states, step_index, what, seconds=15.0):
     """The FP64 oracle's step (O(K) vectorised sweep) of independent scenes
-    from the GPU's states: on one core over the first scenes for about
+    from the GPU's states (step_index: the steps each state has taken, one
+    number or one per scene): on one core over the first scenes for about
     `seconds`, then one scene per process on every usable core over about
     `seconds` of work per core."""
     import multiprocessing as mp
@@ -326,7 +327,7 @@ def cpu_baseline_scenes(engs, states, step_index, what, seconds=15.0):
             if not b.is_fixed:
                 b.r, b.q, b.v, b.w = r[k].copy(), q[k].copy(), v[k].copy(), w[k].copy()
         sc = OS.from_engine(e)
-        sc.step_index = step_index
+        sc.step_index = step_index[i] if isinstance(step_index, (list, tuple)) else step_index
         return sc, e.params.time_step
 
     # one core: scenes in order until `seconds` pass
@@ -349,9 +350,10 @@ def cpu_baseline_scenes(engs, states, step_index, what, seconds=15.0):
     wall = time.perf_counter() - t1
     done = sum(len(r) for r in res)
     return {"value": done / wall, "unit": "scene-steps/s", "cores": len(chunks), "kind": "port",
-            "sample": "one FP64 oracle step (NumPy, O(K) sweep) of each of %d %s scenes from the GPU states after "
-                      "step %d, one scene per process on %d cores, %.1f s wall" % (done, what, step_index, len(chunks),
-                                                                               wall),
+            "sample": "one FP64 oracle step (NumPy, O(K) sweep) of each of %d %s scenes from the GPU states %s, "
+                      "one scene per process on %d cores, %.1f s wall" % (
+                          done, what, "at the end of their runs" if isinstance(step_index, (list, tuple))
+                          else "after step %d" % step_index, len(chunks), wall),
             "single_core": {"value": 1.0 / per_scene, "unit": "scene-steps/s", "cores": 1,
                             "sample": "%d scenes in order, %.1f s" % (len(one), sum(one))},
             "host": hi}
@@ -363,7 +365,7 @@ def tuning_report(res):
             "trials": [[keep(t), round(ms, 4)] for t, ms in res["trials"]]}
 
 
-def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, sat_requests=256):
+def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, sat_requests=256, cpu=True):
     """Poisson arrivals served by continuous batching, one run per arrival rate
     on the same n_req requests, and a saturation run (every one of
     sat_requests requests waiting at t = 0; the first n_req are the same
@@ -448,6 +450,12 @@ def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, sat_requ
     ctx.set_tuning(**autotune.DEFAULT)
     plain = psteps / tp
     sat = runs[-1]
+    cb = None
+    if cpu:
+        log("serving: cpu baseline ...")
+        done = [r for r in reqs[:sat_requests] if r.state is not None]
+        cb = cpu_baseline_scenes([r.engine for r in done], [r.state for r in done], [r.n_steps for r in done],
+                                 "served-request")
     return {"workload": "sphere_drop_requests", "requests": n_req, "saturation_requests": sat_requests,
             "budget_bytes": budget_gib << 30, "tuning": tuning_report(tuned), "runs": runs,
             "binding_budget": bound,
@@ -455,7 +463,8 @@ def serving_leg(n_req, rates, budget_gib, alone_checks, plain_sample=8, sat_requ
                       "note": "each sampled request alone over its whole lifetime, one step per launch",
                       "ms_per_scene_step": 1e3 * tp / psteps},
             "speedup_vs_plain_at_saturation": sat["scene_steps_per_s"] / plain if plain > 0 else None,
-            "max_abs_diff_alone_vs_mixed": max(diffs) if diffs else None, "alone_checked": len(diffs)}
+            "max_abs_diff_alone_vs_mixed": max(diffs) if diffs else None, "alone_checked": len(diffs),
+            "cpu_baseline": cb}
 
 
 def stress_leg(steps):
@@ -588,7 +597,7 @@ def main():
         return
     rates = [float(x) for x in args.serving_rates.split(",") if x]
     if args.only == "serving":
-        print(json.dumps(serving_leg(args.serving_requests, rates, 64, 3)), flush=True)
+        print(json.dumps(serving_leg(args.serving_requests, rates, 64, 3, cpu=not args.no_cpu)), flush=True)
         return
     if args.only == "stress":
         print(json.dumps(stress_leg(args.stress_steps)), flush=True)
@@ -695,7 +704,7 @@ def main():
                                                           result["batched"]["plain"]["scene_steps_per_s"]))
     if not args.no_serving:
         log("serving leg ...")
-        result["serving"] = serving_leg(args.serving_requests, rates, 64, 3)
+        result["serving"] = serving_leg(args.serving_requests, rates, 64, 3, cpu=not args.no_cpu)
     if not args.no_stress:
         log("stress leg ...")
         result["stress"] = stress_leg(args.stress_steps)

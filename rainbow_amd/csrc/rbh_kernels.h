@@ -420,7 +420,20 @@ __device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0,
   const int kY = blk_box(tY, sY.nt);
   const V4<R> xlo = d.clo[blk_box(tX, sX.nt) + x0 / CAND_BLK], xhi = d.chi[blk_box(tX, sX.nt) + x0 / CAND_BLK];
   const V4<R> rYlo = d.b_lo[Y], rYhi = d.b_hi[Y];
-  if (!box_overlap(xlo, xhi, rYlo, rYhi)) {
+  // A triangle of X farther from Y's origin than reach_Y + envelope + Y's cell
+  // diagonal gives no contact against Y's SDF (the bound of within_reach,
+  // taken for the triangle's box, which contains the triangle); such
+  // triangles are not candidates unless every overlap is being counted.
+  const R env = R(d.sp[d.b_slot[X]].envelope);
+  const R lim = (R(sY.reach) + R(sY.hdiag) + env) * R(1.001) + R(1e-4);
+  const V4<R> oY = d.r[Y];
+  auto beyond = [&](const V4<R>& lo, const V4<R>& hi) {
+    const R ex = fmax(fmax(lo.x - oY.x, oY.x - hi.x), R(0));
+    const R ey = fmax(fmax(lo.y - oY.y, oY.y - hi.y), R(0));
+    const R ez = fmax(fmax(lo.z - oY.z, oY.z - hi.z), R(0));
+    return !count_all && ex * ex + ey * ey + ez * ez > lim * lim;
+  };
+  if (!box_overlap(xlo, xhi, rYlo, rYhi) || beyond(xlo, xhi)) {
     for (int p = x0 + lane; p < x1; p += 64) d.slot_flag[slot_base + d.sT[sX.t_off + p].w] = 0;
     return;
   }
@@ -480,12 +493,12 @@ __device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0,
       const V4<R> cbl = d.clo[tX + cl], cbh = d.chi[tX + cl];
       const V4<R> lo = d.tlo[tX + p], hi = d.thi[tX + p];
       unsigned char flag = first ? 0 : d.slot_flag[slot_base + t];
-      if (!box_overlap(cbl, cbh, rYlo, rYhi)) {
+      if (!box_overlap(cbl, cbh, rYlo, rYhi) || beyond(cbl, cbh)) {
         if (first) d.slot_flag[slot_base + t] = 0;
         continue;
       }
       if (flag && !count_all) continue;
-      if (!box_overlap(lo, hi, rYlo, rYhi)) { if (first) d.slot_flag[slot_base + t] = 0; continue; }
+      if (!box_overlap(lo, hi, rYlo, rYhi) || beyond(lo, hi)) { if (first) d.slot_flag[slot_base + t] = 0; continue; }
       unsigned cnt = 0;
       for (int k = 0; k < ncl && !(flag && !count_all); ++k) {
         if (!box_overlap(lo, hi, clo_s[k], chi_s[k])) continue;

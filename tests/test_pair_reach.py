@@ -72,3 +72,32 @@ def test_pairs_beyond_reach_give_no_contact(bank, kind):
                 break
     assert len(closest) == 8
     print(kind, "limit", limit, "first contact within", max(closest), "of it")
+
+
+def test_contact_triangles_lie_within_reach(bank):
+    """The per-triangle form of the bound: every contact the oracle finds in a
+    pile of boxes and spheres comes from a triangle whose box (grown by the
+    envelope) is within reach + envelope + cell diagonal of the SDF owner's
+    origin, so skipping triangles beyond it loses no contact."""
+    rng = np.random.default_rng(5)
+    eng = scenes.new_engine(bank, dict(time_step=0.01), backend=None)
+    scenes.set_friction(eng, 0.5)
+    shapes = [bank.get("sphere_shape", lambda: MESH.create_sphere(0.5, 16, 16)),
+              bank.get("box_shape", lambda: MESH.create_box(1.0, 1.0, 1.0))]
+    for i in range(14):
+        scenes.add_body(eng, "b%d" % i, shapes[i % 2], rng.uniform(-1.0, 1.0, 3), q=random_unit(rng, 4), forces=())
+    sc = OS.from_engine(eng)
+    c = OC.detect(sc, {})
+    assert c.K > 20
+    bodies = list(eng.bodies.values())
+    env = eng.params.envelope
+    for a, b, t in zip(c.ia, c.ib, c.tri):
+        reach, hd = reach_and_diag(bodies[a].shape)
+        lim = reach + hd + env
+        V = np.asarray(bodies[b].shape.mesh.V, dtype=np.float64)
+        T = np.asarray(bodies[b].shape.mesh.T)
+        R = Q.to_matrix(sc.q[b])
+        P = (R @ V[T[t]].T).T + sc.r[b]
+        lo, hi = P.min(axis=0) - env, P.max(axis=0) + env
+        gap = np.maximum(np.maximum(lo - sc.r[a], sc.r[a] - hi), 0.0)
+        assert np.sqrt((gap * gap).sum()) <= lim, (a, b, t)

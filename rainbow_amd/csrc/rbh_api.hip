@@ -131,7 +131,8 @@ struct rbh_ctx {
   std::vector<int> sT;      // 4 per triangle
   std::vector<double> sdf;
   bool shapes_dirty = true;
-  DBuf d_shapes, d_sV, d_sT, d_sdf;
+  DBuf d_shapes, d_sV, d_sT, d_sdf, d_cells;
+  long long cells_total = 0;   // values of the cell-major grid copies of all shapes
   // body arena
   RangeAlloc bodies, tris, res;
   DBuf r, q, v, w, b_shape, b_slot, b_fixed, b_mat, b_tri, b_mass, b_inertia, b_grav, b_damp;
@@ -231,7 +232,7 @@ int ensure(rbh_ctx* c, DBuf& b, size_t bytes, bool keep = false) {
 }
 
 std::vector<DBuf*> all_bufs(rbh_ctx* c) {
-  return {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
+  return {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->d_cells, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
           &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
           &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi, &c->clo, &c->chi,
           &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
@@ -340,7 +341,15 @@ template <typename R> int sync_shapes(rbh_ctx* c) {
   if (!rc) rc = upload(c, c->d_sV, v);
   if (!rc) rc = upload(c, c->d_sT, c->sT);
   if (!rc) rc = upload(c, c->d_sdf, g);
+  if (!rc) rc = ensure(c, c->d_cells, (size_t)std::max<long long>(c->cells_total, 8) * sizeof(R));
   if (rc) return rc;
+  for (const ShapeInfo& s : c->shapes) {
+    const long long nc = (long long)(s.I - 1) * (s.J - 1) * (s.K - 1);
+    hipLaunchKernelGGL(k_sdf_cells<R>, dim3((unsigned)std::min<long long>((nc + 255) / 256, 65535)), dim3(256), 0,
+                       c->stream, reinterpret_cast<const R*>(c->d_sdf.p) + s.g_off, s.I, s.J, s.K,
+                       reinterpret_cast<R*>(c->d_cells.p) + s.c_off);
+  }
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->shapes_dirty = false;
   c->arena_gen++;
@@ -458,6 +467,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
 #define P(T, b) reinterpret_cast<T>(c->b.p)
   d.shapes = P(const ShapeInfo*, d_shapes); d.sV = P(const V4<R>*, d_sV); d.sT = P(const int4*, d_sT);
   d.sdf = P(const R*, d_sdf);
+  d.cells = P(const R*, d_cells);
   d.sp = P(const SceneParams*, d_sp); d.ss = P(SceneState*, d_ss); d.residuals = P(double*, d_res);
   d.mat = P(const double*, d_mat);
   d.r = P(V4<R>*, r); d.q = P(V4<R>*, q); d.v = P(V4<R>*, v); d.w = P(V4<R>*, w);
@@ -1235,6 +1245,8 @@ int rbh_shape_upload(rbh_ctx* c, const double* V, int nv, const int* T, int nt, 
     s.h[a] = (mx[a] - mn[a]) / (double)res[a];   // grid3.py:42-44
   }
   s.hdiag = std::sqrt(s.h[0] * s.h[0] + s.h[1] * s.h[1] + s.h[2] * s.h[2]);
+  s.c_off = c->cells_total;
+  c->cells_total += 8LL * (I - 1) * (J - 1) * (K - 1);
   s.reach = 0.0;
   for (int i = 0; i < nv; ++i)
     s.reach = std::max(s.reach, std::sqrt(V[3 * i] * V[3 * i] + V[3 * i + 1] * V[3 * i + 1] + V[3 * i + 2] * V[3 * i + 2]));

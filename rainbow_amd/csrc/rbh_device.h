@@ -134,6 +134,7 @@ template <typename R> __device__ __forceinline__ void orthonormal_frame(V3<R> ni
 // ---------------------------------------------------------------- SDF grids
 template <typename R> struct GridView {
   const R* val;
+  const R* cells;   // cell-major copy: cell (i, j, k) holds its 8 corners in the order of load_cell
   int I, J, K;
   R mn[3], mx[3], h[3];
 };
@@ -155,21 +156,31 @@ template <typename R> __device__ __forceinline__ void cell_idx(const GridView<R>
 
 template <typename R> struct Cell { R d000, d001, d010, d011, d100, d101, d110, d111; };
 
+// The 8 corner values of cell (i, j, k), read as two vec4 from the
+// cell-major copy of the grid (k_sdf_cells) instead of 8 scattered values:
+// (i,j,k), (i+1,j,k), (i,j+1,k), (i+1,j+1,k), then the same at k+1.
 template <typename R> __device__ __forceinline__ Cell<R> load_cell(const GridView<R>& g, int i, int j, int k) {
-  const int I = g.I, J = g.J;
-  const R* v = g.val;
-  int b = i + I * (j + J * k);
-  int dj = I, dk = I * J;
+  const V4<R>* p = reinterpret_cast<const V4<R>*>(g.cells) +
+                   2 * ((size_t)i + (size_t)(g.I - 1) * ((size_t)j + (size_t)(g.J - 1) * (size_t)k));
+  const V4<R> a = p[0], e = p[1];
   Cell<R> c;
-  c.d000 = v[b];
-  c.d001 = v[b + dk];
-  c.d010 = v[b + dj];
-  c.d011 = v[b + dj + dk];
-  c.d100 = v[b + 1];
-  c.d101 = v[b + 1 + dk];
-  c.d110 = v[b + 1 + dj];
-  c.d111 = v[b + 1 + dj + dk];
+  c.d000 = a.x; c.d100 = a.y; c.d010 = a.z; c.d110 = a.w;
+  c.d001 = e.x; c.d101 = e.y; c.d011 = e.z; c.d111 = e.w;
   return c;
+}
+
+// cell-major copy of one grid (values v, I x J x K nodes, index i + I (j + J k))
+template <typename R> __global__ void k_sdf_cells(const R* __restrict__ v, int I, int J, int K, R* __restrict__ out) {
+  const long long nc = (long long)(I - 1) * (J - 1) * (K - 1);
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(c % (I - 1));
+    const long long r = c / (I - 1);
+    const int j = (int)(r % (J - 1)), k = (int)(r / (J - 1));
+    const long long b = i + (long long)I * (j + (long long)J * k), dj = I, dk = (long long)I * J;
+    R* o = out + 8 * c;
+    o[0] = v[b]; o[1] = v[b + 1]; o[2] = v[b + dj]; o[3] = v[b + 1 + dj];
+    o[4] = v[b + dk]; o[5] = v[b + 1 + dk]; o[6] = v[b + dj + dk]; o[7] = v[b + 1 + dj + dk];
+  }
 }
 
 // grid3.py:126-177

@@ -20,6 +20,7 @@ struct ShapeInfo {
   // reach: largest distance of a vertex from the body-frame origin; hdiag:
   // the SDF cell diagonal (sqrt(hx^2 + hy^2 + hz^2))
   double reach, hdiag;
+  long long c_off;   // first value of the shape's cell-major copy of its grid (8 per cell)
 };
 
 enum Scheme { SCHEME_GS = 0, SCHEME_PGS = 1, SCHEME_JACOBI = 2, SCHEME_JACOBI_HYBRID = 3 };
@@ -60,6 +61,7 @@ enum ErrBits { ERR_CAP_PAIRS = 1, ERR_CAP_SLOTS = 2, ERR_CAP_CAND = 4, ERR_CAP_C
 template <typename R> struct Dev {
   // shapes
   const ShapeInfo* shapes; const V4<R>* sV; const int4* sT; const R* sdf;
+  const R* cells;                      // per shape, per cell (i, j, k) its 8 corner values (cell-major grid)
   // scenes (by slot)
   const SceneParams* sp; SceneState* ss; double* residuals; const double* mat;   // mat: (mu0,mu1,mu2,eps) per pair
   // body arena
@@ -154,6 +156,7 @@ template <typename R> __device__ __forceinline__ GridView<R> grid_of(const Dev<R
   const ShapeInfo& s = d.shapes[shape];
   GridView<R> g;
   g.val = d.sdf + s.g_off;
+  g.cells = d.cells + s.c_off;
   g.I = s.I; g.J = s.J; g.K = s.K;
   for (int i = 0; i < 3; ++i) { g.mn[i] = R(s.mn[i]); g.mx[i] = R(s.mx[i]); g.h[i] = R(s.h[i]); }
   return g;

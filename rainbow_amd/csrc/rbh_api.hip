@@ -155,7 +155,7 @@ struct rbh_ctx {
   std::vector<int> batch;
   long long batch_arena_gen = -1;
   int nb = 0, ns = 0;
-  DBuf bb_body, bs_slot, bs_foff, b2f, f_end, d_nb;
+  DBuf bb_body, bs_slot, bs_order, bs_foff, b2f, f_end, d_nb;
   DBuf row_cnt, row_off, pairs, pair_sz, slot_off, n_pairs, slot_flag, n_slots, pair_items, item_off;
   DBuf side_cnt, side_off, n_sides;
   DBuf cand_pair, cand_tri, pair_cstart, n_cand, cand_p, cand_n, cand_keep, keep_scan;
@@ -235,7 +235,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
   return {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->d_cells, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
           &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
           &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi, &c->clo, &c->chi,
-          &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_foff, &c->b2f,
+          &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_order, &c->bs_foff, &c->b2f,
           &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs, &c->pair_items, &c->item_off,
           &c->slot_flag, &c->side_cnt, &c->side_off, &c->n_sides, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
           &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->kidx, &c->kpos,
@@ -479,7 +479,8 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.ww0 = P(V4<R>*, ww0); d.ww1 = P(V4<R>*, ww1); d.tlo = P(V4<R>*, tlo); d.thi = P(V4<R>*, thi);
   d.clo = P(V4<R>*, clo); d.chi = P(V4<R>*, chi);
   d.nb = c->nb; d.ns = c->ns;
-  d.bb_body = P(const int*, bb_body); d.bs_slot = P(const int*, bs_slot); d.bs_foff = P(const int*, bs_foff);
+  d.bb_body = P(const int*, bb_body); d.bs_slot = P(const int*, bs_slot);
+  d.bs_order = P(const int*, bs_order); d.bs_foff = P(const int*, bs_foff);
   d.b2f = P(const int*, b2f); d.f_end = P(const int*, f_end); d.d_nb = P(const int*, d_nb);
   d.row_cnt = P(int*, row_cnt); d.row_off = P(int*, row_off);
   d.pairs = P(int2*, pairs); d.pair_sz = P(int*, pair_sz); d.slot_off = P(int*, slot_off);
@@ -609,6 +610,14 @@ int build_batch(rbh_ctx* c, const int* ids, int n) {
   std::vector<int> nbv = {c->nb, c->nb + c->ns + 1, c->ns, 0};
   int rc = upload(c, c->bb_body, bb);
   if (!rc) rc = upload(c, c->bs_slot, bsslot);
+  // the batched solve takes the largest scenes first: a workgroup per scene,
+  // so a large scene started last would set the step's length alone
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return c->scenes[want[a]].nbody > c->scenes[want[b]].nbody;
+  });
+  if (!rc) rc = upload(c, c->bs_order, order);
   if (!rc) rc = upload(c, c->bs_foff, bsfoff);
   if (!rc) rc = upload(c, c->b2f, b2f);
   if (!rc) rc = upload(c, c->f_end, fend);

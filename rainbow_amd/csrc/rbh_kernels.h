@@ -70,7 +70,7 @@ template <typename R> struct Dev {
   const R* b_mass; const V4<R>* b_inertia; const V4<R>* b_grav; const V4<R>* b_damp;
   V4<R>* b_W0; V4<R>* b_W1;            // (invI xx,yy,zz,xy), (invI xz,yz, 1/m, 0)
   V4<R>* b_du0; V4<R>* b_du1;          // W f_ext (linear, angular)
-  V4<R>* b_lo; V4<R>* b_hi;            // world AABB grown by the envelope
+  V4<R>* b_lo; V4<R>* b_hi;            // world AABB grown by the envelope; w: the shape's reach, cell diagonal
   V4<R>* ws0; V4<R>* ws1;              // accepted W J^T sol
   V4<R>* ww0; V4<R>* ww1;              // working copy during a sweep
   V4<R>* tlo; V4<R>* thi;              // per body-instance triangle AABBs, in cluster order (b_tri + position)
@@ -80,6 +80,7 @@ template <typename R> struct Dev {
   int nb, ns;
   const int* bb_body;                  // flat batch body -> arena body
   const int* bs_slot;                  // batch scene -> slot
+  const int* bs_order;                 // batch scenes by decreasing body count (the batched solve's workgroup order)
   const int* bs_foff;                  // batch scene -> first flat body
   const int* b2f;                      // arena body -> flat batch body
   const int* f_end;                    // flat batch body -> arena end of its scene
@@ -292,9 +293,10 @@ template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
     lx = fmin(lx, __shfl_xor(lx, o, 64)); ly = fmin(ly, __shfl_xor(ly, o, 64)); lz = fmin(lz, __shfl_xor(lz, o, 64));
     hx = fmax(hx, __shfl_xor(hx, o, 64)); hy = fmax(hy, __shfl_xor(hy, o, 64)); hz = fmax(hz, __shfl_xor(hz, o, 64));
   }
+  // w: the shape's reach and cell diagonal, read beside the box by the pair kernels
   if (threadIdx.x == 0) {
-    d.b_lo[b] = V4<R>{lx, ly, lz, R(0)};
-    d.b_hi[b] = V4<R>{hx, hy, hz, R(0)};
+    d.b_lo[b] = V4<R>{lx, ly, lz, R(sh.reach)};
+    d.b_hi[b] = V4<R>{hx, hy, hz, R(sh.hdiag)};
   }
 }
 
@@ -313,18 +315,18 @@ template <typename R> __global__ void __launch_bounds__(64) k_refit(Dev<R> d) {
 // therefore give no contact, in either direction, and are dropped before
 // their triangles are compared.  Counting triangle overlaps
 // (STEP_COUNT_OVERLAPS, the profile key number_of_overlaps) keeps every pair.
-template <typename R> __device__ __forceinline__ bool within_reach(const Dev<R>& d, int a, int b, R env) {
-  const ShapeInfo& sa = d.shapes[d.b_shape[a]];
-  const ShapeInfo& sb = d.shapes[d.b_shape[b]];
-  const R lim = (R(sa.reach) + R(sb.reach) + env + R(fmax(sa.hdiag, sb.hdiag))) * R(1.001) + R(1e-4);
-  const V4<R> ra = d.r[a], rb = d.r[b];
+// (a's reach and cell diagonal ride in alo.w / ahi.w, b's in blo.w / bhi.w)
+template <typename R> __device__ __forceinline__ bool within_reach(V4<R> ra, V4<R> rb, V4<R> alo, V4<R> ahi, V4<R> blo,
+                                                                  V4<R> bhi, R env) {
+  const R lim = (alo.w + blo.w + env + fmax(ahi.w, bhi.w)) * R(1.001) + R(1e-4);
   const R dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
   return dx * dx + dy * dy + dz * dz <= lim * lim;
 }
 
-template <typename R> __device__ __forceinline__ bool pair_kept(const Dev<R>& d, int a, int b, V4<R> alo, V4<R> ahi,
+template <typename R> __device__ __forceinline__ bool pair_kept(const Dev<R>& d, V4<R> ra, int b, V4<R> alo, V4<R> ahi,
                                                                 R env, bool all) {
-  return box_overlap(alo, ahi, d.b_lo[b], d.b_hi[b]) && (all || within_reach(d, a, b, env));
+  const V4<R> blo = d.b_lo[b], bhi = d.b_hi[b];
+  return box_overlap(alo, ahi, blo, bhi) && (all || within_reach(ra, d.r[b], alo, ahi, blo, bhi, env));
 }
 
 template <typename R> __global__ void __launch_bounds__(64) k_pair_count(Dev<R> d) {
@@ -337,7 +339,8 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_count(Dev<R> 
   const R env = R(d.sp[d.b_slot[b]].envelope);
   const bool all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   int cnt = 0;
-  for (int j = b + 1 + threadIdx.x; j < end; j += 64) cnt += pair_kept(d, b, j, alo, ahi, env, all) ? 1 : 0;
+  const V4<R> ra = d.r[b];
+  for (int j = b + 1 + threadIdx.x; j < end; j += 64) cnt += pair_kept(d, ra, j, alo, ahi, env, all) ? 1 : 0;
   cnt = wave_sum(cnt);
   if (threadIdx.x == 0) d.row_cnt[f] = cnt;
 }
@@ -357,10 +360,11 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
   V4<R> alo = d.b_lo[b], ahi = d.b_hi[b];
   const R env = R(d.sp[d.b_slot[b]].envelope);
   const bool all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
+  const V4<R> ra = d.r[b];
   const int ntA = d.shapes[d.b_shape[b]].nt;
   for (int j0 = b + 1; j0 < end; j0 += 64) {
     int j = j0 + threadIdx.x;
-    bool hit = j < end && pair_kept(d, b, j, alo, ahi, env, all);
+    bool hit = j < end && pair_kept(d, ra, j, alo, ahi, env, all);
     unsigned long long m = __ballot(hit);
     if (hit) {
       int pos = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));

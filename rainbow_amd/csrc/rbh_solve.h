@@ -1917,7 +1917,7 @@ template <typename R, int NT> __global__ void __launch_bounds__(NT) k_solve(Dev<
   __shared__ double sh_red[NT / 64];
   __shared__ int s_ok;
   if (aborted(d.err)) return;
-  const int bs = blockIdx.x;
+  const int bs = d.bs_order[blockIdx.x];
   const int slot = d.bs_slot[bs];
   const SceneParams& sp = d.sp[slot];
   SceneState& st = d.ss[slot];

@@ -416,7 +416,9 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 // Flags of X's positions [x0, x1) (one block) w.r.t. Y.
 template <typename R>
-__device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, int x1, bool count_all,
+// count_all: count every overlapping triangle pair; exact: the reference's
+// candidate set, without the reach bound (set while overlaps are counted)
+__device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, int x1, bool count_all, bool exact,
                            V4<R>* lo_s, V4<R>* hi_s, V4<R>* clo_s, V4<R>* chi_s, int2* crange_s,
                            unsigned long long& overlaps) {
   const int cap = CandLds<R>::CAP;
@@ -438,7 +440,7 @@ __device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0,
     const R ex = fmax(fmax(lo.x - oY.x, oY.x - hi.x), R(0));
     const R ey = fmax(fmax(lo.y - oY.y, oY.y - hi.y), R(0));
     const R ez = fmax(fmax(lo.z - oY.z, oY.z - hi.z), R(0));
-    return !count_all && ex * ex + ey * ey + ez * ez > lim * lim;
+    return !exact && ex * ex + ey * ey + ez * ez > lim * lim;
   };
   if (!box_overlap(xlo, xhi, rYlo, rYhi) || beyond(xlo, xhi)) {
     for (int p = x0 + lane; p < x1; p += 64) d.slot_flag[slot_base + d.sT[sX.t_off + p].w] = 0;
@@ -592,7 +594,7 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
     const int x0 = blk * CAND_BLK, x1 = min(x0 + CAND_BLK, side0 ? ntA : ntB);
     unsigned long long ov = 0;
     const int slot_base = d.slot_off[p] + (side0 ? 0 : ntA);
-    cand_block(d, X, Y, slot_base, x0, x1, count_all && side0, lo_s[wave], hi_s[wave],
+    cand_block(d, X, Y, slot_base, x0, x1, count_all && side0, count_all, lo_s[wave], hi_s[wave],
                clo_s[wave], chi_s[wave], crange_s[wave], ov);
     // candidates of this block (each lane reads back the flags it wrote last)
     const int tXo = d.shapes[d.b_shape[X]].t_off;

@@ -331,6 +331,30 @@ def test_graph_replay_equals_eager(bank):
                 assert np.array_equal(x, y)
 
 
+def test_candidate_set_exact_when_counting(bank):
+    """collision_detection.py:47-84, 204-252 (kdop_bvh traversal): counting
+    every triangle overlap (RBH_STEP_COUNT_OVERLAPS) gives the reference's
+    candidate set, whose size equals the oracle's.  Without it the reach
+    bound drops pairs and triangles that cannot give a contact: fewer
+    candidates, the same contacts."""
+    import oracle.collision as OC
+    eng = mixed_scene(bank, 12, 3, sink=0.3, params=dict(precision="fp64"), spread=1.5)
+    stats = {}
+    oc = OC.detect(OS.from_engine(eng), stats)
+    st = eng.stepper
+    runs = []
+    for flags in (native.STEP_COUNT_OVERLAPS, 0):
+        st._prepare(eng)                       # the same start state both times
+        st.ctx.step([st.scene], 1, 0.01, flags)
+        runs.append((st.ctx.get_counters()["candidates"], st.ctx.get_contacts(st.scene)))
+    (n_exact, c_exact), (n_pruned, c_pruned) = runs
+    print("candidates: oracle %d, counting %d, with the reach bound %d" % (stats["candidates"], n_exact, n_pruned))
+    assert n_exact == stats["candidates"] and 0 < n_pruned <= n_exact
+    assert oc.K > 0
+    for c in (c_exact, c_pruned):
+        assert np.array_equal(c["ia"], oc.ia) and np.array_equal(c["ib"], oc.ib) and np.array_equal(c["tri"], oc.tri)
+
+
 def test_simulate_and_log_keys(bank):
     eng = scenes.box_stack(bank, params=dict(max_iterations=20))
     API.simulate(eng, 0.03, profiling_on=True)

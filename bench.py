@@ -248,6 +248,15 @@ def batched_leg(n_scenes, steps, settle, plain_sample, plain_steps, cpu=True):
     citers = sum(t["contact_iters"] for t in tot)
     contacts = sum(t["contacts"] for t in tot)
     res = np.array([ctx.get_log(sid, 30)["natural_residual"] for sid in sids])
+    # one more batch step launched kernel by kernel with stage events: the batched
+    # solve kernel's duration for its roofline entry (a graph replay has no stage marks)
+    for sid in sids:
+        ctx.scene_totals(sid, clear=True)
+    ctx.step(sids, 1, 0.0, native.STEP_TIMING)
+    stage = ctx.get_timings()
+    solve_ci = sum(ctx.scene_totals(sid)["contact_iters"] for sid in sids)
+    solve_ms = 1e3 * stage["solver_time"]
+    solve_gbs = BYTES_PER_CONTACT_ITER * solve_ci / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     end_states = {i: ctx.get_state(sids[i], init[i][3]) for i in range(n_scenes)} if cpu else None
     # plain path: the sample scenes from their states at step `settle`, one scene per launch
     dup = []
@@ -284,6 +293,11 @@ def batched_leg(n_scenes, steps, settle, plain_sample, plain_steps, cpu=True):
            "p50_step_ms": 1e3 * float(np.percentile(lat, 50)), "p99_step_ms": 1e3 * float(np.percentile(lat, 99)),
            "contact_iters_per_s": citers / tb if tb > 0 else 0.0,
            "contacts_per_scene_step": contacts / (n_scenes * steps),
+           "roofline": {"kernel": "k_solve", "bound": "hbm", "achieved": solve_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": solve_gbs / HBM_PEAK_GBS, "bytes_per_launch": BYTES_PER_CONTACT_ITER * solve_ci,
+                        "launch_ms": solve_ms,
+                        "note": "one eager batch step after the timed ones; 288 B per contact-iteration"},
+           "stages_ms": {k: round(1e3 * v, 4) for k, v in stage.items()},
            "natural_residual_last_step": {"median": float(np.median(res)), "max": float(res.max())},
            "plain": {"scene_steps_per_s": plain, "sample_scenes": len(dup), "steps": plain_steps,
                      "steps_timed": "%d..%d" % (settle + 1, at),
@@ -293,7 +307,7 @@ def batched_leg(n_scenes, steps, settle, plain_sample, plain_steps, cpu=True):
            "tuning": tuning_report(tuned)}
     if cpu:
         log("batched: cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline_scenes(engs, end_states, at + steps + 10, "mixed_drop")
+        out["cpu_baseline"] = cpu_baseline_scenes(engs, end_states, at + steps + 11, "mixed_drop")
     return out
 
 

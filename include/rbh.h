@@ -121,6 +121,9 @@ typedef struct rbh_tuning {
   int waves_per_group;    /* wavefronts per workgroup of the batched solve: 4 (default) or 8 */
   int color_lds_contacts; /* most contacts of a colouring unit whose colouring state is kept in LDS
                              (default and upper bound 4096; larger units keep it in global memory) */
+  int color_body_bits;    /* most words of each per-body colour bitmap array of a unit of at most 256
+                             contacts (default and upper bound 768; a unit needing more finds its
+                             contacts' neighbours through its bodies' contact lists) */
 } rbh_tuning;
 
 typedef struct rbh_device_info {

@@ -26,7 +26,7 @@ import time
 from . import native
 
 DEFAULT = dict(unit_size=0, lanes_per_scene=0, resident_contacts=0, resident_bodies=0,
-               lanes_per_group=0, waves_per_group=0, color_lds_contacts=0)
+               lanes_per_group=0, waves_per_group=0, color_lds_contacts=0, color_body_bits=0)
 
 
 def batch_candidates():

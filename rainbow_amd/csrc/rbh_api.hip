@@ -507,6 +507,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.unit_split = multi ? 0 : (c->tune.lanes_per_scene > 0 ? c->tune.lanes_per_scene : batch_waves(c));
   d.u_T = P(int*, u_T);
   d.color_cap = c->tune.color_lds_contacts > 0 ? c->tune.color_lds_contacts : UNIT_CAP;
+  d.color_bits = c->tune.color_body_bits > 0 ? std::min(c->tune.color_body_bits, BODY_BITS) : BODY_BITS;
   d.isl_unit = P(int*, isl_unit); d.u_nbody = P(int*, u_nbody); d.u_boff = P(int*, u_boff);
   d.f_loc = P(int*, f_loc); d.f_unit = P(int*, f_unit); d.u_blist = P(int*, u_blist);
   d.s_loc = P(int2*, s_loc); d.s_W = P(V4<R>*, s_W);
@@ -1585,7 +1586,8 @@ int rbh_set_tuning(rbh_ctx* c, const rbh_tuning* t) {
   if (t->unit_size < 0 || t->unit_size > 4096 || t->lanes_per_scene < 0 || t->lanes_per_scene > 64 ||
       t->resident_contacts < 0 || t->resident_bodies < 0 || t->lanes_per_group < 0 || t->lanes_per_group > 64 ||
       (t->waves_per_group != 0 && t->waves_per_group != 4 && t->waves_per_group != 8) ||
-      (t->resident_contacts % 64) || (t->resident_bodies % 64) || t->color_lds_contacts < 0)
+      (t->resident_contacts % 64) || (t->resident_bodies % 64) || t->color_lds_contacts < 0 ||
+      t->color_body_bits < 0)
     return fail(c, RBH_E_ARG, "tuning values out of range (resident capacities are multiples of 64)");
   const int cap = c->prec == 4 ? ResCap<float>::C : ResCap<double>::C;
   if (t->resident_contacts > cap || t->resident_bodies > cap)

@@ -108,6 +108,7 @@ template <typename R> struct Dev {
   int* u_cnt; int* u_base; int* n_units; int cap_units; int unit_T; int unit_split;
   int* u_T;                            // sweep-lane size of each batch scene
   int color_cap;                       // most contacts of a unit coloured with its state in LDS
+  int color_bits;                      // most words of each per-body colour bitmap array of a small unit
   int* l_cnt; int* l_base; int* n_lanes;   // sweep lanes per batch scene, their scan, total
   int* l_start; int* l_end; int* l_ncol; int* l_bs;   // per lane (capacity cap_units: lanes <= units)
   int* u_start; int* u_end; int* u_ncol; int* u_bs;

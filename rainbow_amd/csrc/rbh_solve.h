@@ -211,10 +211,13 @@ template <typename R> __global__ void k_csr_fill(Dev<R> d) {
 // contacts are adjacent if they share a free body.  Units of up to
 // SMALL_UNIT contacts are coloured by one wavefront with the palettes in LDS
 // (ceil((2K + 2) / 32) words per contact: colours of an island stay below
-// twice its size); larger units by a workgroup of eight wavefronts, palettes
-// in global memory (PALW words per contact), the rest of the state in LDS up
-// to UNIT_CAP contacts (gfx950: 160 KB of LDS per workgroup), in global
-// memory beyond.
+// twice its size) and, per free body of the unit, bitmaps of the colours its
+// contacts drew and won, so that a round costs O(palette words) per contact;
+// larger units (and small ones whose bitmaps would not fit) by a workgroup of
+// eight wavefronts, palettes in global memory (PALW words per contact), the
+// neighbours found through the bodies' contact lists, the rest of the state
+// in LDS up to UNIT_CAP contacts (gfx950: 160 KB of LDS per workgroup), in
+// global memory beyond.
 constexpr int COLOR_BLOCK = 256;
 constexpr int COLOR_UNIT_BLOCK = 512;   // threads colouring one large unit (8 wavefronts)
 constexpr int UNIT_CAP = 4096;        // contacts of a large unit staged in LDS
@@ -223,6 +226,27 @@ constexpr int SMALL_UNIT = 256;
 constexpr int SMALL_BODY = 32;        // bodies with at most this many contacts are checked entry by entry
 constexpr int SMALL_LIST = 1024;
 constexpr int SMALL_PAL = SMALL_UNIT * ((2 * SMALL_UNIT + 2 + 31) / 32);
+constexpr int BODY_BITS = 768;        // words of each per-body colour bitmap array of a small unit
+constexpr int PAIR_HASH = 2 * SMALL_UNIT;   // body-pair table of a small unit's set-up (power of two)
+constexpr unsigned short NO_SLOT = 0xffff;
+
+// State of a small unit coloured through per-body colour bitmaps.  Body b of
+// the unit (its slot f_loc) owns words [b * bw, (b + 1) * bw) of each bitmap
+// array, bw = palette words in use.
+struct BitsColorLds {
+  int key[SMALL_UNIT];                // contact index within its scene (draws, ranks)
+  int isl[SMALL_UNIT];                // island label
+  unsigned short la[SMALL_UNIT], lb[SMALL_UNIT];   // unit slots of the free bodies (NO_SLOT: fixed)
+  unsigned short col[SMALL_UNIT], pc[SMALL_UNIT];
+  unsigned char done[SMALL_UNIT];     // 0 uncoloured, 1 coloured, 2 won this round
+  unsigned pal[SMALL_PAL];
+  union {
+    struct { unsigned seen[BODY_BITS], dup[BODY_BITS], won[BODY_BITS]; } m;   // rounds
+    struct { int cnt[2 * SMALL_UNIT]; unsigned hk[PAIR_HASH]; int hc[PAIR_HASH]; } s;   // set-up
+  } u;
+  int mx, cmax;
+};
+static_assert(sizeof(BitsColorLds::u.s) <= sizeof(BitsColorLds::u.m), "set-up tables share the bitmaps' space");
 
 template <int NT, int CAP, int LCAP, int PAL> struct ColorLds {
   // colours a palette can hold (at most PALMAX in either palette layout)
@@ -681,15 +705,261 @@ __device__ void color_unit(const Dev<R>& d, int u, ColorLds<NT, CAP, LCAP, PAL>&
 static_assert(sizeof(ColorLds<COLOR_UNIT_BLOCK, UNIT_CAP, LIST_CAP, 0>) <= 160 * 1024 - 2048,
               "large-unit colouring state must fit a workgroup's LDS");
 
-// units of up to SMALL_UNIT contacts: one wavefront each
+// idx-th set bit of an LDS palette of pwe words (the draw's colour), or -1
+__device__ __forceinline__ int lds_select(const unsigned* p, int pwe, int idx) {
+  for (int w = 0; w < pwe; ++w) {
+    unsigned m = p[w];
+    const int c = __popc(m);
+    if (idx < c) {   // halve the word until the bit is found
+      int pos = 0, lo;
+      lo = __popc(m & 0xffffu); if (idx >= lo) { idx -= lo; m >>= 16; pos += 16; }
+      lo = __popc(m & 0xffu);   if (idx >= lo) { idx -= lo; m >>= 8; pos += 8; }
+      lo = __popc(m & 0xfu);    if (idx >= lo) { idx -= lo; m >>= 4; pos += 4; }
+      lo = __popc(m & 0x3u);    if (idx >= lo) { idx -= lo; m >>= 2; pos += 2; }
+      if (idx >= (int)(m & 1u)) pos += 1;
+      return w * 32 + pos;
+    }
+    idx -= c;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ unsigned pair_hash(unsigned key) { return (key * 2654435761u) >> 23; }   // 9 bits
+
+// The rounds of color_rounds for a unit of up to SMALL_UNIT contacts by one
+// wavefront, with a contact's neighbours met through per-body bitmaps instead
+// of its bodies' contact lists:
+//  * degree = (contacts of body a - 1) + (contacts of body b - 1) - (other
+//    contacts between a and b), from per-body counts and a body-pair table;
+//  * a draw is lost iff one of its free bodies has the colour in `dup` (drawn
+//    by two of its uncoloured contacts in this round);
+//  * the colours won on either free body leave the palette of a contact still
+//    uncoloured (its palette is read and written by its own lane only).
+// The same draws, winners, palettes, fresh colours and leftovers as
+// color_rounds, so the same colours.  Returns false, having written nothing
+// outside LDS, when the unit is not for this path (serial scheme, colouring
+// state forced to global memory) or its bitmaps outgrow color_bits words;
+// color_unit then colours it from the start.
+template <typename R>
+__device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
+  const int p0 = d.u_start[u], p1 = d.u_end[u], K = p1 - p0;
+  const int bs = d.u_bs[u], slot = d.bs_slot[bs];
+  const SceneParams& sp = d.sp[slot];
+  const SceneState& st = d.ss[slot];
+  const int nbu = d.u_nbody[u];
+  if (sp.scheme == SCHEME_GS || K > d.color_cap || nbu > 2 * SMALL_UNIT) return false;
+  const int lane = threadIdx.x;
+  const int k_off = st.k_off;
+  const unsigned long long seed = sp.seed, stepk = (unsigned long long)st.step_index;
+  const int pw = (2 * K + 2 + 31) / 32, pcap = 32 * pw;
+  const int cap = d.color_bits;
+  unsigned* const raw = s.u.m.seen;   // the whole union as words
+  for (int w = lane; w < 3 * BODY_BITS; w += 64) raw[w] = 0;
+  if (lane == 0) { s.mx = 0; s.cmax = 0; }
+  __syncthreads();
+  // contacts: key, island, body slots; contacts per body and per body pair
+  for (int i = lane; i < K; i += 64) {
+    const int k = d.uperm[p0 + i];
+    const int A = d.c_ia[k], B = d.c_ib[k];
+    const int fa = d.b_fixed[A] ? -1 : d.b2f[A], fb = d.b_fixed[B] ? -1 : d.b2f[B];
+    const int la = fa >= 0 ? d.f_loc[fa] : NO_SLOT, lb = fb >= 0 ? d.f_loc[fb] : NO_SLOT;
+    s.key[i] = k - k_off;
+    s.isl[i] = d.c_lab[k];
+    s.la[i] = (unsigned short)la;
+    s.lb[i] = (unsigned short)lb;
+    if (la != NO_SLOT) atomicAdd(&s.u.s.cnt[la], 1);
+    if (lb != NO_SLOT) atomicAdd(&s.u.s.cnt[lb], 1);
+    if (la != NO_SLOT && lb != NO_SLOT) {
+      const unsigned key = (((unsigned)min(la, lb) << 10) | (unsigned)max(la, lb)) + 1u;
+      unsigned h = pair_hash(key);
+      for (int t = 0; t < PAIR_HASH; ++t, h = (h + 1) & (PAIR_HASH - 1)) {
+        const unsigned old = atomicCAS(&s.u.s.hk[h], 0u, key);
+        if (old == 0u || old == key) { atomicAdd(&s.u.s.hc[h], 1); break; }
+      }
+    }
+  }
+  __syncthreads();
+  // palettes {0, ..., deg - 1}
+  for (int i = lane; i < K; i += 64) {
+    const int la = s.la[i], lb = s.lb[i];
+    int deg = 0;
+    if (la != NO_SLOT) deg += s.u.s.cnt[la] - 1;
+    if (lb != NO_SLOT) deg += s.u.s.cnt[lb] - 1;
+    if (la != NO_SLOT && lb != NO_SLOT) {
+      const unsigned key = (((unsigned)min(la, lb) << 10) | (unsigned)max(la, lb)) + 1u;
+      unsigned h = pair_hash(key);
+      for (int t = 0; t < PAIR_HASH; ++t, h = (h + 1) & (PAIR_HASH - 1))
+        if (s.u.s.hk[h] == key) { deg -= s.u.s.hc[h] - 1; break; }
+    }
+    if (deg > 0) atomicMax(&s.mx, min(deg, pcap));
+    if (deg > pcap) atomicOr(d.err, ERR_COLORS);
+    unsigned* p = s.pal + i * pw;
+    for (int w = 0; w < pw; ++w) {
+      const int lo = w * 32;
+      p[w] = deg >= lo + 32 ? 0xffffffffu : deg > lo ? (1u << (deg - lo)) - 1u : 0u;
+    }
+    s.pc[i] = (unsigned short)min(deg, pcap);
+    s.col[i] = 0;
+    s.done[i] = 0;
+  }
+  __syncthreads();
+  for (int w = lane; w < 3 * BODY_BITS; w += 64) raw[w] = 0;
+  int pwe = (s.mx + 31) / 32;   // palette words in use
+  __syncthreads();
+  if (nbu * max(pwe, 1) > cap) return false;
+  unsigned* const seen = s.u.m.seen;
+  unsigned* const dup = s.u.m.dup;
+  unsigned* const won = s.u.m.won;
+  int rnd = 0;
+  for (; rnd < MAX_ROUNDS; ++rnd) {
+    const int bw = max(pwe, 1);
+    // last round's winners are coloured; draws; the colours drawn on each body
+    for (int w = lane; w < nbu * bw; w += 64) won[w] = 0;
+    int left = 0;
+    for (int i = lane; i < K; i += 64) {
+      const int dn = s.done[i];
+      if (dn == 2) s.done[i] = 1;
+      if (dn) continue;
+      ++left;
+      int c = s.col[i];
+      const int n = s.pc[i];
+      if (n > 0) {
+        c = lds_select(s.pal + i * pw, pwe, (int)color_draw(seed, stepk, (unsigned long long)s.key[i],
+                                                             (unsigned long long)rnd, (unsigned)n));
+        s.col[i] = (unsigned short)c;
+      }
+      if (c < 0 || c >= 32 * bw) continue;   // only after a colour-space overflow (ERR_COLORS)
+      const unsigned bit = 1u << (c & 31);
+      const int la = s.la[i], lb = s.lb[i];
+      if (la != NO_SLOT) { const int w = la * bw + (c >> 5); if (atomicOr(&seen[w], bit) & bit) atomicOr(&dup[w], bit); }
+      if (lb != NO_SLOT) { const int w = lb * bw + (c >> 5); if (atomicOr(&seen[w], bit) & bit) atomicOr(&dup[w], bit); }
+    }
+    if (!__any(left)) break;
+    __syncthreads();
+    // a draw is kept if no other uncoloured contact of either body drew it
+    int win = 0;
+    for (int i = lane; i < K; i += 64) {
+      if (s.done[i]) continue;
+      const int c = s.col[i];
+      if (c >= 32 * bw) continue;
+      const unsigned bit = 1u << (c & 31);
+      const int la = s.la[i], lb = s.lb[i];
+      const bool lose = (la != NO_SLOT && (dup[la * bw + (c >> 5)] & bit)) ||
+                        (lb != NO_SLOT && (dup[lb * bw + (c >> 5)] & bit));
+      if (lose) continue;
+      s.done[i] = 2;
+      win = 1;
+      if (la != NO_SLOT) atomicOr(&won[la * bw + (c >> 5)], bit);
+      if (lb != NO_SLOT) atomicOr(&won[lb * bw + (c >> 5)], bit);
+    }
+    __syncthreads();
+    // the colours won on its bodies leave the palette of a contact still
+    // uncoloured, whose size is recounted
+    int moving = win, empty = 0;
+    for (int i = lane; i < K; i += 64) {
+      if (s.done[i]) continue;
+      const int la = s.la[i], lb = s.lb[i];
+      unsigned* p = s.pal + i * pw;
+      int n = 0;
+      for (int w = 0; w < pwe; ++w) {
+        unsigned m = 0;
+        if (la != NO_SLOT) m |= won[la * bw + w];
+        if (lb != NO_SLOT) m |= won[lb * bw + w];
+        const unsigned v = p[w] & ~m;
+        p[w] = v;
+        n += __popc(v);
+      }
+      s.pc[i] = (unsigned short)n;
+      if (n == 0) empty = 1;
+      if (n != 1) moving = 1;
+    }
+    for (int w = lane; w < nbu * bw; w += 64) { seen[w] = 0; dup[w] = 0; }
+    // No winner and every uncoloured palette a single colour: every later
+    // round draws the same colours and keeps none; go to the leftovers.
+    if (!__any(moving)) { rnd = MAX_ROUNDS; break; }
+    __syncthreads();
+    if (__any(empty)) {
+      // empty palettes get max+1, max+2, ... in increasing contact order per
+      // island (max over the island's kept and tentative colours)
+      for (int i = lane; i < K; i += 64) {
+        if (s.done[i] || s.pc[i] != 0) continue;
+        const int il = s.isl[i], ki = s.key[i];
+        int mx = -1, rank = 0;
+        for (int j = 0; j < K; ++j) {
+          if (s.isl[j] != il) continue;
+          mx = max(mx, (int)s.col[j]);
+          if (j != i && !s.done[j] && s.pc[j] == 0 && s.key[j] < ki) ++rank;
+        }
+        const int c = mx + 1 + rank;
+        if (c >= pcap) { atomicOr(d.err, ERR_COLORS); continue; }
+        s.pal[i * pw + (c >> 5)] |= 1u << (c & 31);
+        atomicMax(&s.mx, c + 1);
+      }
+      __syncthreads();
+      for (int i = lane; i < K; i += 64)
+        if (!s.done[i] && s.pc[i] == 0) s.pc[i] = 1;
+      pwe = (s.mx + 31) / 32;
+      __syncthreads();
+      if (nbu * max(pwe, 1) > cap) return false;
+    }
+  }
+  if (rnd >= MAX_ROUNDS) {
+    // leftovers get fresh colours max+1, ... in increasing contact order per island
+    __syncthreads();
+    int fresh[SMALL_UNIT / 64];
+#pragma unroll
+    for (int q = 0; q < SMALL_UNIT / 64; ++q) {
+      const int i = lane + 64 * q;
+      fresh[q] = -1;
+      if (i >= K || s.done[i]) continue;
+      const int il = s.isl[i], ki = s.key[i];
+      int mx = -1, rank = 0;
+      for (int j = 0; j < K; ++j) {
+        if (s.isl[j] != il) continue;
+        mx = max(mx, (int)s.col[j]);
+        if (j != i && !s.done[j] && s.key[j] < ki) ++rank;
+      }
+      fresh[q] = mx + 1 + rank;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SMALL_UNIT / 64; ++q) {
+      if (fresh[q] < 0) continue;
+      if (fresh[q] >= pcap) atomicOr(d.err, ERR_COLORS);
+      s.col[lane + 64 * q] = (unsigned short)min(fresh[q], pcap - 1);
+    }
+    __syncthreads();
+  }
+  int mx = 0;
+  for (int i = lane; i < K; i += 64) {
+    mx = max(mx, s.col[i] + 1);
+    d.c_color[s.key[i] + k_off] = s.col[i];
+  }
+  atomicMax(&s.cmax, mx);
+  __syncthreads();
+  if (lane == 0) {
+    if (s.cmax > 2 * K + 1) atomicOr(d.err, ERR_COLORS);   // beyond the lane's colour offset space
+    d.u_ncol[u] = s.cmax;
+    atomicMax(&d.ss[slot].ncolor, s.cmax);
+  }
+  return true;
+}
+
+// units of up to SMALL_UNIT contacts: one wavefront each (through per-body
+// bitmaps, or through the bodies' contact lists when those do not fit)
 template <typename R> __global__ void __launch_bounds__(64) k_color_small(Dev<R> d) {
-  __shared__ ColorLds<64, SMALL_UNIT, SMALL_LIST, SMALL_PAL> s;
+  __shared__ union {
+    ColorLds<64, SMALL_UNIT, SMALL_LIST, SMALL_PAL> l;
+    BitsColorLds b;
+  } s;
   if (aborted(d.err)) return;
   const int u = blockIdx.x;
   if (u >= *d.n_units) return;
   const int K = d.u_end[u] - d.u_start[u];
   if (K <= 0 || K > SMALL_UNIT) return;
-  color_unit<R>(d, u, s);
+  if (color_small_bits<R>(d, u, s.b)) return;
+  __syncthreads();
+  color_unit<R>(d, u, s.l);
 }
 
 // larger units: a workgroup of COLOR_UNIT_BLOCK threads each

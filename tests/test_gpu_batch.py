@@ -1,8 +1,9 @@
 """GPU: launch shapes and the serving loop change no result.
 
 * Every tuning of the step (unit size, sweep lanes per scene, resident LDS
-  capacities small enough to force the velocity-only and global layouts)
-  gives bitwise the same contacts, impulses and states.
+  capacities small enough to force the velocity-only and global layouts,
+  per-body colour bitmaps too small for some or all colouring units) gives
+  bitwise the same contacts, colours, impulses and states.
 * The autotuner, which steps the batch under each candidate, leaves every
   scene where it found it: the steps after it are bitwise the untuned ones.
 * A request served by continuous batching, among other requests arriving
@@ -24,6 +25,8 @@ TUNINGS = [
     dict(waves_per_group=8, lanes_per_scene=3),
     dict(unit_size=256, lanes_per_scene=1),
     dict(resident_contacts=64, resident_bodies=64),           # groups too big: velocity-only layout
+    dict(color_body_bits=1),                                  # small units coloured through body lists
+    dict(unit_size=256, color_body_bits=24),                  # bitmaps for the units with few bodies only
 ]
 
 
@@ -63,7 +66,7 @@ def run_batch(bank, tuning, seeds, steps, single=False, tune_first=False):
             snap = []
             for sid, e in zip(ids, engines):
                 c = ctx.get_contacts(sid)
-                snap.append([c["tri"], c["impulse"]] + list(ctx.get_state(sid, len(e.bodies))))
+                snap.append([c["tri"], c["color"], c["impulse"]] + list(ctx.get_state(sid, len(e.bodies))))
             out.append(snap)
         total = sum(ctx.scene_totals(s)["contacts"] for s in ids)
         return out, total

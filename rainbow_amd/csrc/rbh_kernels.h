@@ -415,13 +415,19 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Flags of X's positions [x0, x1) (one block) w.r.t. Y.
-template <typename R>
+// Flags of X's positions [x0, x1) (one block) w.r.t. Y: a position is a
+// candidate iff its triangle box meets one of Y's triangle boxes.  The
+// block's positions stay in registers (CAND_BLK / 64 per lane) while Y's
+// triangles that meet the block box are staged in LDS, a run at a time; the
+// flags are written once at the end.
 // count_all: count every overlapping triangle pair; exact: the reference's
-// candidate set, without the reach bound (set while overlaps are counted)
-__device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, int x1, bool count_all, bool exact,
+// candidate set, without the reach bound (set while overlaps are counted).
+// Returns the number of this lane's positions flagged.
+template <typename R>
+__device__ int cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, int x1, bool count_all, bool exact,
                            V4<R>* lo_s, V4<R>* hi_s, V4<R>* clo_s, V4<R>* chi_s, int2* crange_s,
                            unsigned long long& overlaps) {
+  constexpr int XQ = CAND_BLK / 64;
   const int cap = CandLds<R>::CAP;
   const int lane = threadIdx.x & 63;
   const ShapeInfo& sX = d.shapes[d.b_shape[X]];
@@ -443,103 +449,128 @@ __device__ void cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0,
     const R ez = fmax(fmax(lo.z - oY.z, oY.z - hi.z), R(0));
     return !exact && ex * ex + ey * ey + ez * ez > lim * lim;
   };
-  if (!box_overlap(xlo, xhi, rYlo, rYhi) || beyond(xlo, xhi)) {
-    for (int p = x0 + lane; p < x1; p += 64) d.slot_flag[slot_base + d.sT[sX.t_off + p].w] = 0;
-    return;
-  }
-  int ybase = 0;
-  bool first = true;
-  while (true) {
-    // stage the next run of Y triangles that overlap the block box, two
-    // clusters (64 positions) at a time, with the boxes of their clusters
-    int nfill = 0, ncl = 0;
-    while (ybase < sY.nt) {
-      if (ybase % CAND_BLK == 0) {
-        // the next Y block whose box meets the block box, 64 block boxes per load
-        int yb = ybase / CAND_BLK;
-        const int nbY = n_blocks(sY.nt);
-        while (yb < nbY) {
-          const int j = yb + lane;
-          const unsigned long long m = __ballot(j < nbY && box_overlap(d.clo[kY + j], d.chi[kY + j], xlo, xhi));
-          if (m) { yb += __ffsll((long long)m) - 1; break; }
-          yb += 64;
-        }
-        if (yb >= nbY) { ybase = sY.nt; break; }
-        ybase = yb * CAND_BLK;
-      }
-      // this lane's cluster box and triangle box, loaded together
-      const int p = ybase + lane;
-      const bool live = p < sY.nt;
-      const int cl = min(p, sY.nt - 1) / CLUSTER;
-      const V4<R> cbl = d.clo[tY + cl], cbh = d.chi[tY + cl];
-      V4<R> lo = cbl, hi = cbh;
-      if (live) { lo = d.tlo[tY + p]; hi = d.thi[tY + p]; }
-      const bool oc = live && box_overlap(cbl, cbh, xlo, xhi);
-      const bool in = oc && box_overlap(lo, hi, xlo, xhi);
-      const unsigned long long mc = __ballot(oc);
-      if (!mc) { ybase += 64; continue; }
-      const V4<R> b0lo = shfl_xyz(cbl, 0), b0hi = shfl_xyz(cbh, 0);
-      const V4<R> b1lo = shfl_xyz(cbl, 32), b1hi = shfl_xyz(cbh, 32);
-      const unsigned long long m = __ballot(in);
-      const int n0 = __popcll(m & 0xffffffffull), n1 = __popcll(m >> 32);
-      if (nfill + n0 + n1 > cap || ncl + 2 > CAND_CL) break;   // uniform: stage full, test what is staged
-      if (in) { const int pos = nfill + lanes_below(m); lo_s[pos] = lo; hi_s[pos] = hi; }
-      if (lane == 0) {
-        int k = ncl;
-        if (n0) { clo_s[k] = b0lo; chi_s[k] = b0hi; crange_s[k] = make_int2(nfill, n0); ++k; }
-        if (n1) { clo_s[k] = b1lo; chi_s[k] = b1hi; crange_s[k] = make_int2(nfill + n0, n1); }
-      }
-      ncl += (n0 ? 1 : 0) + (n1 ? 1 : 0);
-      nfill += n0 + n1;
-      ybase += 64;
-    }
-    wave_sync_lds();
-    for (int xbase = x0; xbase < x1; xbase += 64) {
-      const int p = xbase + lane;
-      if (p >= x1) continue;
-      // cluster box, triangle box and flag of this lane's position, loaded together
-      const int cl = p / CLUSTER;
-      const int t = d.sT[sX.t_off + p].w;
-      const V4<R> cbl = d.clo[tX + cl], cbh = d.chi[tX + cl];
-      const V4<R> lo = d.tlo[tX + p], hi = d.thi[tX + p];
-      unsigned char flag = first ? 0 : d.slot_flag[slot_base + t];
-      if (!box_overlap(cbl, cbh, rYlo, rYhi) || beyond(cbl, cbh)) {
-        if (first) d.slot_flag[slot_base + t] = 0;
-        continue;
-      }
-      if (flag && !count_all) continue;
-      if (!box_overlap(lo, hi, rYlo, rYhi) || beyond(lo, hi)) { if (first) d.slot_flag[slot_base + t] = 0; continue; }
-      unsigned cnt = 0;
-      for (int k = 0; k < ncl && !(flag && !count_all); ++k) {
-        if (!box_overlap(lo, hi, clo_s[k], chi_s[k])) continue;
-        const int2 rg = crange_s[k];
-        const int end = rg.x + rg.y;
-        // four staged boxes per pass, loaded together (indexes clamped into
-        // the cluster's run; a clamped repeat is not tested)
-        for (int e = rg.x; e < end; e += 4) {
-          V4<R> sl[4], sh[4];
+  const bool block_in = box_overlap(xlo, xhi, rYlo, rYhi) && !beyond(xlo, xhi);
+  // this lane's positions x0 + lane + 64 q: triangle, box, whether it can
+  // meet Y at all (its cluster box and its own box meet Y's body box)
+  V3<R> xl[XQ], xh[XQ];             // (box corners: x, y, z only)
+  int xt[XQ];
+  unsigned live = 0, flg = 0;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int ej = min(e + j, end - 1);
-            sl[j] = lo_s[ej]; sh[j] = hi_s[ej];
+  for (int q = 0; q < XQ; ++q) {
+    const int p = x0 + lane + 64 * q;
+    xt[q] = -1;
+    xl[q] = xyz(xlo); xh[q] = xyz(xhi);
+    if (p >= x1) continue;
+    xt[q] = d.sT[sX.t_off + p].w;
+    if (!block_in) continue;
+    const int cl = p / CLUSTER;
+    const V4<R> cbl = d.clo[tX + cl], cbh = d.chi[tX + cl];
+    const V4<R> lo = d.tlo[tX + p], hi = d.thi[tX + p];
+    if (!box_overlap(cbl, cbh, rYlo, rYhi) || beyond(cbl, cbh)) continue;
+    if (!box_overlap(lo, hi, rYlo, rYhi) || beyond(lo, hi)) continue;
+    xl[q] = xyz(lo); xh[q] = xyz(hi);
+    live |= 1u << q;
+  }
+  unsigned long long cnt = 0;
+  if (__ballot(live != 0)) {
+    const int nbY = n_blocks(sY.nt);
+    int ywin = 0;                          // window of 64 Y blocks ...
+    unsigned long long ymask = 0;          // ... whose boxes meet the block box, not yet staged
+    {
+      const int j = lane;
+      ymask = __ballot(j < nbY && box_overlap(d.clo[kY + j], d.chi[kY + j], xlo, xhi));
+    }
+    int ybase = 0;
+    while (true) {
+      // stage the next run of Y triangles that meet the block box, two
+      // clusters (64 positions) at a time, with the boxes of their clusters
+      int nfill = 0, ncl = 0;
+      while (ybase < sY.nt) {
+        if (ybase % CAND_BLK == 0) {
+          // the next Y block whose box meets the block box
+          int yb = ybase / CAND_BLK;
+          while (true) {
+            if (yb >= ywin + 64) {
+              ywin += 64;
+              if (ywin >= nbY) { yb = nbY; break; }
+              const int j = ywin + lane;
+              ymask = __ballot(j < nbY && box_overlap(d.clo[kY + j], d.chi[kY + j], xlo, xhi));
+            }
+            const unsigned long long m = yb - ywin >= 64 ? 0ull : ymask & (~0ull << (yb - ywin));
+            if (m) { yb = ywin + __ffsll((long long)m) - 1; break; }
+            yb = ywin + 64;
           }
-          unsigned hit = 0;
+          if (yb >= nbY) { ybase = sY.nt; break; }
+          ybase = yb * CAND_BLK;
+        }
+        // this lane's cluster box and triangle box, loaded together
+        const int p = ybase + lane;
+        const bool livey = p < sY.nt;
+        const int cl = min(p, sY.nt - 1) / CLUSTER;
+        const V4<R> cbl = d.clo[tY + cl], cbh = d.chi[tY + cl];
+        V4<R> lo = cbl, hi = cbh;
+        if (livey) { lo = d.tlo[tY + p]; hi = d.thi[tY + p]; }
+        const bool oc = livey && box_overlap(cbl, cbh, xlo, xhi);
+        const bool in = oc && box_overlap(lo, hi, xlo, xhi);
+        const unsigned long long mc = __ballot(oc);
+        if (!mc) { ybase += 64; continue; }
+        const V4<R> b0lo = shfl_xyz(cbl, 0), b0hi = shfl_xyz(cbh, 0);
+        const V4<R> b1lo = shfl_xyz(cbl, 32), b1hi = shfl_xyz(cbh, 32);
+        const unsigned long long m = __ballot(in);
+        const int n0 = __popcll(m & 0xffffffffull), n1 = __popcll(m >> 32);
+        if (nfill + n0 + n1 > cap || ncl + 2 > CAND_CL) break;   // uniform: stage full, test what is staged
+        if (in) { const int pos = nfill + lanes_below(m); lo_s[pos] = lo; hi_s[pos] = hi; }
+        if (lane == 0) {
+          int k = ncl;
+          if (n0) { clo_s[k] = b0lo; chi_s[k] = b0hi; crange_s[k] = make_int2(nfill, n0); ++k; }
+          if (n1) { clo_s[k] = b1lo; chi_s[k] = b1hi; crange_s[k] = make_int2(nfill + n0, n1); }
+        }
+        ncl += (n0 ? 1 : 0) + (n1 ? 1 : 0);
+        nfill += n0 + n1;
+        ybase += 64;
+      }
+      wave_sync_lds();
 #pragma unroll
-          for (int j = 0; j < 4; ++j) hit |= (box_overlap(lo, hi, sl[j], sh[j]) & (e + j < end)) ? 1u << j : 0u;
-          if (hit) {
-            flag = 1;
-            if (!count_all) break;
-            cnt += __popc(hit);
+      for (int q = 0; q < XQ; ++q) {
+        if (!((live >> q) & 1u) || (((flg >> q) & 1u) && !count_all)) continue;
+        const V4<R> lo{xl[q].x, xl[q].y, xl[q].z, R(0)}, hi{xh[q].x, xh[q].y, xh[q].z, R(0)};
+        bool hitq = false;
+        for (int k = 0; k < ncl && !(hitq && !count_all); ++k) {
+          if (!box_overlap(lo, hi, clo_s[k], chi_s[k])) continue;
+          const int2 rg = crange_s[k];
+          const int end = rg.x + rg.y;
+          // four staged boxes per pass, loaded together (indexes clamped into
+          // the cluster's run; a clamped repeat is not tested)
+          for (int e = rg.x; e < end; e += 4) {
+            V4<R> sl[4], sh[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int ej = min(e + j, end - 1);
+              sl[j] = lo_s[ej]; sh[j] = hi_s[ej];
+            }
+            unsigned hit = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) hit |= (box_overlap(lo, hi, sl[j], sh[j]) & (e + j < end)) ? 1u << j : 0u;
+            if (hit) {
+              hitq = true;
+              if (!count_all) break;
+              cnt += __popc(hit);
+            }
           }
         }
+        if (hitq) flg |= 1u << q;
       }
-      overlaps += cnt;
-      d.slot_flag[slot_base + t] = flag;
+      wave_sync_lds();
+      if (ybase >= sY.nt) break;
+      // every position that can meet Y is already a candidate: the rest of Y changes no flag
+      if (!count_all && !__ballot((live & ~flg) != 0)) break;
     }
-    wave_sync_lds();
-    first = false;
-    if (ybase >= sY.nt) break;
   }
+  overlaps += cnt;
+#pragma unroll
+  for (int q = 0; q < XQ; ++q)
+    if (xt[q] >= 0) d.slot_flag[slot_base + xt[q]] = (unsigned char)((flg >> q) & 1u);
+  return __popc(flg);
 }
 
 // the pair p with item_off[p] <= item < item_off[p + 1] (item_off strictly
@@ -575,16 +606,17 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
   const int nitems = d.item_off[np];
   const bool count_all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   const int wave = threadIdx.x >> 6;
-  // Workgroups go to the 8 XCDs in turn.  When one pass of the grid covers
-  // the items, workgroup w takes logical block (w % 8) * per + w / 8, so each
-  // XCD walks a contiguous run of pairs: the pairs of one body follow each
-  // other and then share its triangle boxes in that XCD's L2.
-  const int per = (nitems + 8 * CAND_WAVES - 1) / (8 * CAND_WAVES);
-  const bool remap = (int)gridDim.x >= 8 * per;
-  if (remap && (int)blockIdx.x >= 8 * per) return;
-  const int first = remap ? ((int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8)) * CAND_WAVES + wave
-                          : (int)blockIdx.x * CAND_WAVES + wave;
-  for (int item = first; item < nitems; item += remap ? nitems : gridDim.x * CAND_WAVES) {
+  // Workgroups go to the 8 XCDs in turn (w on XCD w % 8).  Each XCD takes a
+  // contiguous run of the items, in proportion to its workgroups, and its
+  // wavefronts walk that run together: the blocks of one pair (same two
+  // bodies) are then in flight at once on one XCD and share the bodies'
+  // boxes in its L2, as do the consecutive pairs of one body.
+  const int G = (int)gridDim.x, xcd = (int)blockIdx.x & 7, wx = (int)blockIdx.x >> 3;
+  int before = 0;
+  for (int x = 0; x < xcd; ++x) before += (G - x + 7) >> 3;
+  const int gx = (G - xcd + 7) >> 3;
+  const int lo = (int)((long long)nitems * before / G), hi = (int)((long long)nitems * (before + gx) / G);
+  for (int item = lo + wx * CAND_WAVES + wave; item < hi; item += gx * CAND_WAVES) {
     const int p = item_pair(d.item_off, np, item);
     const int2 ab = d.pairs[p];
     const int ntA = d.shapes[d.b_shape[ab.x]].nt, ntB = d.shapes[d.b_shape[ab.y]].nt;
@@ -595,13 +627,9 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
     const int x0 = blk * CAND_BLK, x1 = min(x0 + CAND_BLK, side0 ? ntA : ntB);
     unsigned long long ov = 0;
     const int slot_base = d.slot_off[p] + (side0 ? 0 : ntA);
-    cand_block(d, X, Y, slot_base, x0, x1, count_all && side0, count_all, lo_s[wave], hi_s[wave],
-               clo_s[wave], chi_s[wave], crange_s[wave], ov);
-    // candidates of this block (each lane reads back the flags it wrote last)
-    const int tXo = d.shapes[d.b_shape[X]].t_off;
-    int nset = 0;
-    for (int pp = x0 + (threadIdx.x & 63); pp < x1; pp += 64) nset += d.slot_flag[slot_base + d.sT[tXo + pp].w] ? 1 : 0;
-    nset = wave_sum(nset);
+    int nset = cand_block(d, X, Y, slot_base, x0, x1, count_all && side0, count_all, lo_s[wave], hi_s[wave],
+                          clo_s[wave], chi_s[wave], crange_s[wave], ov);
+    nset = wave_sum(nset);   // candidates of this block
     if ((threadIdx.x & 63) == 0 && nset) atomicAdd(&d.side_cnt[2 * p + (side0 ? 0 : 1)], nset);
     if (count_all && side0) {
       const unsigned long long tot = wave_sum_u64(ov);
@@ -658,7 +686,13 @@ template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
   int ncand = 0;
   int cur = -1;
   const long long nwaves = (long long)gridDim.x * (blockDim.x >> 6);
-  const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // workgroup w runs on XCD w % 8: number the workgroups XCD by XCD, so that
+  // each XCD takes a contiguous run of the candidates (those of one pair read
+  // the same SDF cells, then from that XCD's L2)
+  const int G = (int)gridDim.x, xcd = (int)blockIdx.x & 7;
+  int lwg = (int)blockIdx.x >> 3;
+  for (int x = 0; x < xcd; ++x) lwg += (G - x + 7) >> 3;
+  const long long wave = (long long)lwg * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const long long chunk = ((nc + nwaves - 1) / nwaves + 63) / 64 * 64;
   const int c_lo = (int)min<long long>(wave * chunk, nc), c_hi = (int)min<long long>(wave * chunk + chunk, nc);
   for (int c = c_lo + (int)(threadIdx.x & 63); c < c_hi; c += 64) {

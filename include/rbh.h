@@ -122,7 +122,7 @@ typedef struct rbh_tuning {
   int color_lds_contacts; /* most contacts of a colouring unit whose colouring state is kept in LDS
                              (default and upper bound 4096; larger units keep it in global memory) */
   int color_body_bits;    /* most words of each per-body colour bitmap array of a unit of at most 256
-                             contacts (default and upper bound 768; a unit needing more finds its
+                             contacts (default and upper bound 512; a unit needing more finds its
                              contacts' neighbours through its bodies' contact lists) */
 } rbh_tuning;
 

@@ -210,36 +210,36 @@ template <typename R> __global__ void k_csr_fill(Dev<R> d) {
 // the deterministic form of oracle/coloring.py).  Node = contact; two
 // contacts are adjacent if they share a free body.  Units of up to
 // SMALL_UNIT contacts are coloured by one wavefront with the palettes in LDS
-// (ceil((2K + 2) / 32) words per contact: colours of an island stay below
-// twice its size) and, per free body of the unit, bitmaps of the colours its
-// contacts drew and won, so that a round costs O(palette words) per contact;
-// larger units (and small ones whose bitmaps would not fit) by a workgroup of
-// eight wavefronts, palettes in global memory (PALW words per contact), the
-// neighbours found through the bodies' contact lists, the rest of the state
-// in LDS up to UNIT_CAP contacts (gfx950: 160 KB of LDS per workgroup), in
-// global memory beyond.
+// (SMALL_PW words per contact) and, per free body of the unit, bitmaps of the
+// colours its contacts drew and won, so that a round costs O(palette words)
+// per contact; larger units (and small ones whose palettes or bitmaps would
+// not fit) by a workgroup of eight wavefronts, palettes in global memory
+// (PALW words per contact; colours of an island stay below twice its size),
+// the neighbours found through the bodies' contact lists, the rest of the
+// state in LDS up to UNIT_CAP contacts (gfx950: 160 KB of LDS per
+// workgroup), in global memory beyond.
 constexpr int COLOR_BLOCK = 256;
 constexpr int COLOR_UNIT_BLOCK = 512;   // threads colouring one large unit (8 wavefronts)
 constexpr int UNIT_CAP = 4096;        // contacts of a large unit staged in LDS
 constexpr int LIST_CAP = 8192;        // body-list entries staged in LDS
 constexpr int SMALL_UNIT = 256;
 constexpr int SMALL_BODY = 32;        // bodies with at most this many contacts are checked entry by entry
-constexpr int SMALL_LIST = 1024;
-constexpr int SMALL_PAL = SMALL_UNIT * ((2 * SMALL_UNIT + 2 + 31) / 32);
-constexpr int BODY_BITS = 768;        // words of each per-body colour bitmap array of a small unit
+constexpr int BODY_BITS = 512;        // words of each per-body colour bitmap array of a small unit
+constexpr int SMALL_PW = 4;           // palette words per contact of a small unit (colours below 128)
 constexpr int PAIR_HASH = 2 * SMALL_UNIT;   // body-pair table of a small unit's set-up (power of two)
 constexpr unsigned short NO_SLOT = 0xffff;
+constexpr int U_LISTS = -1;           // u_ncol of a small unit left to the contact-list path
 
-// State of a small unit coloured through per-body colour bitmaps.  Body b of
-// the unit (its slot f_loc) owns words [b * bw, (b + 1) * bw) of each bitmap
-// array, bw = palette words in use.
+// State of a small unit coloured through per-body colour bitmaps (about 14 KB,
+// so that many units share a CU).  Body b of the unit (its slot f_loc) owns
+// words [b * bw, (b + 1) * bw) of each bitmap array, bw = palette words in use.
 struct BitsColorLds {
   int key[SMALL_UNIT];                // contact index within its scene (draws, ranks)
   int isl[SMALL_UNIT];                // island label
   unsigned short la[SMALL_UNIT], lb[SMALL_UNIT];   // unit slots of the free bodies (NO_SLOT: fixed)
   unsigned short col[SMALL_UNIT], pc[SMALL_UNIT];
   unsigned char done[SMALL_UNIT];     // 0 uncoloured, 1 coloured, 2 won this round
-  unsigned pal[SMALL_PAL];
+  unsigned pal[SMALL_UNIT * SMALL_PW];
   union {
     struct { unsigned seen[BODY_BITS], dup[BODY_BITS], won[BODY_BITS]; } m;   // rounds
     struct { int cnt[2 * SMALL_UNIT]; unsigned hk[PAIR_HASH]; int hc[PAIR_HASH]; } s;   // set-up
@@ -736,9 +736,11 @@ __device__ __forceinline__ unsigned pair_hash(unsigned key) { return (key * 2654
 //  * the colours won on either free body leave the palette of a contact still
 //    uncoloured (its palette is read and written by its own lane only).
 // The same draws, winners, palettes, fresh colours and leftovers as
-// color_rounds, so the same colours.  Returns false, having written nothing
-// outside LDS, when the unit is not for this path (serial scheme, colouring
-// state forced to global memory) or its bitmaps outgrow color_bits words;
+// color_rounds, so the same colours.  Palettes hold colours below
+// 32 SMALL_PW (leftovers may take larger ones: they need no palette).
+// Returns false, having written nothing outside LDS, when the unit is not for
+// this path (colouring state forced to global memory) or outgrows it (a
+// palette colour of 32 SMALL_PW or more, bitmaps beyond color_bits words);
 // color_unit then colours it from the start.
 template <typename R>
 __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
@@ -747,11 +749,22 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
   const SceneParams& sp = d.sp[slot];
   const SceneState& st = d.ss[slot];
   const int nbu = d.u_nbody[u];
-  if (sp.scheme == SCHEME_GS || K > d.color_cap || nbu > 2 * SMALL_UNIT) return false;
   const int lane = threadIdx.x;
   const int k_off = st.k_off;
+  if (sp.scheme == SCHEME_GS) {   // the reference's serial order: contact i is colour i
+    for (int i = lane; i < K; i += 64) {
+      const int k = d.uperm[p0 + i];
+      d.c_color[k] = k - k_off;
+    }
+    if (lane == 0) { d.u_ncol[u] = 0; atomicMax(&d.ss[slot].ncolor, st.k_cnt); }
+    return true;
+  }
+  if (K > d.color_cap || nbu > 2 * SMALL_UNIT) return false;
   const unsigned long long seed = sp.seed, stepk = (unsigned long long)st.step_index;
-  const int pw = (2 * K + 2 + 31) / 32, pcap = 32 * pw;
+  // pcap: the colour space of the unit's lanes (colours of an island stay
+  // below twice its size); palettes here hold colours below 32 SMALL_PW
+  constexpr int pw = SMALL_PW;
+  const int pcap = 32 * ((2 * K + 2 + 31) / 32);
   const int cap = d.color_bits;
   unsigned* const raw = s.u.m.seen;   // the whole union as words
   for (int w = lane; w < 3 * BODY_BITS; w += 64) raw[w] = 0;
@@ -791,14 +804,13 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
       for (int t = 0; t < PAIR_HASH; ++t, h = (h + 1) & (PAIR_HASH - 1))
         if (s.u.s.hk[h] == key) { deg -= s.u.s.hc[h] - 1; break; }
     }
-    if (deg > 0) atomicMax(&s.mx, min(deg, pcap));
-    if (deg > pcap) atomicOr(d.err, ERR_COLORS);
+    if (deg > 0) atomicMax(&s.mx, deg);
     unsigned* p = s.pal + i * pw;
     for (int w = 0; w < pw; ++w) {
       const int lo = w * 32;
       p[w] = deg >= lo + 32 ? 0xffffffffu : deg > lo ? (1u << (deg - lo)) - 1u : 0u;
     }
-    s.pc[i] = (unsigned short)min(deg, pcap);
+    s.pc[i] = (unsigned short)deg;
     s.col[i] = 0;
     s.done[i] = 0;
   }
@@ -806,7 +818,7 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
   for (int w = lane; w < 3 * BODY_BITS; w += 64) raw[w] = 0;
   int pwe = (s.mx + 31) / 32;   // palette words in use
   __syncthreads();
-  if (nbu * max(pwe, 1) > cap) return false;
+  if (pwe > pw || nbu * max(pwe, 1) > cap) return false;
   unsigned* const seen = s.u.m.seen;
   unsigned* const dup = s.u.m.dup;
   unsigned* const won = s.u.m.won;
@@ -892,6 +904,7 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
         }
         const int c = mx + 1 + rank;
         if (c >= pcap) { atomicOr(d.err, ERR_COLORS); continue; }
+        if (c >= 32 * pw) { atomicMax(&s.mx, c + 1); continue; }   // beyond these palettes: start over below
         s.pal[i * pw + (c >> 5)] |= 1u << (c & 31);
         atomicMax(&s.mx, c + 1);
       }
@@ -900,7 +913,7 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
         if (!s.done[i] && s.pc[i] == 0) s.pc[i] = 1;
       pwe = (s.mx + 31) / 32;
       __syncthreads();
-      if (nbu * max(pwe, 1) > cap) return false;
+      if (pwe > pw || nbu * max(pwe, 1) > cap) return false;
     }
   }
   if (rnd >= MAX_ROUNDS) {
@@ -945,31 +958,27 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
   return true;
 }
 
-// units of up to SMALL_UNIT contacts: one wavefront each (through per-body
-// bitmaps, or through the bodies' contact lists when those do not fit)
+// units of up to SMALL_UNIT contacts: one wavefront each, through per-body
+// bitmaps; a unit outgrowing them is marked U_LISTS for k_color_units
 template <typename R> __global__ void __launch_bounds__(64) k_color_small(Dev<R> d) {
-  __shared__ union {
-    ColorLds<64, SMALL_UNIT, SMALL_LIST, SMALL_PAL> l;
-    BitsColorLds b;
-  } s;
+  __shared__ BitsColorLds s;
   if (aborted(d.err)) return;
   const int u = blockIdx.x;
   if (u >= *d.n_units) return;
   const int K = d.u_end[u] - d.u_start[u];
   if (K <= 0 || K > SMALL_UNIT) return;
-  if (color_small_bits<R>(d, u, s.b)) return;
-  __syncthreads();
-  color_unit<R>(d, u, s.l);
+  if (!color_small_bits<R>(d, u, s) && threadIdx.x == 0) d.u_ncol[u] = U_LISTS;
 }
 
-// larger units: a workgroup of COLOR_UNIT_BLOCK threads each
+// larger units and the small units marked U_LISTS: a workgroup of
+// COLOR_UNIT_BLOCK threads each, neighbours through the bodies' contact lists
 template <typename R> __global__ void __launch_bounds__(COLOR_UNIT_BLOCK) k_color_units(Dev<R> d) {
   __shared__ ColorLds<COLOR_UNIT_BLOCK, UNIT_CAP, LIST_CAP, 0> s;
   if (aborted(d.err)) return;
   const int u = blockIdx.x;
   if (u >= *d.n_units) return;
   const int K = d.u_end[u] - d.u_start[u];
-  if (K <= SMALL_UNIT) return;
+  if (K <= 0 || (K <= SMALL_UNIT && d.u_ncol[u] != U_LISTS)) return;
   color_unit<R>(d, u, s);
 }
 

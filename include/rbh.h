@@ -124,6 +124,9 @@ typedef struct rbh_tuning {
   int color_body_bits;    /* most words of each per-body colour bitmap array of a unit of at most 256
                              contacts (default and upper bound 512; a unit needing more finds its
                              contacts' neighbours through its bodies' contact lists) */
+  int solve_groups_per_cu;/* batched solve: 1 = full LDS layout (one workgroup per CU), 2 = half layout
+                             (FP32 192 resident contacts; two per CU); 0 = 2 when the batch has more
+                             scenes than the device has CUs, else 1 */
 } rbh_tuning;
 
 typedef struct rbh_device_info {

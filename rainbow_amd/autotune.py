@@ -26,14 +26,15 @@ import time
 from . import native
 
 DEFAULT = dict(unit_size=0, lanes_per_scene=0, resident_contacts=0, resident_bodies=0,
-               lanes_per_group=0, waves_per_group=0, color_lds_contacts=0, color_body_bits=0)
+               lanes_per_group=0, waves_per_group=0, color_lds_contacts=0, color_body_bits=0,
+               solve_groups_per_cu=0)
 
 
 def batch_candidates():
     """Shapes for a batch of several scenes (one workgroup per scene; default first)."""
     out = []
-    for waves, lanes, unit in itertools.product((4, 8), (0, 1, 2), (64, 128)):
-        out.append(dict(waves_per_group=waves, lanes_per_scene=lanes, unit_size=unit))
+    for groups, waves, lanes, unit in itertools.product((0, 1), (4, 8), (0, 1, 2), (64, 128)):
+        out.append(dict(waves_per_group=waves, lanes_per_scene=lanes, unit_size=unit, solve_groups_per_cu=groups))
     return out
 
 

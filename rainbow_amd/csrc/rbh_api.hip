@@ -172,7 +172,7 @@ struct rbh_ctx {
   rbh_tuning tune{};        // rbh_set_tuning (0 fields: defaults)
   // resident solve groups: LDS capacities (contacts, bodies, colour offsets) and the launch's dynamic LDS
   int res_cap_c = 0, res_cap_b = 0, res_cap_o = 0;
-  size_t solve_lds = 0;
+  size_t solve_lds = 0, solve_lds_half = 0;   // full and half layouts (one or two workgroups per CU)
   int max_iter_batch = 1, coop_max = 0, last_units = 0;
   bool multi_ok = false;
   bool any_post = false;    // a scene of the batch has post-stabilisation on (set per run_steps call)
@@ -460,6 +460,25 @@ int check_params(rbh_ctx* c, const rbh_params& p) {
 // ------------------------------------------------------------ batch workspace
 // wavefronts per workgroup of the batched solve (4 or 8; rbh_tuning.waves_per_group)
 int batch_waves(const rbh_ctx* c) { return c->tune.waves_per_group == 8 ? 8 : 4; }
+// the batched solve's LDS layout: half (two workgroups per CU) when asked, or
+// by default when the batch has more scenes than the device has CUs
+bool batch_half(const rbh_ctx* c, int ns) {
+  if (c->tune.solve_groups_per_cu == 1) return false;
+  if (c->tune.solve_groups_per_cu == 2) return true;
+  return ns > c->prop.multiProcessorCount;
+}
+// one workgroup per batch scene (the post-stabilisation solve too)
+template <typename R> void launch_batch_solve(rbh_ctx* c, const Dev<R>& d, int ns, hipStream_t s) {
+  const bool half = batch_half(c, ns);
+  const size_t lds = half ? c->solve_lds_half : c->solve_lds;
+  if (batch_waves(c) == 8) {
+    if (half) hipLaunchKernelGGL((k_solve<R, 512, 1>), dim3(ns), dim3(512), lds, s, d);
+    else hipLaunchKernelGGL((k_solve<R, 512, 0>), dim3(ns), dim3(512), lds, s, d);
+  } else {
+    if (half) hipLaunchKernelGGL((k_solve<R, SOLVE_BLOCK, 1>), dim3(ns), dim3(SOLVE_BLOCK), lds, s, d);
+    else hipLaunchKernelGGL((k_solve<R, SOLVE_BLOCK, 0>), dim3(ns), dim3(SOLVE_BLOCK), lds, s, d);
+  }
+}
 
 // multi: a single scene solved by a cooperative grid (its lanes are its units)
 template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool multi) {
@@ -761,10 +780,7 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   } else {
     mark(ST_COLOR);
     hinge_pass<R>(c, d, 0);
-    if (batch_waves(c) == 8)
-      hipLaunchKernelGGL((k_solve<R, 512>), dim3(ns), dim3(512), c->solve_lds, s, d);
-    else
-      hipLaunchKernelGGL((k_solve<R, SOLVE_BLOCK>), dim3(ns), dim3(SOLVE_BLOCK), c->solve_lds, s, d);
+    launch_batch_solve<R>(c, d, ns, s);
   }
   hinge_pass<R>(c, d, 1);
   mark(ST_SOLVE);
@@ -783,10 +799,8 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
       hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<void*>(k_solve_multi<R>), dim3(multi_grid),
                                                 dim3(SOLVE_BLOCK), args, c->solve_lds, s);
       if (e != hipSuccess) return e;
-    } else if (batch_waves(c) == 8) {
-      hipLaunchKernelGGL((k_solve<R, 512>), dim3(ns), dim3(512), c->solve_lds, s, dp);
     } else {
-      hipLaunchKernelGGL((k_solve<R, SOLVE_BLOCK>), dim3(ns), dim3(SOLVE_BLOCK), c->solve_lds, s, dp);
+      launch_batch_solve<R>(c, dp, ns, s);
     }
     hipLaunchKernelGGL(k_post_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
   }
@@ -818,15 +832,21 @@ template <typename R> int setup_solve_lds(rbh_ctx* c) {
   c->res_cap_c = c->tune.resident_contacts > 0 ? std::min(c->tune.resident_contacts, ResCap<R>::C) : ResCap<R>::C;
   c->res_cap_b = c->tune.resident_bodies > 0 ? std::min(c->tune.resident_bodies, ResCap<R>::B) : ResCap<R>::B;
   c->res_cap_o = ResCap<R>::O;
-  size_t bytes = std::max<size_t>(SOLVE_LDS_BYTES, ResCap<R>::BYTES);
-  const void* ks[3] = {reinterpret_cast<const void*>(k_solve<R, SOLVE_BLOCK>),
-                       reinterpret_cast<const void*>(k_solve<R, 512>),
-                       reinterpret_cast<const void*>(k_solve_multi<R>)};
-  for (const void* k : ks)
-    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+  const size_t bytes = std::max<size_t>(SOLVE_LDS_BYTES, ResCap<R>::BYTES);
+  const size_t half = std::max<size_t>(SOLVE_LDS_BYTES, ResCap<R, 1>::BYTES);
+  const void* ks[5] = {reinterpret_cast<const void*>(k_solve<R, SOLVE_BLOCK, 0>),
+                       reinterpret_cast<const void*>(k_solve<R, 512, 0>),
+                       reinterpret_cast<const void*>(k_solve_multi<R>),
+                       reinterpret_cast<const void*>(k_solve<R, SOLVE_BLOCK, 1>),
+                       reinterpret_cast<const void*>(k_solve<R, 512, 1>)};
+  for (int i = 0; i < 5; ++i) {
+    const size_t b = i < 3 ? bytes : half;
+    if (hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) != hipSuccess) {
       (void)hipGetLastError();
-      return fail(c, RBH_E_HIP, "solve kernels cannot use " + std::to_string(bytes) + " bytes of LDS");
+      return fail(c, RBH_E_HIP, "solve kernels cannot use " + std::to_string(b) + " bytes of LDS");
     }
+  }
+  c->solve_lds_half = half;
   const int hbytes = (int)hinge_lds_bytes<R>(HINGE_BODY_MAX, HINGE_MAX);
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_hinge_solve<R>), hipFuncAttributeMaxDynamicSharedMemorySize,
                           hbytes) != hipSuccess) {
@@ -1585,6 +1605,7 @@ int rbh_set_tuning(rbh_ctx* c, const rbh_tuning* t) {
   if (!c || !t) return RBH_E_ARG;
   if (t->unit_size < 0 || t->unit_size > 4096 || t->lanes_per_scene < 0 || t->lanes_per_scene > 64 ||
       t->resident_contacts < 0 || t->resident_bodies < 0 || t->lanes_per_group < 0 || t->lanes_per_group > 64 ||
+      t->solve_groups_per_cu < 0 || t->solve_groups_per_cu > 2 ||
       (t->waves_per_group != 0 && t->waves_per_group != 4 && t->waves_per_group != 8) ||
       (t->resident_contacts % 64) || (t->resident_bodies % 64) || t->color_lds_contacts < 0 ||
       t->color_body_bits < 0)

@@ -300,10 +300,12 @@ __device__ __forceinline__ void prox_ellipsoid(R zs_, R zt_, R zr_, R mus_, R mu
 // out of line: anisotropic friction is the rare case, and inlining its
 // bisection into every sweep loop only lengthens the isotropic path's code
 template <typename R>
-__device__ __attribute__((noinline)) void prox_ellipsoid_call(R zs, R zt, R zr, R ms, R mt, R mr, R xn, double tol,
-                                                             double expansion, int max_it, R* out) {
+__device__ __attribute__((noinline)) V3<R> prox_ellipsoid_call(R zs, R zt, R zr, R ms, R mt, R mr, R xn, double tol,
+                                                              double expansion, int max_it) {
   EllipsoidParams ep{tol, expansion, max_it};
-  prox_ellipsoid<R>(zs, zt, zr, ms, mt, mr, xn, ep, out[0], out[1], out[2]);
+  V3<R> o;
+  prox_ellipsoid<R>(zs, zt, zr, ms, mt, mr, xn, ep, o.x, o.y, o.z);
+  return o;
 }
 
 // ---------------------------------------------------------------- misc

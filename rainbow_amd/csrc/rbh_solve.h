@@ -1231,9 +1231,8 @@ __device__ __forceinline__ V4<R> contact_prox(const ProxParams& pp, const Contac
   R xr = use_old_xn ? xo.x : xn;
   R os, ot, orr;
   if (pp.friction == 1) {
-    R o[3];
-    prox_ellipsoid_call<R>(z1, z2, z3, mu.x, mu.y, mu.z, xr, pp.ell_tol, pp.ell_expansion, pp.ell_max_iter, o);
-    os = o[0]; ot = o[1]; orr = o[2];
+    const V3<R> o = prox_ellipsoid_call<R>(z1, z2, z3, mu.x, mu.y, mu.z, xr, pp.ell_tol, pp.ell_expansion, pp.ell_max_iter);
+    os = o.x; ot = o.y; orr = o.z;
   } else {
     prox_sphere<R>(z1, z2, z3, mu.x, xr, os, ot, orr);
   }
@@ -1701,9 +1700,8 @@ __device__ __forceinline__ double quad_core(const ProxParams& pp, const QuadRow<
   const R xr = (JAC && !hybrid) ? xo0 : xnrm;
   R os, ot, orr;
   if (pp.friction == 1) {
-    R o[3];
-    prox_ellipsoid_call<R>(z1, z2, z3, mu0, mu1, mu2, xr, pp.ell_tol, pp.ell_expansion, pp.ell_max_iter, o);
-    os = o[0]; ot = o[1]; orr = o[2];
+    const V3<R> o = prox_ellipsoid_call<R>(z1, z2, z3, mu0, mu1, mu2, xr, pp.ell_tol, pp.ell_expansion, pp.ell_max_iter);
+    os = o.x; ot = o.y; orr = o.z;
   } else {
     prox_sphere<R>(z1, z2, z3, mu0, xr, os, ot, orr);
   }
@@ -1788,9 +1786,11 @@ template <typename R> __host__ __device__ constexpr size_t res_lds_bytes(int cc,
 constexpr int RES_LANES = 64;   // lanes of a resident group
 
 // LDS capacities of the resident layout (compile-time, so array offsets fold
-// into the LDS instructions); Dev::res_cap_c/b may set lower fit limits
-template <typename R> struct ResCap {
-  static constexpr int C = sizeof(R) == 4 ? 384 : 192;   // contacts
+// into the LDS instructions); Dev::res_cap_c/b may set lower fit limits.
+// H = 1: the half layout (FP32 ~68 KB), so that two solve workgroups share a
+// CU when a batch has more scenes than the device has CUs.
+template <typename R, int H = 0> struct ResCap {
+  static constexpr int C = (sizeof(R) == 4 ? 384 : 192) >> H;   // contacts
   static constexpr int B = C;                             // bodies
   static constexpr int O = 4 * C;                         // colour offsets (nc + 3 per lane)
   static constexpr size_t BYTES = res_lds_bytes<R>(C, B, O, RES_LANES);
@@ -1798,19 +1798,19 @@ template <typename R> struct ResCap {
   static constexpr int VB = (int)(BYTES / (6 * sizeof(V4<R>))) - 2;
 };
 
-template <typename R> struct ResLds {
+template <typename R, int H = 0> struct ResLds {
   V4<R> *JL, *JA, *JB;                     // per contact row (4 per contact)
   R *x0, *x1;                              // per contact row
   BodyLds<R> B;                            // per body (slot - b_lo), zero slot zs, discard slot ts
   int2* loc;                               // a contact's two bodies (slot - b_lo; zs: fixed)
   int4* lt;                                // per lane: first contact, K, colours, first colour offset
   int* coff;
-  static constexpr int zs = ResCap<R>::B;
-  static constexpr int ts = ResCap<R>::B + 1;
+  static constexpr int zs = ResCap<R, H>::B;
+  static constexpr int ts = ResCap<R, H>::B + 1;
   static constexpr int zsb = zs * rec_bytes<R>(), tsb = ts * rec_bytes<R>();   // their records' byte offsets
-  static constexpr int nul = ResCap<R>::C;
+  static constexpr int nul = ResCap<R, H>::C;
   __device__ explicit ResLds(unsigned char* p) {
-    constexpr int cc = ResCap<R>::C + 1, nbs = ResCap<R>::B + 2;
+    constexpr int cc = ResCap<R, H>::C + 1, nbs = ResCap<R, H>::B + 2;
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
     JL = v; JA = JL + 4 * cc; JB = JA + 4 * cc;
     B.at(JB + 4 * cc);
@@ -1822,12 +1822,12 @@ template <typename R> struct ResLds {
 };
 
 // ---- streaming layout: bodies in LDS, contact rows and iterates in global memory
-template <typename R> struct StreamLds {
+template <typename R, int H = 0> struct StreamLds {
   BodyLds<R> B;
-  static constexpr int zs = ResCap<R>::VB;
-  static constexpr int ts = ResCap<R>::VB + 1;
+  static constexpr int zs = ResCap<R, H>::VB;
+  static constexpr int ts = ResCap<R, H>::VB + 1;
   __device__ explicit StreamLds(unsigned char* p) {
-    constexpr int nbs = ResCap<R>::VB + 2;
+    constexpr int nbs = ResCap<R, H>::VB + 2;
     V4<R>* v = reinterpret_cast<V4<R>*>(p);
     (void)nbs;
     B.at(v);
@@ -1853,8 +1853,10 @@ template <typename R> __device__ GroupSize group_size(const Dev<R>& d, const Gro
   return g;
 }
 
-template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d, const Group& gr, const GroupSize& g) {
-  return g.nc <= d.res_cap_c && gr.b_hi - gr.b_lo <= d.res_cap_b && g.no <= ResCap<R>::O && g.nl <= RES_LANES;
+template <typename R, int H>
+__device__ __forceinline__ bool group_fits(const Dev<R>& d, const Group& gr, const GroupSize& g) {
+  return g.nc <= min(d.res_cap_c, ResCap<R, H>::C) && gr.b_hi - gr.b_lo <= min(d.res_cap_b, ResCap<R, H>::B) &&
+         g.no <= ResCap<R, H>::O && g.nl <= RES_LANES;
 }
 
 // One sweep of one resident lane by STRIDE threads (a wavefront or the
@@ -1863,8 +1865,8 @@ template <typename R> __device__ __forceinline__ bool group_fits(const Dev<R>& d
 // offsets end with two repeats of co[nc], so the colours after the last are
 // empty.  The merit, max |x_new - x_old| in FP64, is taken after the sweep
 // over the lane's rows (a sweep updates each of its contacts once).
-template <typename R, int STRIDE, bool SERIAL, bool JAC>
-__device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool hybrid, const R* xa, R* xn, R rs,
+template <typename R, int STRIDE, bool SERIAL, bool JAC, int H>
+__device__ double res_sweep(const ProxParams& pp, ResLds<R, H>& L, int4 lt, bool hybrid, const R* xa, R* xn, R rs,
                             int tid) {
   // the lane's table entry is the same for every thread: scalar loop control
   const int c0 = __builtin_amdgcn_readfirstlane(lt.x), K = __builtin_amdgcn_readfirstlane(lt.y);
@@ -1876,7 +1878,7 @@ __device__ double res_sweep(const ProxParams& pp, ResLds<R>& L, int4 lt, bool hy
   // end of colour c (a load; made scalar where it is used)
   auto col_end = [&](int c) { return SERIAL ? min(c + 1, K) : co[c + 1]; };
   // contact at position pos of a colour ending at end, or the null contact
-  auto pick = [&](int pos, int end) { return pos < end ? c0 + pos : ResLds<R>::nul; };
+  auto pick = [&](int pos, int end) { return pos < end ? c0 + pos : ResLds<R, H>::nul; };
   // rows, iterate and body slots of contact i (none of them change during a sweep)
   auto load = [&](int i, QuadRow<R>& it) {
     it.jl = L.JL[4 * i + q]; it.ja = L.JA[4 * i + q]; it.jb = L.JB[4 * i + q];
@@ -1976,7 +1978,7 @@ __device__ void finish_group(const Dev<R>& d, SceneState& st, const SolveCtl& ct
   }
 }
 
-template <typename R, bool COOP, int NT>
+template <typename R, bool COOP, int NT, int H>
 __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Group& gr, const GroupSize& gs,
                                      unsigned char* smem, double* sh_red, int* s_ok) {
   const SceneParams& sp = d.sp[slot];
@@ -1985,7 +1987,7 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
   const ProxParams pp(sp);
-  ResLds<R> L(smem);
+  ResLds<R, H> L(smem);
   const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
   // stage: lane table, colour offsets, contact rows and iterates, bodies
   if (threadIdx.x == 0) {
@@ -2021,10 +2023,10 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
   // the null contact: zero rows and iterates, both bodies fixed
   if (threadIdx.x < 4) {
     const V4<R> z = {R(0), R(0), R(0), R(0)};
-    const int r = 4 * ResLds<R>::nul + threadIdx.x;
+    const int r = 4 * ResLds<R, H>::nul + threadIdx.x;
     L.JL[r] = z; L.JA[r] = z; L.JB[r] = z;
     L.x0[r] = R(0); L.x1[r] = R(0);
-    if (threadIdx.x == 0) L.loc[ResLds<R>::nul] = make_int2(L.zsb, L.zsb);
+    if (threadIdx.x == 0) L.loc[ResLds<R, H>::nul] = make_int2(L.zsb, L.zsb);
   }
   stage_bodies(d, L.B, b0, nb, L.zs);
   __syncthreads();
@@ -2040,15 +2042,15 @@ __device__ void solve_group_resident(const Dev<R>& d, const int slot, const Grou
     for (int t = wave; t < gs.nl; t += nwave) {
       const int4 lt = L.lt[t];
       if (lt.y > WAVE_LANE_MAX && !serial) continue;
-      m = fmax(m, serial   ? res_sweep<R, 64, true, false>(pp, L, lt, hybrid, xa, xn, rs, lane)
-                  : jacobi ? res_sweep<R, 64, false, true>(pp, L, lt, hybrid, xa, xn, rs, lane)
-                           : res_sweep<R, 64, false, false>(pp, L, lt, hybrid, xa, xn, rs, lane));
+      m = fmax(m, serial   ? res_sweep<R, 64, true, false, H>(pp, L, lt, hybrid, xa, xn, rs, lane)
+                  : jacobi ? res_sweep<R, 64, false, true, H>(pp, L, lt, hybrid, xa, xn, rs, lane)
+                           : res_sweep<R, 64, false, false, H>(pp, L, lt, hybrid, xa, xn, rs, lane));
     }
     for (int t = 0; t < gs.nl; ++t) {
       const int4 lt = L.lt[t];
       if (lt.y <= WAVE_LANE_MAX || serial) continue;
-      m = fmax(m, jacobi ? res_sweep<R, NT, false, true>(pp, L, lt, hybrid, xa, xn, rs, (int)threadIdx.x)
-                         : res_sweep<R, NT, false, false>(pp, L, lt, hybrid, xa, xn, rs, (int)threadIdx.x));
+      m = fmax(m, jacobi ? res_sweep<R, NT, false, true, H>(pp, L, lt, hybrid, xa, xn, rs, (int)threadIdx.x)
+                         : res_sweep<R, NT, false, false, H>(pp, L, lt, hybrid, xa, xn, rs, (int)threadIdx.x));
     }
     const int dec = sweep_decide<R, COOP>(d, sp, ctl, m, it, gr.lead, sh_red, s_ok, L.B, nb, abort);
     if (abort) return;
@@ -2116,7 +2118,7 @@ __device__ double stream_sweep(const Dev<R>& d, const ProxParams& pp, const Body
   return m;
 }
 
-template <typename R, bool COOP, int NT>
+template <typename R, bool COOP, int NT, int H>
 __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group& gr, unsigned char* smem,
                                    double* sh_red, int* s_ok) {
   const SceneParams& sp = d.sp[slot];
@@ -2125,7 +2127,7 @@ __device__ void solve_group_stream(const Dev<R>& d, const int slot, const Group&
   const bool jacobi = sp.scheme == SCHEME_JACOBI || sp.scheme == SCHEME_JACOBI_HYBRID;
   const bool hybrid = sp.scheme == SCHEME_JACOBI_HYBRID;
   const ProxParams pp(sp);
-  StreamLds<R> L(smem);
+  StreamLds<R, H> L(smem);
   const int b0 = gr.b_lo, nb = gr.b_hi - gr.b_lo;
   stage_bodies(d, L.B, b0, nb, L.zs);
   __syncthreads();
@@ -2180,18 +2182,19 @@ template <typename R> __device__ __forceinline__ void no_contacts(SceneState& st
   st.solved = 0; st.iterations = 0; st.exitcode = 0; st.sweeps = 0; st.rejects = 0; st.final_merit = 0.0;
 }
 
-template <typename R, bool COOP, int NT>
+template <typename R, bool COOP, int NT, int H>
 __device__ __forceinline__ void solve_dispatch(const Dev<R>& d, int slot, const Group& gr, unsigned char* smem,
                                                double* sh_red, int* s_ok) {
   const GroupSize gs = group_size(d, gr, d.sp[slot].scheme == SCHEME_GS);
   V4<R>* vs = reinterpret_cast<V4<R>*>(smem);
-  if (group_fits(d, gr, gs)) solve_group_resident<R, COOP, NT>(d, slot, gr, gs, smem, sh_red, s_ok);
-  else if (gr.b_hi - gr.b_lo <= ResCap<R>::VB) solve_group_stream<R, COOP, NT>(d, slot, gr, smem, sh_red, s_ok);
+  if (group_fits<R, H>(d, gr, gs)) solve_group_resident<R, COOP, NT, H>(d, slot, gr, gs, smem, sh_red, s_ok);
+  else if (gr.b_hi - gr.b_lo <= ResCap<R, H>::VB) solve_group_stream<R, COOP, NT, H>(d, slot, gr, smem, sh_red, s_ok);
   else solve_group<R, false, COOP, NT>(d, slot, gr, vs, sh_red, s_ok);
 }
 
-// One workgroup of NT threads (NT / 64 wavefronts, one per sweep lane) per batch scene.
-template <typename R, int NT> __global__ void __launch_bounds__(NT) k_solve(Dev<R> d) {
+// One workgroup of NT threads (NT / 64 wavefronts, one per sweep lane) per
+// batch scene; H: the LDS layout (0 full, 1 half).
+template <typename R, int NT, int H> __global__ void __launch_bounds__(NT) k_solve(Dev<R> d) {
   extern __shared__ __align__(32) unsigned char solve_smem[];
   __shared__ double sh_red[NT / 64];
   __shared__ int s_ok;
@@ -2209,7 +2212,7 @@ template <typename R, int NT> __global__ void __launch_bounds__(NT) k_solve(Dev<
   }
   const int u0 = d.u_base[bs], u1 = u0 + d.u_cnt[bs];
   const Group gr{d.l_base[bs], d.l_base[bs] + d.l_cnt[bs], d.u_boff[u0], d.u_boff[u1], true};
-  solve_dispatch<R, false, NT>(d, slot, gr, solve_smem, sh_red, &s_ok);
+  solve_dispatch<R, false, NT, H>(d, slot, gr, solve_smem, sh_red, &s_ok);
 }
 
 // A single scene (its lanes are its colouring units): the lanes split into
@@ -2233,5 +2236,5 @@ template <typename R> __global__ void __launch_bounds__(SOLVE_BLOCK) k_solve_mul
   const long long nl = *d.n_lanes, G = gridDim.x;
   const int l_lo = (int)(blockIdx.x * nl / G), l_hi = (int)((blockIdx.x + 1) * nl / G);
   const Group gr{l_lo, l_hi, d.u_boff[l_lo], d.u_boff[l_hi], blockIdx.x == 0};
-  solve_dispatch<R, true, SOLVE_BLOCK>(d, slot, gr, solve_smem, sh_red, &s_ok);
+  solve_dispatch<R, true, SOLVE_BLOCK, 0>(d, slot, gr, solve_smem, sh_red, &s_ok);
 }

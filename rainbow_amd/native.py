@@ -60,7 +60,7 @@ class StepLog(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("unit_size", "lanes_per_scene", "resident_contacts", "resident_bodies",
                                         "lanes_per_group", "waves_per_group", "color_lds_contacts",
-                                        "color_body_bits")]
+                                        "color_body_bits", "solve_groups_per_cu")]
 
 
 class DeviceInfo(C.Structure):

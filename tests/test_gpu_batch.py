@@ -2,8 +2,9 @@
 
 * Every tuning of the step (unit size, sweep lanes per scene, resident LDS
   capacities small enough to force the velocity-only and global layouts,
-  per-body colour bitmaps too small for some or all colouring units) gives
-  bitwise the same contacts, colours, impulses and states.
+  the half LDS layout of two solve workgroups per CU, per-body colour
+  bitmaps too small for some or all colouring units) gives bitwise the same
+  contacts, colours, impulses and states.
 * The autotuner, which steps the batch under each candidate, leaves every
   scene where it found it: the steps after it are bitwise the untuned ones.
 * A request served by continuous batching, among other requests arriving
@@ -27,6 +28,8 @@ TUNINGS = [
     dict(resident_contacts=64, resident_bodies=64),           # groups too big: velocity-only layout
     dict(color_body_bits=1),                                  # small units coloured through body lists
     dict(unit_size=256, color_body_bits=24),                  # bitmaps for the units with few bodies only
+    dict(solve_groups_per_cu=2),                              # half LDS layout: two solve groups per CU
+    dict(solve_groups_per_cu=2, waves_per_group=8, lanes_per_scene=1),
 ]
 
 

@@ -121,6 +121,8 @@ template <typename R> struct Dev {
   unsigned* pal;                       // PALW words per unit-order position
   int* g_pc; unsigned char* g_done;    // colouring state of units too large for LDS
   int* col_off;                        // colour offsets of lane l at 2 * (l_start[l] + l)
+  int* col_fill;                       // the next free position of each colour of a lane (same layout)
+  int* big_list;                       // [0]: count, then the units left to k_color_units
   // a scene solved by several workgroups: per sweep and workgroup, the bits
   // of that workgroup's merit (all ones until it publishes them)
   unsigned long long* merit;

@@ -113,6 +113,7 @@ template <typename R> __global__ void k_unit_count(Dev<R> d) {
 }
 
 template <typename R> __global__ void k_unit_reset(Dev<R> d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.big_list[0] = 0;
   for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < d.cap_units; u += gridDim.x * blockDim.x) {
     d.u_start[u] = 0x7fffffff;
     d.u_end[u] = 0;
@@ -228,7 +229,6 @@ constexpr int BODY_BITS = 512;        // words of each per-body colour bitmap ar
 constexpr int SMALL_PW = 4;           // palette words per contact of a small unit (colours below 128)
 constexpr int PAIR_HASH = 2 * SMALL_UNIT;   // body-pair table of a small unit's set-up (power of two)
 constexpr unsigned short NO_SLOT = 0xffff;
-constexpr int U_LISTS = -1;           // u_ncol of a small unit left to the contact-list path
 
 // State of a small unit coloured through per-body colour bitmaps (about 14 KB,
 // so that many units share a CU).  Body b of the unit (its slot f_loc) owns
@@ -959,27 +959,30 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
 }
 
 // units of up to SMALL_UNIT contacts: one wavefront each, through per-body
-// bitmaps; a unit outgrowing them is marked U_LISTS for k_color_units
+// bitmaps; larger units, and those outgrowing the bitmaps, are listed in
+// big_list for k_color_units
 template <typename R> __global__ void __launch_bounds__(64) k_color_small(Dev<R> d) {
   __shared__ BitsColorLds s;
   if (aborted(d.err)) return;
   const int u = blockIdx.x;
   if (u >= *d.n_units) return;
   const int K = d.u_end[u] - d.u_start[u];
-  if (K <= 0 || K > SMALL_UNIT) return;
-  if (!color_small_bits<R>(d, u, s) && threadIdx.x == 0) d.u_ncol[u] = U_LISTS;
+  if (K <= 0) return;
+  if ((K > SMALL_UNIT || !color_small_bits<R>(d, u, s)) && threadIdx.x == 0)
+    d.big_list[1 + atomicAdd(&d.big_list[0], 1)] = u;
 }
 
-// larger units and the small units marked U_LISTS: a workgroup of
-// COLOR_UNIT_BLOCK threads each, neighbours through the bodies' contact lists
+// the listed units: a workgroup of COLOR_UNIT_BLOCK threads each (a grid of
+// at most one workgroup per CU walks the list), neighbours through the
+// bodies' contact lists
 template <typename R> __global__ void __launch_bounds__(COLOR_UNIT_BLOCK) k_color_units(Dev<R> d) {
   __shared__ ColorLds<COLOR_UNIT_BLOCK, UNIT_CAP, LIST_CAP, 0> s;
   if (aborted(d.err)) return;
-  const int u = blockIdx.x;
-  if (u >= *d.n_units) return;
-  const int K = d.u_end[u] - d.u_start[u];
-  if (K <= 0 || (K <= SMALL_UNIT && d.u_ncol[u] != U_LISTS)) return;
-  color_unit<R>(d, u, s);
+  const int n = d.big_list[0];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    color_unit<R>(d, d.big_list[1 + i], s);
+    __syncthreads();
+  }
 }
 
 template <typename R> struct ContactRow {
@@ -1142,20 +1145,35 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order_lan
   for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicAdd(&s_cnt[d.c_color[d.uperm[p0 + i]]], 1);
   __syncthreads();
   if (threadIdx.x == 0) {
+    int* cfill = d.col_fill + 2 * ((size_t)p0 + l);
     int acc = 0;
     for (int c = 0; c < nc; ++c) {
       const int v = s_cnt[c];
-      s_cnt[c] = acc;
       coff[c] = acc;
+      cfill[c] = acc;
       acc += v;
     }
     coff[nc] = acc;
     d.l_ncol[l] = nc;
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) {
+}
+
+// Contacts into colour order inside their lane (k_order_lanes set the
+// colours' offsets): GATHER_SPLIT workgroups per lane, each taking every
+// GATHER_SPLIT-th run of COLOR_BLOCK positions; a contact takes the next free
+// position of its colour (the order inside a colour changes no result: its
+// contacts share no free body).
+constexpr int GATHER_SPLIT = 8;
+template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_lane_gather(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int l = blockIdx.x;
+  if (l >= *d.n_lanes) return;
+  const int p0 = d.l_start[l], p1 = d.l_end[l];
+  if (p1 <= p0 || d.sp[d.bs_slot[d.l_bs[l]]].scheme == SCHEME_GS) return;   // (gathered by k_order_lanes)
+  int* cfill = d.col_fill + 2 * ((size_t)p0 + l);
+  for (int i = blockIdx.y * COLOR_BLOCK + threadIdx.x; i < p1 - p0; i += GATHER_SPLIT * COLOR_BLOCK) {
     const int k = d.uperm[p0 + i];
-    gather_contact(d, p0 + atomicAdd(&s_cnt[d.c_color[k]], 1), k);
+    gather_contact(d, p0 + atomicAdd(&cfill[d.c_color[k]], 1), k);
   }
 }
 

@@ -525,6 +525,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.u_cnt = P(int*, u_cnt); d.u_base = P(int*, u_base); d.n_units = P(int*, n_units); d.cap_units = c->cap_units;
   d.unit_T = c->unit_T;
   d.unit_split = multi ? 0 : (c->tune.lanes_per_scene > 0 ? c->tune.lanes_per_scene : batch_waves(c));
+  d.items_nb = multi ? -1 : (batch_half(c, c->ns) ? ResCap<R, 1>::VB : ResCap<R, 0>::VB);
   d.u_T = P(int*, u_T);
   d.color_cap = c->tune.color_lds_contacts > 0 ? c->tune.color_lds_contacts : UNIT_CAP;
   d.color_bits = c->tune.color_body_bits > 0 ? std::min(c->tune.color_body_bits, BODY_BITS) : BODY_BITS;

@@ -1002,7 +1002,8 @@ __device__ __forceinline__ void contact_rows(V4<R> n4, V4<R> rA4, V4<R> rB4, Con
 // ------------------------------------------------------------ sweep order
 // Gather one contact into the solver arrays at position `pos`; s_loc holds
 // the bodies' slots (index into u_blist; -1 for a fixed body).
-template <typename R> __device__ __forceinline__ void gather_contact(const Dev<R>& d, int pos, int k) {
+// ITEMS: also the per-contact items of the global layout
+template <typename R> __device__ __forceinline__ void gather_contact(const Dev<R>& d, int pos, int k, bool items) {
   const V4<R> mu = d.c_mu[k];
   const V4<R> n = d.c_n[k], rA = d.c_rA[k], rB = d.c_rB[k];
   const int A = d.c_ia[k], B = d.c_ib[k];
@@ -1011,21 +1012,23 @@ template <typename R> __device__ __forceinline__ void gather_contact(const Dev<R
   if (fa) { const int f = d.b2f[A]; sa = d.u_boff[d.f_unit[f]] + d.f_loc[f]; }
   if (fb) { const int f = d.b2f[B]; sb = d.u_boff[d.f_unit[f]] + d.f_loc[f]; }
   d.perm[pos] = k;
-  d.s_id[pos] = make_int2(A, B);
   d.s_loc[pos] = make_int2(sa, sb);
-  d.s_n[pos] = V4<R>{n.x, n.y, n.z, mu.x};
-  d.s_rA[pos] = V4<R>{rA.x, rA.y, rA.z, mu.y};
-  d.s_rB[pos] = V4<R>{rB.x, rB.y, rB.z, mu.z};
-  d.s_b[pos] = d.c_b[k];
-  d.s_r0[pos] = d.c_r0[k];
-  const V4<R> z = {R(0), R(0), R(0), R(0)};
-  V4<R>* W = d.s_W + 4 * (size_t)pos;
-  W[0] = fa ? d.b_W0[A] : z;
-  W[1] = fa ? d.b_W1[A] : z;
-  W[2] = fb ? d.b_W0[B] : z;
-  W[3] = fb ? d.b_W1[B] : z;
-  d.x0[pos] = z;
-  d.x1[pos] = z;
+  if (items) {
+    d.s_id[pos] = make_int2(A, B);
+    d.s_n[pos] = V4<R>{n.x, n.y, n.z, mu.x};
+    d.s_rA[pos] = V4<R>{rA.x, rA.y, rA.z, mu.y};
+    d.s_rB[pos] = V4<R>{rB.x, rB.y, rB.z, mu.z};
+    d.s_b[pos] = d.c_b[k];
+    d.s_r0[pos] = d.c_r0[k];
+    const V4<R> z = {R(0), R(0), R(0), R(0)};
+    V4<R>* W = d.s_W + 4 * (size_t)pos;
+    W[0] = fa ? d.b_W0[A] : z;
+    W[1] = fa ? d.b_W1[A] : z;
+    W[2] = fb ? d.b_W0[B] : z;
+    W[3] = fb ? d.b_W1[B] : z;
+    d.x0[pos] = z;
+    d.x1[pos] = z;
+  }
   // the rows of J once per step, for the row-major (quad) sweeps
   ContactRow<R> J;
   contact_rows(V4<R>{n.x, n.y, n.z, R(0)}, rA, rB, J);
@@ -1127,7 +1130,7 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order_lan
       const int k = d.uperm[p0 + i];
       int rank = 0;
       for (int j = 0; j < K; ++j) rank += d.uperm[p0 + j] < k ? 1 : 0;
-      gather_contact(d, p0 + rank, k);
+      gather_contact(d, p0 + rank, k, true);
     }
     if (threadIdx.x == 0) d.l_ncol[l] = 0;
     return;
@@ -1171,9 +1174,17 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_lane_gath
   const int p0 = d.l_start[l], p1 = d.l_end[l];
   if (p1 <= p0 || d.sp[d.bs_slot[d.l_bs[l]]].scheme == SCHEME_GS) return;   // (gathered by k_order_lanes)
   int* cfill = d.col_fill + 2 * ((size_t)p0 + l);
+  // the items only where the global layout may read them: a group too large
+  // for the velocity-only layout, or the post-stabilisation rows
+  const int bs = d.l_bs[l];
+  bool items = d.items_nb < 0 || d.sp[d.bs_slot[bs]].use_post_stab;
+  if (!items) {
+    const int u0 = d.u_base[bs];
+    items = d.u_boff[u0 + d.u_cnt[bs]] - d.u_boff[u0] > d.items_nb;
+  }
   for (int i = blockIdx.y * COLOR_BLOCK + threadIdx.x; i < p1 - p0; i += GATHER_SPLIT * COLOR_BLOCK) {
     const int k = d.uperm[p0 + i];
-    gather_contact(d, p0 + atomicAdd(&cfill[d.c_color[k]], 1), k);
+    gather_contact(d, p0 + atomicAdd(&cfill[d.c_color[k]], 1), k, items);
   }
 }
 

@@ -180,6 +180,45 @@ def test_large_unit_colouring_paths_bitwise(bank):
         ctx.close()
 
 
+def test_half_layout_global_groups_bitwise(bank):
+    """A batch scene of more bodies than the half LDS layout holds (800
+    spheres; about 700 fit) solves from the per-contact items in global
+    memory under solve_groups_per_cu = 2, and from the velocity-only layout
+    under 1: both give bitwise the same colours, impulses and states."""
+    ctx = native.Context(0, "fp32")
+    try:
+        engs = [scenes.serving_request(4400, bank, nmin=800, nmax=800, params=dict(precision="fp32")),
+                scenes.mixed_drop(3210, bank, nmax=96, params=dict(precision="fp32"))]
+        sids = []
+        for e in engs:
+            desc, keep = engine_desc(e, ctx)
+            sid = ctx.scene_create(desc)
+            ctx.set_state(sid, *gather_state(list(e.bodies.values())))
+            sids.append(sid)
+        ctx.step(sids, 300, 0.0, 0)
+        ctx.sync()
+        start = [ctx.get_state(sid, len(e.bodies)) for sid, e in zip(sids, engs)]
+        runs = []
+        for groups in (1, 2):
+            ctx.set_tuning(solve_groups_per_cu=groups, lanes_per_scene=1)
+            for sid, st in zip(sids, start):
+                ctx.set_state(sid, *st)
+                ctx.set_step_index(sid, 300)
+            snaps = []
+            for _ in range(3):
+                ctx.step(sids, 1, 0.0, 0)
+                for sid, e in zip(sids, engs):
+                    c = ctx.get_contacts(sid)
+                    snaps.append([c["color"], c["impulse"]] + list(ctx.get_state(sid, len(e.bodies))))
+            runs.append(snaps)
+        assert len(runs[0][0][0]) > 500, len(runs[0][0][0])
+        for a, b in zip(*runs):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+    finally:
+        ctx.close()
+
+
 def test_serving_alone_equals_mixed(bank):
     ctx = native.Context(0, "fp32")
     try:

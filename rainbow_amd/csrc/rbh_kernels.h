@@ -683,7 +683,9 @@ template <typename R> __global__ void __launch_bounds__(256) k_cand_write(Dev<R>
 // Each wavefront takes a contiguous run of candidates (pairs, and so scenes,
 // are contiguous), so a lane's scene changes only at scene boundaries and the
 // per-scene counters are merged once per wavefront.
-template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
+// (six wavefronts per SIMD: FP32 fits 78 VGPRs without spilling; the gathers'
+// latency is hidden by occupancy)
+template <typename R> __global__ void __launch_bounds__(256, 6) k_frank_wolfe(Dev<R> d) {
   if (aborted(d.err)) return;
   const int nc = *d.n_cand;
   unsigned long long evals = 0;
@@ -738,7 +740,8 @@ template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
     R best = gn0;
     if (gn1 < best) { best = gn1; i0 = 1; }
     if (gn2 < best) { i0 = 2; }
-    V3<R> x = P[i0];
+    // corners picked by selects, not by index: an indexed P would live in memory
+    V3<R> x = i0 == 0 ? P[0] : (i0 == 1 ? P[1] : P[2]);
     const R tol = R(sp.fw_tol);
     int nfw = 0;
     for (int i = 0; i < sp.fw_max_iter; ++i) {
@@ -750,7 +753,8 @@ template <typename R> __global__ void k_frank_wolfe(Dev<R> d) {
       if (o1 < ob) { ob = o1; vtx = 1; }
       if (o2 < ob) { ob = o2; vtx = 2; }
       R alpha = R(2) / R(i + 2);
-      x = x + alpha * (P[vtx] - x);
+      const V3<R> pv = vtx == 0 ? P[0] : (vtx == 1 ? P[1] : P[2]);
+      x = x + alpha * (pv - x);
       if (ob > tol) break;
     }
     bool ok = false;

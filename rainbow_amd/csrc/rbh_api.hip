@@ -135,7 +135,7 @@ struct rbh_ctx {
   long long cells_total = 0;   // values of the cell-major grid copies of all shapes
   // body arena
   RangeAlloc bodies, tris, res;
-  DBuf r, q, v, w, b_shape, b_slot, b_fixed, b_mat, b_tri, b_mass, b_inertia, b_grav, b_damp;
+  DBuf r, q, v, w, b_shape, b_slot, b_fixed, b_mat, b_tri, b_mass, b_inertia, b_grav, b_damp, b_broad;
   DBuf W0, W1, du0, du1, lo, hi, ws0, ws1, ww0, ww1, hd0, hd1;
   DBuf tlo, thi, clo, chi;
   // hinges of all live scenes, packed by scene slot (rebuilt when they change)
@@ -234,7 +234,7 @@ int ensure(rbh_ctx* c, DBuf& b, size_t bytes, bool keep = false) {
 
 std::vector<DBuf*> all_bufs(rbh_ctx* c) {
   return {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->d_cells, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
-          &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->W0,
+          &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->b_broad, &c->W0,
           &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi, &c->clo, &c->chi,
           &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_order, &c->bs_foff, &c->b2f,
           &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs, &c->pair_items, &c->item_off,
@@ -282,7 +282,8 @@ int body_elem_sizes(rbh_ctx* c, std::vector<std::pair<DBuf*, size_t>>& out) {
          {&c->b_slot, 4}, {&c->b_fixed, 4}, {&c->b_mat, 4},     {&c->b_tri, 4},    {&c->b_mass, R},
          {&c->b_inertia, V}, {&c->b_grav, V}, {&c->b_damp, V},  {&c->W0, V},       {&c->W1, V},
          {&c->du0, V},    {&c->du1, V},     {&c->lo, V},        {&c->hi, V},       {&c->ws0, V},
-         {&c->ws1, V},    {&c->ww0, V},     {&c->ww1, V},      {&c->hd0, V},      {&c->hd1, V}};
+         {&c->ws1, V},    {&c->ww0, V},     {&c->ww1, V},      {&c->hd0, V},      {&c->hd1, V},
+         {&c->b_broad, 16}};
   return 0;
 }
 
@@ -493,6 +494,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.r = P(V4<R>*, r); d.q = P(V4<R>*, q); d.v = P(V4<R>*, v); d.w = P(V4<R>*, w);
   d.b_shape = P(const int*, b_shape); d.b_slot = P(const int*, b_slot); d.b_fixed = P(const int*, b_fixed);
   d.b_mat = P(const int*, b_mat); d.b_tri = P(const int*, b_tri); d.b_mass = P(const R*, b_mass);
+  d.b_broad = P(const int4*, b_broad);
   d.b_inertia = P(const V4<R>*, b_inertia); d.b_grav = P(const V4<R>*, b_grav); d.b_damp = P(const V4<R>*, b_damp);
   d.b_W0 = P(V4<R>*, W0); d.b_W1 = P(V4<R>*, W1); d.b_du0 = P(V4<R>*, du0); d.b_du1 = P(V4<R>*, du1);
   d.b_lo = P(V4<R>*, lo); d.b_hi = P(V4<R>*, hi); d.ws0 = P(V4<R>*, ws0); d.ws1 = P(V4<R>*, ws1);
@@ -1050,6 +1052,7 @@ template <typename R> int get_state(rbh_ctx* c, const SceneHost& s, double* r, d
 template <typename R> int scene_statics(rbh_ctx* c, const rbh_scene_desc* d, const SceneHost& s, int slot) {
   const int n = d->n_bodies;
   std::vector<int> sh(n), sl(n, slot), fx(n), mt(n), tr(n);
+  std::vector<int4> bb(n);
   std::vector<R> ms(n);
   std::vector<V4<R>> in(n), gr(n), dp(n), z(n, V4<R>{R(0), R(0), R(0), R(0)});
   std::vector<V4<R>> qid(n, V4<R>{R(1), R(0), R(0), R(0)});
@@ -1059,6 +1062,7 @@ template <typename R> int scene_statics(rbh_ctx* c, const rbh_scene_desc* d, con
     fx[i] = d->fixed ? (d->fixed[i] != 0) : 0;
     mt[i] = d->material ? d->material[i] : 0;
     tr[i] = toff;
+    bb[i] = make_int4(toff, c->shapes[sh[i]].nt, c->shapes[sh[i]].t_off, slot);
     toff += c->shapes[sh[i]].nt;
     ms[i] = R(d->mass[i]);
     in[i] = V4<R>{R(d->inertia[3 * i]), R(d->inertia[3 * i + 1]), R(d->inertia[3 * i + 2]), R(0)};
@@ -1073,7 +1077,7 @@ template <typename R> int scene_statics(rbh_ctx* c, const rbh_scene_desc* d, con
   rc |= put(c->b_shape, sh.data(), 4); rc |= put(c->b_slot, sl.data(), 4); rc |= put(c->b_fixed, fx.data(), 4);
   rc |= put(c->b_mat, mt.data(), 4); rc |= put(c->b_tri, tr.data(), 4); rc |= put(c->b_mass, ms.data(), sizeof(R));
   rc |= put(c->b_inertia, in.data(), sizeof(V4<R>)); rc |= put(c->b_grav, gr.data(), sizeof(V4<R>));
-  rc |= put(c->b_damp, dp.data(), sizeof(V4<R>));
+  rc |= put(c->b_damp, dp.data(), sizeof(V4<R>)); rc |= put(c->b_broad, bb.data(), sizeof(int4));
   rc |= put(c->r, z.data(), sizeof(V4<R>)); rc |= put(c->q, qid.data(), sizeof(V4<R>));
   rc |= put(c->v, z.data(), sizeof(V4<R>)); rc |= put(c->w, z.data(), sizeof(V4<R>));
   if (rc) return rc;

@@ -67,6 +67,8 @@ template <typename R> struct Dev {
   // body arena
   V4<R>* r; V4<R>* q; V4<R>* v; V4<R>* w;
   const int* b_shape; const int* b_slot; const int* b_fixed; const int* b_mat; const int* b_tri;
+  const int4* b_broad;                 // per body, read by the broad phase in one load: b_tri, the shape's
+                                       // nt and t_off, b_slot
   const R* b_mass; const V4<R>* b_inertia; const V4<R>* b_grav; const V4<R>* b_damp;
   V4<R>* b_W0; V4<R>* b_W1;            // (invI xx,yy,zz,xy), (invI xz,yz, 1/m, 0)
   V4<R>* b_du0; V4<R>* b_du1;          // W f_ext (linear, angular)
@@ -428,24 +430,24 @@ __device__ __forceinline__ void wave_sync_lds() {
 // candidate set, without the reach bound (set while overlaps are counted).
 // Returns the number of this lane's positions flagged.
 template <typename R>
-__device__ int cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, int x1, bool count_all, bool exact,
-                           V4<R>* lo_s, V4<R>* hi_s, V4<R>* clo_s, V4<R>* chi_s, int2* crange_s,
-                           unsigned long long& overlaps) {
+__device__ int cand_block(const Dev<R>& d, int X, int Y, int4 gX, int4 gY, int slot_base, int x0, int x1,
+                          bool count_all, bool exact, V4<R>* lo_s, V4<R>* hi_s, V4<R>* clo_s, V4<R>* chi_s,
+                          int2* crange_s, unsigned long long& overlaps) {
   constexpr int XQ = CAND_BLK / 64;
   const int cap = CandLds<R>::CAP;
   const int lane = threadIdx.x & 63;
-  const ShapeInfo& sX = d.shapes[d.b_shape[X]];
-  const ShapeInfo& sY = d.shapes[d.b_shape[Y]];
-  const int tX = d.b_tri[X], tY = d.b_tri[Y];
-  const int kY = blk_box(tY, sY.nt);
-  const V4<R> xlo = d.clo[blk_box(tX, sX.nt) + x0 / CAND_BLK], xhi = d.chi[blk_box(tX, sX.nt) + x0 / CAND_BLK];
+  // gX, gY: the bodies' b_broad records (first triangle box, shape nt and t_off, scene slot)
+  const int tX = gX.x, tY = gY.x, ntY = gY.y, tXo = gX.z;
+  const int kY = blk_box(tY, ntY);
+  const V4<R> xlo = d.clo[blk_box(tX, gX.y) + x0 / CAND_BLK], xhi = d.chi[blk_box(tX, gX.y) + x0 / CAND_BLK];
   const V4<R> rYlo = d.b_lo[Y], rYhi = d.b_hi[Y];
   // A triangle of X farther from Y's origin than reach_Y + envelope + Y's cell
   // diagonal gives no contact against Y's SDF (the bound of within_reach,
   // taken for the triangle's box, which contains the triangle); such
   // triangles are not candidates unless every overlap is being counted.
-  const R env = R(d.sp[d.b_slot[X]].envelope);
-  const R lim = (R(sY.reach) + R(sY.hdiag) + env) * R(1.001) + R(1e-4);
+  // (Y's reach and cell diagonal ride in the w of its body box)
+  const R env = R(d.sp[gX.w].envelope);
+  const R lim = (rYlo.w + rYhi.w + env) * R(1.001) + R(1e-4);
   const V4<R> oY = d.r[Y];
   auto beyond = [&](const V4<R>& lo, const V4<R>& hi) {
     const R ex = fmax(fmax(lo.x - oY.x, oY.x - hi.x), R(0));
@@ -465,7 +467,7 @@ __device__ int cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, 
     xt[q] = -1;
     xl[q] = xyz(xlo); xh[q] = xyz(xhi);
     if (p >= x1) continue;
-    xt[q] = d.sT[sX.t_off + p].w;
+    xt[q] = d.sT[tXo + p].w;
     if (!block_in) continue;
     const int cl = p / CLUSTER;
     const V4<R> cbl = d.clo[tX + cl], cbh = d.chi[tX + cl];
@@ -477,7 +479,7 @@ __device__ int cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, 
   }
   unsigned long long cnt = 0;
   if (__ballot(live != 0)) {
-    const int nbY = n_blocks(sY.nt);
+    const int nbY = n_blocks(ntY);
     int ywin = 0;                          // window of 64 Y blocks ...
     unsigned long long ymask = 0;          // ... whose boxes meet the block box, not yet staged
     {
@@ -489,7 +491,7 @@ __device__ int cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, 
       // stage the next run of Y triangles that meet the block box, two
       // clusters (64 positions) at a time, with the boxes of their clusters
       int nfill = 0, ncl = 0;
-      while (ybase < sY.nt) {
+      while (ybase < ntY) {
         if (ybase % CAND_BLK == 0) {
           // the next Y block whose box meets the block box
           int yb = ybase / CAND_BLK;
@@ -504,13 +506,13 @@ __device__ int cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, 
             if (m) { yb = ywin + __ffsll((long long)m) - 1; break; }
             yb = ywin + 64;
           }
-          if (yb >= nbY) { ybase = sY.nt; break; }
+          if (yb >= nbY) { ybase = ntY; break; }
           ybase = yb * CAND_BLK;
         }
         // this lane's cluster box and triangle box, loaded together
         const int p = ybase + lane;
-        const bool livey = p < sY.nt;
-        const int cl = min(p, sY.nt - 1) / CLUSTER;
+        const bool livey = p < ntY;
+        const int cl = min(p, ntY - 1) / CLUSTER;
         const V4<R> cbl = d.clo[tY + cl], cbh = d.chi[tY + cl];
         V4<R> lo = cbl, hi = cbh;
         if (livey) { lo = d.tlo[tY + p]; hi = d.thi[tY + p]; }
@@ -565,7 +567,7 @@ __device__ int cand_block(const Dev<R>& d, int X, int Y, int slot_base, int x0, 
         if (hitq) flg |= 1u << q;
       }
       wave_sync_lds();
-      if (ybase >= sY.nt) break;
+      if (ybase >= ntY) break;
       // every position that can meet Y is already a candidate: the rest of Y changes no flag
       if (!count_all && !__ballot((live & ~flg) != 0)) break;
     }
@@ -623,7 +625,8 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
   for (int item = lo + wx * CAND_WAVES + wave; item < hi; item += gx * CAND_WAVES) {
     const int p = item_pair(d.item_off, np, item);
     const int2 ab = d.pairs[p];
-    const int ntA = d.shapes[d.b_shape[ab.x]].nt, ntB = d.shapes[d.b_shape[ab.y]].nt;
+    const int4 gA = d.b_broad[ab.x], gB = d.b_broad[ab.y];
+    const int ntA = gA.y, ntB = gB.y;
     const int k = item - d.item_off[p];
     const bool side0 = k < n_blocks(ntA);
     const int X = side0 ? ab.x : ab.y, Y = side0 ? ab.y : ab.x;
@@ -631,8 +634,8 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
     const int x0 = blk * CAND_BLK, x1 = min(x0 + CAND_BLK, side0 ? ntA : ntB);
     unsigned long long ov = 0;
     const int slot_base = d.slot_off[p] + (side0 ? 0 : ntA);
-    int nset = cand_block(d, X, Y, slot_base, x0, x1, count_all && side0, count_all, lo_s[wave], hi_s[wave],
-                          clo_s[wave], chi_s[wave], crange_s[wave], ov);
+    int nset = cand_block(d, X, Y, side0 ? gA : gB, side0 ? gB : gA, slot_base, x0, x1, count_all && side0,
+                          count_all, lo_s[wave], hi_s[wave], clo_s[wave], chi_s[wave], crange_s[wave], ov);
     nset = wave_sum(nset);   // candidates of this block
     if ((threadIdx.x & 63) == 0 && nset) atomicAdd(&d.side_cnt[2 * p + (side0 ? 0 : 1)], nset);
     if (count_all && side0) {

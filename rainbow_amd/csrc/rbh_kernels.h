@@ -90,6 +90,7 @@ template <typename R> struct Dev {
   int* row_cnt; int* row_off;          // [nb], [nb+1]
   int2* pairs; int* pair_sz; int* slot_off; int* n_pairs; int cap_pairs;
   int* pair_items; int* item_off;      // candidate work items (triangle blocks of both sides) per pair, scan
+  int* item_pair_of; int cap_items;    // the pair of each work item (k_item_map), its capacity
   unsigned char* slot_flag; int* n_slots; int cap_slots;
   int* side_cnt; int* side_off; int* n_sides;   // candidates per (pair, side) 2 p + side, their scan, 2 np
   int* cand_pair; int* cand_tri; int* pair_cstart; int* n_cand; int cap_cand;
@@ -594,6 +595,14 @@ __device__ __forceinline__ int item_pair(const int* __restrict__ item_off, int n
   return lo;
 }
 
+// the pair of every work item, written once per step for the wavefronts of k_cand_flags
+template <typename R> __global__ void k_item_map(Dev<R> d) {
+  const int np = *d.n_pairs;
+  if (np > d.cap_pairs || aborted(d.err)) return;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x)
+    for (int i = d.item_off[p], e = min(d.item_off[p + 1], d.cap_items); i < e; ++i) d.item_pair_of[i] = p;
+}
+
 // one wavefront per item: the items of pair p are A's blocks, then B's
 template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags(Dev<R> d) {
   __shared__ V4<R> lo_s[CAND_WAVES][CandLds<R>::CAP];
@@ -623,7 +632,7 @@ template <typename R> __global__ void __launch_bounds__(CAND_BLOCK) k_cand_flags
   const int gx = (G - xcd + 7) >> 3;
   const int lo = (int)((long long)nitems * before / G), hi = (int)((long long)nitems * (before + gx) / G);
   for (int item = lo + wx * CAND_WAVES + wave; item < hi; item += gx * CAND_WAVES) {
-    const int p = item_pair(d.item_off, np, item);
+    const int p = nitems <= d.cap_items ? d.item_pair_of[item] : item_pair(d.item_off, np, item);
     const int2 ab = d.pairs[p];
     const int4 gA = d.b_broad[ab.x], gB = d.b_broad[ab.y];
     const int ntA = gA.y, ntB = gB.y;

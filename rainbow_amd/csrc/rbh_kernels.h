@@ -369,14 +369,14 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
   const R env = R(d.sp[d.b_slot[b]].envelope);
   const bool all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   const V4<R> ra = d.r[b];
-  const int ntA = d.shapes[d.b_shape[b]].nt;
+  const int ntA = d.b_broad[b].y;
   for (int j0 = b + 1; j0 < end; j0 += 64) {
     int j = j0 + threadIdx.x;
     bool hit = j < end && pair_kept(d, ra, j, alo, ahi, env, all);
     unsigned long long m = __ballot(hit);
     if (hit) {
       int pos = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));
-      const int ntB = d.shapes[d.b_shape[j]].nt;
+      const int ntB = d.b_broad[j].y;
       d.pairs[pos] = make_int2(b, j);
       d.pair_sz[pos] = ntA + ntB;
       d.pair_items[pos] = n_blocks(ntA) + n_blocks(ntB);
@@ -671,8 +671,8 @@ template <typename R> __global__ void __launch_bounds__(256) k_cand_write(Dev<R>
   for (int sd = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; sd < 2 * np; sd += nw) {
     const int p = sd >> 1, side = sd & 1;
     const int2 ab = d.pairs[p];
-    const int ntA = d.shapes[d.b_shape[ab.x]].nt;
-    const int nt = side ? d.shapes[d.b_shape[ab.y]].nt : ntA;
+    const int ntA = d.b_broad[ab.x].y;
+    const int nt = side ? d.b_broad[ab.y].y : ntA;
     const int base = d.slot_off[p] + (side ? ntA : 0);
     int off = d.side_off[sd];
     if (side == 0 && lane == 0) d.pair_cstart[p] = off;

@@ -148,7 +148,7 @@ struct rbh_ctx {
   std::vector<SceneParams> sp_host;
   std::vector<double> mat_host;   // interaction tables of the live scenes (4 per entry), ranges from `mats`
   RangeAlloc mats;                // entries of mat_host: a scene's nmat^2 block, reused after destroy
-  DBuf d_sp, d_ss, d_mat, d_res;
+  DBuf d_sp, d_ss, d_mat, d_res, d_fwacc;
   bool scenes_dirty = true;
   long long arena_gen = 0;
   // batch workspace
@@ -236,7 +236,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
   return {&c->d_shapes, &c->d_sV, &c->d_sT, &c->d_sdf, &c->d_cells, &c->r, &c->q, &c->v, &c->w, &c->b_shape, &c->b_slot,
           &c->b_fixed, &c->b_mat, &c->b_tri, &c->b_mass, &c->b_inertia, &c->b_grav, &c->b_damp, &c->b_broad, &c->W0,
           &c->W1, &c->du0, &c->du1, &c->lo, &c->hi, &c->ws0, &c->ws1, &c->ww0, &c->ww1, &c->tlo, &c->thi, &c->clo, &c->chi,
-          &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->bb_body, &c->bs_slot, &c->bs_order, &c->bs_foff, &c->b2f,
+          &c->d_sp, &c->d_ss, &c->d_mat, &c->d_res, &c->d_fwacc, &c->bb_body, &c->bs_slot, &c->bs_order, &c->bs_foff, &c->b2f,
           &c->f_end, &c->d_nb, &c->row_cnt, &c->row_off, &c->pairs, &c->pair_sz, &c->slot_off, &c->n_pairs, &c->pair_items, &c->item_off,
           &c->slot_flag, &c->side_cnt, &c->side_off, &c->n_sides, &c->n_slots, &c->cand_pair, &c->cand_tri, &c->pair_cstart,
           &c->n_cand, &c->cand_p, &c->cand_n, &c->cand_keep, &c->keep_scan, &c->kidx, &c->kpos,
@@ -380,6 +380,7 @@ int sync_scenes(rbh_ctx* c) {
   int rc = upload(c, c->d_sp, c->sp_host);
   if (!rc) rc = upload(c, c->d_mat, c->mat_host);
   if (!rc) rc = ensure(c, c->d_ss, std::max<size_t>(n, 1) * sizeof(SceneState), true);
+  if (!rc) rc = ensure(c, c->d_fwacc, std::max<size_t>(n, 1) * FW_SPREAD * FW_STRIDE * sizeof(unsigned long long), true);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->scenes_dirty = false;
@@ -493,6 +494,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.sdf = P(const R*, d_sdf);
   d.cells = P(const R*, d_cells);
   d.sp = P(const SceneParams*, d_sp); d.ss = P(SceneState*, d_ss); d.residuals = P(double*, d_res);
+  d.fw_acc = P(unsigned long long*, d_fwacc);
   d.mat = P(const double*, d_mat);
   d.r = P(V4<R>*, r); d.q = P(V4<R>*, q); d.v = P(V4<R>*, v); d.w = P(V4<R>*, w);
   d.b_shape = P(const int*, b_shape); d.b_slot = P(const int*, b_slot); d.b_fixed = P(const int*, b_fixed);

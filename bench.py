@@ -311,6 +311,42 @@ def batched_leg(n_scenes, steps, settle, plain_sample, plain_steps, cpu=True):
     return out
 
 
+def batched_trace_leg(n_scenes, settle=155, steps=5):
+    """The batched leg's scenes stepped kernel by kernel, for a profiler's kernel
+    trace: `settle` steps (the drops land), then `steps` steps with one sweep
+    lane per scene (the shape the autotuner picks on this batch), one launch
+    sequence and one synchronisation each.  No hipGraph: a graph replay of this
+    batch under rocprofv3's kernel trace ended the process."""
+    from rainbow_amd import native, scenes
+    from rainbow_amd.stepper import engine_desc, gather_state, shared_context
+    ctx = shared_context(0, "fp32")
+    bank = scenes.ShapeBank(resolution=32)
+    engs = [scenes.mixed_drop(3000 + s, bank) for s in range(n_scenes)]
+    sids = []
+    for e in engs:
+        desc, keep = engine_desc(e, ctx)
+        sid = ctx.scene_create(desc)
+        ctx.set_state(sid, *gather_state(list(e.bodies.values())))
+        sids.append(sid)
+    log("batched trace: %d scenes built; %d steps to land" % (n_scenes, settle))
+    ctx.step(sids, settle, 0.0, 0)
+    ctx.sync()
+    ctx.set_tuning(lanes_per_scene=1)
+    ms = []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        ctx.step(sids, 1, 0.0, 0)
+        ctx.sync()
+        ms.append(1e3 * (time.perf_counter() - t0))
+        log("batched trace: step %d" % (settle + k + 1))
+    for sid in sids:
+        ctx.scene_destroy(sid)
+    from rainbow_amd import autotune
+    ctx.set_tuning(**autotune.DEFAULT)
+    return {"workload": "mixed_drop_%d_scenes" % n_scenes, "steps_traced": "%d..%d" % (settle + 1, settle + steps),
+            "ms_per_step_eager": ms}
+
+
 def _oracle_scene_steps(job):
     """Worker: one oracle step of each (scene, dt) in job on one thread; seconds per scene."""
     from threadpoolctl import threadpool_limits
@@ -603,7 +639,7 @@ def main():
     ap.add_argument("--no-hinges", action="store_true", help="skip the hinge-chain leg")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the hinge leg's batched steps one kernel at a time instead of replaying a hipGraph")
-    ap.add_argument("--only", choices=("batched", "serving", "stress", "hinges"), default=None,
+    ap.add_argument("--only", choices=("batched", "batched-trace", "serving", "stress", "hinges"), default=None,
                     help="run one of the extra legs alone (for profiling) and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -617,6 +653,9 @@ def main():
         return
     if args.only == "batched":
         print(json.dumps(batched_leg(args.batched_scenes, 20, 150, 64, 5, cpu=not args.no_cpu)), flush=True)
+        return
+    if args.only == "batched-trace":
+        print(json.dumps(batched_trace_leg(args.batched_scenes)), flush=True)
         return
     rates = [float(x) for x in args.serving_rates.split(",") if x]
     if args.only == "serving":

@@ -734,7 +734,7 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   auto mark = [&](int st) { if (ev) (void)hipEventRecord(ev[st], s); };
   mark(ST_BEGIN);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, d.err, 16, 0);
-  hipLaunchKernelGGL(k_scene_reset<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_scene_reset<R>, dim3(grid_for(ns * 64, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_body_prep<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_refit<R>, dim3(nb), dim3(64), 0, s, d);
   mark(ST_PREP);
@@ -821,7 +821,7 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
     hipLaunchKernelGGL(k_post_integrate<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
   }
   mark(ST_POST);
-  hipLaunchKernelGGL(k_scene_finalize<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_scene_finalize<R>, dim3(grid_for(ns * 64, 256)), dim3(256), 0, s, d);
   mark(ST_INT);
   return hipGetLastError();
 }

@@ -1020,28 +1020,31 @@ template <typename R> __global__ void k_integrate(Dev<R> d) {
   d.q[b] = V4<R>{qn.s, qn.x, qn.y, qn.z};
 }
 
+// k_scene_reset and k_scene_finalize: one wavefront per scene, lane k on the
+// scene's k-th partial Frank-Wolfe counter (FW_SPREAD is the wavefront width)
+static_assert(FW_SPREAD == 64, "one partial counter per lane");
+
 template <typename R> __global__ void k_scene_reset(Dev<R> d) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = (int)(threadIdx.x & 63);
   if (s >= d.ns) return;
+  unsigned long long* acc = d.fw_acc + ((size_t)d.bs_slot[s] * FW_SPREAD + lane) * FW_STRIDE;
+  acc[0] = 0; acc[1] = 0;
+  if (lane) return;
   SceneState& st = d.ss[d.bs_slot[s]];
-  unsigned long long* acc = d.fw_acc + (size_t)d.bs_slot[s] * FW_SPREAD * FW_STRIDE;
-  for (int k = 0; k < FW_SPREAD; ++k) { acc[k * FW_STRIDE] = 0; acc[k * FW_STRIDE + 1] = 0; }
   st.k_off = 0x7fffffff; st.k_cnt = 0; st.ncolor = 0; st.n_overlaps = 0; st.fw_evals = 0; st.max_pen = 0.0;
   st.solved = 0; st.nat_r = 0.0; st.nat_x = 0.0; st.n_cand = 0; st.sweeps = 0;
   st.post_active = 0; st.post_iters = 0; st.post_exit = 0;
 }
 
 template <typename R> __global__ void k_scene_finalize(Dev<R> d) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = (int)(threadIdx.x & 63);
   if (s >= d.ns || aborted(d.err)) return;
+  const unsigned long long* acc = d.fw_acc + ((size_t)d.bs_slot[s] * FW_SPREAD + lane) * FW_STRIDE;
+  const unsigned long long ev = wave_sum_u64(acc[0]), nc = wave_sum_u64(acc[1]);
+  if (lane) return;
   SceneState& st = d.ss[d.bs_slot[s]];
-  {
-    const unsigned long long* acc = d.fw_acc + (size_t)d.bs_slot[s] * FW_SPREAD * FW_STRIDE;
-    unsigned long long ev = 0, nc = 0;
-    for (int k = 0; k < FW_SPREAD; ++k) { ev += acc[k * FW_STRIDE]; nc += acc[k * FW_STRIDE + 1]; }
-    st.fw_evals = ev;
-    st.n_cand = (int)nc;
-  }
+  st.fw_evals = ev;
+  st.n_cand = (int)nc;
   if (st.k_cnt == 0) st.k_off = 0;
   st.step_index += 1;
   const unsigned long long sw = st.solved ? (unsigned long long)st.sweeps : 0ull;

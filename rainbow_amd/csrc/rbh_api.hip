@@ -167,8 +167,10 @@ struct rbh_ctx {
   DBuf u_T, isl_unit, u_nbody, u_boff, f_loc, f_unit, u_blist, s_loc, s_W;
   DBuf l_cnt, l_base, n_lanes, l_start, l_end, l_ncol, l_bs;
   DBuf s_JL, s_JA, s_JB, xr0, xr1;
-  DBuf perm, s_id, s_n, s_rA, s_rB, s_b, s_r0, x0, x1, errbuf, scan_part;
+  DBuf perm, s_id, s_n, s_rA, s_rB, s_b, s_r0, x0, x1, errbuf, scan_state;
   int cap_pairs = 0, cap_slots = 0, cap_cand = 0, cap_contacts = 0, cap_units = 0;
+  int scan_stride = 0;       // 64-bit words of look-back state per scan of the step (ticket + tile status)
+  int scan_site = 0;         // scans launched so far in the step being enqueued
   int unit_T = 64;          // contacts per colouring unit (a scene's units are windows of this size)
   rbh_tuning tune{};        // rbh_set_tuning (0 fields: defaults)
   // resident solve groups: LDS capacities (contacts, bodies, colour offsets) and the launch's dynamic LDS
@@ -246,7 +248,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->c_lab, &c->uperm, &c->u_cnt, &c->u_base, &c->n_units, &c->u_start, &c->u_end, &c->u_ncol,
           &c->u_bs, &c->body_claim, &c->body_loc, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->pal,
           &c->g_pc, &c->g_done, &c->col_off, &c->merit, &c->col_fill, &c->big_list, &c->item_pair_of, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
-          &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_part, &c->u_T, &c->isl_unit,
+          &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_state, &c->u_T, &c->isl_unit,
           &c->u_nbody, &c->u_boff, &c->f_loc, &c->f_unit, &c->u_blist, &c->s_loc, &c->s_W, &c->l_cnt,
           &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs, &c->s_JL, &c->s_JA, &c->s_JB,
           &c->xr0, &c->xr1, &c->hd0, &c->hd1, &c->h_ids, &c->h_geo, &c->hb_body, &c->hb_inc_off, &c->hb_inc,
@@ -566,7 +568,11 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   return d;
 }
 
-int scan_parts(int cap) { return (cap + SCAN_TILE - 1) / SCAN_TILE + 2; }
+// Look-back state of the step's multi-tile scans: SCAN_SITES regions (one per
+// scan of the launch sequence, which has 11), each a ticket word and one status
+// word per look-back tile of the largest capacity.
+constexpr int SCAN_SITES = 16;
+template <typename T> int scan_tiles(int cap) { return (cap + LbTile<T>::TILE - 1) / LbTile<T>::TILE + 1; }
 
 int size_workspace(rbh_ctx* c) {
   const size_t V = 4 * (size_t)c->prec;
@@ -604,13 +610,13 @@ int size_workspace(rbh_ctx* c) {
       {&c->s_JL, 4 * ck * V}, {&c->s_JA, 4 * ck * V}, {&c->s_JB, 4 * ck * V}, {&c->xr0, 4 * ck * c->prec},
       {&c->xr1, 4 * ck * c->prec},
       {&c->perm, ck * 4}, {&c->s_id, ck * 8}, {&c->s_n, ck * V}, {&c->s_rA, ck * V}, {&c->s_rB, ck * V},
-      {&c->s_b, ck * V}, {&c->s_r0, ck * V}, {&c->x0, ck * V}, {&c->x1, ck * V}, {&c->errbuf, 64},
-      {&c->scan_part, (size_t)scan_parts(maxcap) * 4}};
+      {&c->s_b, ck * V}, {&c->s_r0, ck * V}, {&c->x0, ck * V}, {&c->x1, ck * V}, {&c->errbuf, 64}};
   for (auto& e : el) {
     int rc = ensure(c, *e.b, e.bytes);
     if (rc) return rc;
   }
-  return 0;
+  c->scan_stride = 1 + std::max(scan_tiles<int>(maxcap), scan_tiles<unsigned char>(maxcap));
+  return ensure(c, c->scan_state, (size_t)SCAN_SITES * c->scan_stride * 8);
 }
 
 int build_batch(rbh_ctx* c, const int* ids, int n) {
@@ -687,16 +693,29 @@ int grid_for(long n, int block, int maxg = 8192) {
   return (int)std::max<long>(1, std::min<long>(g, maxg));
 }
 
-template <typename T> void scan(rbh_ctx* c, const T* in, int* out, const int* n_dev, int cap, int* total) {
+// The first kernel of a step: clears the error words and the scans'
+// look-back state (SCAN_SITES regions of scan_stride words).
+__global__ void k_step_clear(int* err, unsigned long long* scan_state, int nwords) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 16) err[i] = 0;
+  for (int k = i; k < nwords; k += gridDim.x * blockDim.x) scan_state[k] = 0;
+}
+
+// Exclusive prefix sum of in[0..n) with n = *n_dev clamped to cap (rbh_scan.h).
+// Each multi-tile scan of the step takes the next region of look-back state;
+// false if the step has more such scans than SCAN_SITES (nothing launched).
+template <typename T> bool scan(rbh_ctx* c, const T* in, int* out, const int* n_dev, int cap, int* total) {
   if (cap < SCAN_SINGLE_TILES * SCAN_TILE) {
     hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, out, total);
-    return;
+    return true;
   }
-  int nb = std::max(1, (cap + SCAN_TILE - 1) / SCAN_TILE + 1);
-  int* part = reinterpret_cast<int*>(c->scan_part.p);
-  hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, part);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, c->stream, part, n_dev, cap, total);
-  hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, part, out);
+  const int site = c->scan_site++;
+  const int tiles = scan_tiles<T>(cap);
+  if (site >= SCAN_SITES || 1 + tiles > c->scan_stride) return false;
+  unsigned long long* st = reinterpret_cast<unsigned long long*>(c->scan_state.p) + (size_t)site * c->scan_stride;
+  hipLaunchKernelGGL(k_scan_lookback<T>, dim3(tiles), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, out, total, st,
+                     reinterpret_cast<int*>(c->errbuf.p));
+  return true;
 }
 
 // steps between re-sizings of a single scene's cooperative solve grid
@@ -733,29 +752,36 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   const long merit_slots = (long)std::max(c->max_iter_batch, 1) * multi_grid;
   auto mark = [&](int st) { if (ev) (void)hipEventRecord(ev[st], s); };
   mark(ST_BEGIN);
-  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, d.err, 16, 0);
+  const int scan_words = SCAN_SITES * c->scan_stride;
+  hipLaunchKernelGGL(k_step_clear, dim3(grid_for(scan_words, 256, 64)), dim3(256), 0, s, d.err,
+                     reinterpret_cast<unsigned long long*>(c->scan_state.p), scan_words);
+  c->scan_site = 0;
+  // a scan that could not be launched (more multi-tile scans than SCAN_SITES)
+  // ends the sequence before anything reads its output
+#define RBH_SCAN(T, ...) \
+  if (!scan<T>(__VA_ARGS__)) return hipErrorInvalidConfiguration
   hipLaunchKernelGGL(k_scene_reset<R>, dim3(grid_for(ns * 64, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_body_prep<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_refit<R>, dim3(nb), dim3(64), 0, s, d);
   mark(ST_PREP);
   // body pairs: count per row, scan, write
   hipLaunchKernelGGL(k_pair_count<R>, dim3(nb), dim3(64), 0, s, d);
-  scan<int>(c, d.row_cnt, d.row_off, d.d_nb, nb, d.n_pairs);
+  RBH_SCAN(int, c, d.row_cnt, d.row_off, d.d_nb, nb, d.n_pairs);
   hipLaunchKernelGGL(k_pair_write<R>, dim3(nb), dim3(64), 0, s, d);
   // candidate slots: (nt_a + nt_b) per pair; work items: triangle blocks of both sides
-  scan<int>(c, d.pair_sz, d.slot_off, d.n_pairs, c->cap_pairs, d.n_slots);
-  scan<int>(c, d.pair_items, d.item_off, d.n_pairs, c->cap_pairs, nullptr);
+  RBH_SCAN(int, c, d.pair_sz, d.slot_off, d.n_pairs, c->cap_pairs, d.n_slots);
+  RBH_SCAN(int, c, d.pair_items, d.item_off, d.n_pairs, c->cap_pairs, nullptr);
   hipLaunchKernelGGL(k_item_map<R>, dim3(grid_for(c->cap_pairs, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_cand_flags<R>, dim3(grid_for(c->cap_pairs, 1, 16384)), dim3(CAND_BLOCK), 0, s, d);
-  scan<int>(c, d.side_cnt, d.side_off, d.n_sides, 2 * c->cap_pairs, d.n_cand);
+  RBH_SCAN(int, c, d.side_cnt, d.side_off, d.n_sides, 2 * c->cap_pairs, d.n_cand);
   hipLaunchKernelGGL(k_cand_write<R>, dim3(grid_for(2L * c->cap_pairs * 64, 256)), dim3(256), 0, s, d);
   mark(ST_BROAD);
   hipLaunchKernelGGL(k_frank_wolfe<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
   mark(ST_FWK);
-  scan<unsigned char>(c, d.cand_keep, d.keep_scan, d.n_cand, c->cap_cand, nullptr);
+  RBH_SCAN(unsigned char, c, d.cand_keep, d.keep_scan, d.n_cand, c->cap_cand, nullptr);
   hipLaunchKernelGGL(k_kept_list<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_dedup<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
-  scan<unsigned char>(c, d.kept_flag, d.kept_scan, d.n_kept, c->cap_cand, nullptr);
+  RBH_SCAN(unsigned char, c, d.kept_flag, d.kept_scan, d.n_kept, c->cap_cand, nullptr);
   hipLaunchKernelGGL(k_contact_write<R>, dim3(grid_for(c->cap_cand, 256)), dim3(256), 0, s, d);
   mark(ST_FW);
   hipLaunchKernelGGL(k_assemble<R>, dim3(grid_for(c->cap_contacts, 256)), dim3(256), 0, s, d);
@@ -765,20 +791,21 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_uf_init<R>, dim3(grid_for(nb + ns + 1, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_uf_union<R>, dim3(G), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_isl_count<R>, dim3(G), dim3(256), 0, s, d);
-  scan<int>(c, d.isl_cnt, d.isl_off, d.d_nb + 1, nb + ns + 1, nullptr);
+  RBH_SCAN(int, c, d.isl_cnt, d.isl_off, d.d_nb + 1, nb + ns + 1, nullptr);
   hipLaunchKernelGGL(k_isl_fill<R>, dim3(G), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_unit_count<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
-  scan<int>(c, d.u_cnt, d.u_base, d.d_nb + 2, ns, d.n_units);
-  scan<int>(c, d.l_cnt, d.l_base, d.d_nb + 2, ns, d.n_lanes);
+  RBH_SCAN(int, c, d.u_cnt, d.u_base, d.d_nb + 2, ns, d.n_units);
+  RBH_SCAN(int, c, d.l_cnt, d.l_base, d.d_nb + 2, ns, d.n_lanes);
   hipLaunchKernelGGL(k_unit_reset<R>, dim3(grid_for(c->cap_units, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_unit_assign<R>, dim3(grid_for(nb + ns, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_csr_count<R>, dim3(G), dim3(256), 0, s, d);
-  scan<int>(c, d.bc_cnt, d.bc_off, d.d_nb, nb, nullptr);
+  RBH_SCAN(int, c, d.bc_cnt, d.bc_off, d.d_nb, nb, nullptr);
   hipLaunchKernelGGL(k_csr_fill<R>, dim3(G), dim3(256), 0, s, d);
   // free bodies -> slots of their unit
   hipLaunchKernelGGL(k_unit_bodies<R>, dim3(grid_for(nb, 256)), dim3(256), 0, s, d);
-  scan<int>(c, d.u_nbody, d.u_boff, d.n_units, c->cap_units, nullptr);
+  RBH_SCAN(int, c, d.u_nbody, d.u_boff, d.n_units, c->cap_units, nullptr);
   hipLaunchKernelGGL(k_unit_blist<R>, dim3(grid_for(nb, 256)), dim3(256), 0, s, d);
+#undef RBH_SCAN
   hipLaunchKernelGGL(k_color_small<R>, dim3(c->cap_units), dim3(64), 0, s, d);
   hipLaunchKernelGGL(k_color_units<R>, dim3(std::min(c->cap_units, c->prop.multiProcessorCount)),
                      dim3(COLOR_UNIT_BLOCK), 0, s, d);
@@ -969,6 +996,7 @@ template <typename R> int run_steps(rbh_ctx* c, int n_steps, double dt, int flag
     if (e[0] & ERR_CAP_CONTACTS) c->cap_contacts = grow(c->cap_contacts, e[4]);
     if (e[0] & ERR_CAP_UNITS) c->cap_contacts = grow(c->cap_contacts, e[5] * c->unit_T);
     if (e[0] & ERR_SOLVER) return fail(c, RBH_E_HIP, "grid barrier of the solver timed out");
+    if (e[0] & ERR_SCAN) return fail(c, RBH_E_HIP, "prefix sum: a tile's wait for its predecessors timed out");
     c->cap_cand = std::max(c->cap_cand, std::min(c->cap_slots, c->cap_cand));
     drop_graph(c);
     c->arena_gen++;

@@ -56,7 +56,8 @@ constexpr int MAX_ROUNDS = 200;
 
 // error flags (dev.err[0] bits); capacity overflows record the needed size
 enum ErrBits { ERR_CAP_PAIRS = 1, ERR_CAP_SLOTS = 2, ERR_CAP_CAND = 4, ERR_CAP_CONTACTS = 8, ERR_COLORS = 16,
-               ERR_SOLVER = 32, ERR_CAP_UNITS = 64 };
+               ERR_SOLVER = 32, ERR_CAP_UNITS = 64, ERR_SCAN = SCAN_ERR_STALL };
+static_assert(ERR_SCAN == 128, "distinct error bit");
 
 template <typename R> struct Dev {
   // shapes

@@ -693,14 +693,6 @@ int grid_for(long n, int block, int maxg = 8192) {
   return (int)std::max<long>(1, std::min<long>(g, maxg));
 }
 
-// The first kernel of a step: clears the error words and the scans'
-// look-back state (SCAN_SITES regions of scan_stride words).
-__global__ void k_step_clear(int* err, unsigned long long* scan_state, int nwords) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 16) err[i] = 0;
-  for (int k = i; k < nwords; k += gridDim.x * blockDim.x) scan_state[k] = 0;
-}
-
 // Exclusive prefix sum of in[0..n) with n = *n_dev clamped to cap (rbh_scan.h).
 // Each multi-tile scan of the step takes the next region of look-back state;
 // false if the step has more such scans than SCAN_SITES (nothing launched).
@@ -753,14 +745,17 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   auto mark = [&](int st) { if (ev) (void)hipEventRecord(ev[st], s); };
   mark(ST_BEGIN);
   const int scan_words = SCAN_SITES * c->scan_stride;
-  hipLaunchKernelGGL(k_step_clear, dim3(grid_for(scan_words, 256, 64)), dim3(256), 0, s, d.err,
-                     reinterpret_cast<unsigned long long*>(c->scan_state.p), scan_words);
+  const int n_merit = multi ? (int)(2 * merit_slots) : 0;
+  const long begin_n =
+      std::max<long>({(long)scan_words, 64L * ns, (long)nb + ns + 1, (long)c->cap_units, (long)n_merit});
+  hipLaunchKernelGGL(k_step_begin<R>, dim3(grid_for(begin_n, 256, 1024)), dim3(256), 0, s, d,
+                     reinterpret_cast<unsigned long long*>(c->scan_state.p), scan_words,
+                     reinterpret_cast<int*>(d.merit), n_merit);
   c->scan_site = 0;
   // a scan that could not be launched (more multi-tile scans than SCAN_SITES)
   // ends the sequence before anything reads its output
 #define RBH_SCAN(T, ...) \
   if (!scan<T>(__VA_ARGS__)) return hipErrorInvalidConfiguration
-  hipLaunchKernelGGL(k_scene_reset<R>, dim3(grid_for(ns * 64, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_body_prep<R>, dim3(grid_for(nb, 256, 1 << 20)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_refit<R>, dim3(nb), dim3(64), 0, s, d);
   mark(ST_PREP);
@@ -788,7 +783,6 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   mark(ST_ASM);
   // islands -> colouring units -> body lists in unit order
   const int G = grid_for(c->cap_contacts, 256);
-  hipLaunchKernelGGL(k_uf_init<R>, dim3(grid_for(nb + ns + 1, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_uf_union<R>, dim3(G), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_isl_count<R>, dim3(G), dim3(256), 0, s, d);
   RBH_SCAN(int, c, d.isl_cnt, d.isl_off, d.d_nb + 1, nb + ns + 1, nullptr);
@@ -796,7 +790,6 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_unit_count<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
   RBH_SCAN(int, c, d.u_cnt, d.u_base, d.d_nb + 2, ns, d.n_units);
   RBH_SCAN(int, c, d.l_cnt, d.l_base, d.d_nb + 2, ns, d.n_lanes);
-  hipLaunchKernelGGL(k_unit_reset<R>, dim3(grid_for(c->cap_units, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_unit_assign<R>, dim3(grid_for(nb + ns, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_csr_count<R>, dim3(G), dim3(256), 0, s, d);
   RBH_SCAN(int, c, d.bc_cnt, d.bc_off, d.d_nb, nb, nullptr);
@@ -812,8 +805,6 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_order_lanes<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
   hipLaunchKernelGGL(k_lane_gather<R>, dim3(c->cap_units, GATHER_SPLIT), dim3(COLOR_BLOCK), 0, s, d);
   if (multi) {
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(2L * merit_slots, 256)), dim3(256), 0, s, reinterpret_cast<int*>(d.merit),
-                       2 * merit_slots, -1);
     mark(ST_COLOR);
     hinge_pass<R>(c, d, 0);
     void* args[] = {&d};

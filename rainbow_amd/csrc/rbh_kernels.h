@@ -336,9 +336,26 @@ template <typename R> __device__ __forceinline__ bool within_reach(V4<R> ra, V4<
 }
 
 template <typename R> __device__ __forceinline__ bool pair_kept(const Dev<R>& d, V4<R> ra, int b, V4<R> alo, V4<R> ahi,
-                                                                R env, bool all) {
-  const V4<R> blo = d.b_lo[b], bhi = d.b_hi[b];
+                                                                V4<R> blo, V4<R> bhi, R env, bool all) {
   return box_overlap(alo, ahi, blo, bhi) && (all || within_reach(ra, d.r[b], alo, ahi, blo, bhi, env));
+}
+
+// A row is one wavefront, which has little else to hide load latency with:
+// it walks its columns PAIR_U chunks of 64 at a time, their boxes loaded
+// together (column j0 + 64 u for lane-relative start j0).  Columns past the
+// row's end read the row's last column instead (end > j0 - lane, so end - 1
+// is a body of the scene), which keeps every load unconditional: no branch,
+// and no wait, between them.
+constexpr int PAIR_U = 4;
+
+template <typename R>
+__device__ __forceinline__ void pair_boxes(const Dev<R>& d, int j0, int end, V4<R> (&blo)[PAIR_U], V4<R> (&bhi)[PAIR_U]) {
+#pragma unroll
+  for (int u = 0; u < PAIR_U; ++u) {
+    const int j = min(j0 + 64 * u, end - 1);
+    blo[u] = d.b_lo[j];
+    bhi[u] = d.b_hi[j];
+  }
 }
 
 template <typename R> __global__ void __launch_bounds__(64) k_pair_count(Dev<R> d) {
@@ -352,7 +369,15 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_count(Dev<R> 
   const bool all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   int cnt = 0;
   const V4<R> ra = d.r[b];
-  for (int j = b + 1 + threadIdx.x; j < end; j += 64) cnt += pair_kept(d, ra, j, alo, ahi, env, all) ? 1 : 0;
+  for (int j0 = b + 1 + (int)threadIdx.x; j0 < end; j0 += 64 * PAIR_U) {
+    V4<R> blo[PAIR_U], bhi[PAIR_U];
+    pair_boxes(d, j0, end, blo, bhi);
+#pragma unroll
+    for (int u = 0; u < PAIR_U; ++u) {
+      const int j = j0 + 64 * u;
+      cnt += (j < end && pair_kept(d, ra, j, alo, ahi, blo[u], bhi[u], env, all)) ? 1 : 0;
+    }
+  }
   cnt = wave_sum(cnt);
   if (threadIdx.x == 0) d.row_cnt[f] = cnt;
 }
@@ -374,20 +399,26 @@ template <typename R> __global__ void __launch_bounds__(64) k_pair_write(Dev<R> 
   const bool all = (d.flags & STEP_COUNT_OVERLAPS) != 0;
   const V4<R> ra = d.r[b];
   const int ntA = d.b_broad[b].y;
-  for (int j0 = b + 1; j0 < end; j0 += 64) {
-    int j = j0 + threadIdx.x;
-    bool hit = j < end && pair_kept(d, ra, j, alo, ahi, env, all);
-    unsigned long long m = __ballot(hit);
-    if (hit) {
-      int pos = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));
-      const int ntB = d.b_broad[j].y;
-      d.pairs[pos] = make_int2(b, j);
-      d.pair_sz[pos] = ntA + ntB;
-      d.pair_items[pos] = n_blocks(ntA) + n_blocks(ntB);
-      d.side_cnt[2 * pos] = 0;
-      d.side_cnt[2 * pos + 1] = 0;
+  for (int j0 = b + 1 + (int)threadIdx.x; j0 - (int)threadIdx.x < end; j0 += 64 * PAIR_U) {
+    V4<R> blo[PAIR_U], bhi[PAIR_U];
+    pair_boxes(d, j0, end, blo, bhi);
+    // chunks in order, lanes in order within a chunk: pairs in column order
+#pragma unroll
+    for (int u = 0; u < PAIR_U; ++u) {
+      const int j = j0 + 64 * u;
+      const bool hit = j < end && pair_kept(d, ra, j, alo, ahi, blo[u], bhi[u], env, all);
+      const unsigned long long m = __ballot(hit);
+      if (hit) {
+        int pos = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));
+        const int ntB = d.b_broad[j].y;
+        d.pairs[pos] = make_int2(b, j);
+        d.pair_sz[pos] = ntA + ntB;
+        d.pair_items[pos] = n_blocks(ntA) + n_blocks(ntB);
+        d.side_cnt[2 * pos] = 0;
+        d.side_cnt[2 * pos + 1] = 0;
+      }
+      base += __popcll(m);
     }
-    base += __popcll(m);
   }
 }
 
@@ -1021,13 +1052,11 @@ template <typename R> __global__ void k_integrate(Dev<R> d) {
   d.q[b] = V4<R>{qn.s, qn.x, qn.y, qn.z};
 }
 
-// k_scene_reset and k_scene_finalize: one wavefront per scene, lane k on the
+// k_step_begin and k_scene_finalize: one wavefront per scene, lane k on the
 // scene's k-th partial Frank-Wolfe counter (FW_SPREAD is the wavefront width)
 static_assert(FW_SPREAD == 64, "one partial counter per lane");
 
-template <typename R> __global__ void k_scene_reset(Dev<R> d) {
-  const int s = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = (int)(threadIdx.x & 63);
-  if (s >= d.ns) return;
+template <typename R> __device__ __forceinline__ void scene_reset(const Dev<R>& d, int s, int lane) {
   unsigned long long* acc = d.fw_acc + ((size_t)d.bs_slot[s] * FW_SPREAD + lane) * FW_STRIDE;
   acc[0] = 0; acc[1] = 0;
   if (lane) return;
@@ -1035,6 +1064,25 @@ template <typename R> __global__ void k_scene_reset(Dev<R> d) {
   st.k_off = 0x7fffffff; st.k_cnt = 0; st.ncolor = 0; st.n_overlaps = 0; st.fw_evals = 0; st.max_pen = 0.0;
   st.solved = 0; st.nat_r = 0.0; st.nat_x = 0.0; st.n_cand = 0; st.sweeps = 0;
   st.post_active = 0; st.post_iters = 0; st.post_exit = 0;
+}
+
+// The step's first kernel: clears the error words, the prefix sums'
+// look-back state (scan_words words) and the cooperative solve's merit slots
+// (n_merit words, none for a batch), resets every scene's counters, and
+// resets the island and colouring-unit arrays (union-find parents, island and
+// body-contact counters, unit and sweep-lane ranges), which the kernels before
+// the island pass do not touch.  Grid-stride over the longest of these.
+template <typename R>
+__global__ void k_step_begin(Dev<R> d, unsigned long long* scan_state, int scan_words, int* merit_words, int n_merit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, G = gridDim.x * blockDim.x;
+  if (i < 16) d.err[i] = 0;
+  for (int k = i; k < scan_words; k += G) scan_state[k] = 0;
+  for (int k = i; k < n_merit; k += G) merit_words[k] = -1;   // the cooperative solve's merit slots: empty
+  for (int t = i; t < d.ns * 64; t += G) scene_reset(d, t >> 6, t & 63);
+  for (int f = i; f < d.nb; f += G) uf_reset(d, f);
+  for (int l = i; l <= d.nb + d.ns; l += G) isl_reset(d, l);
+  if (i == 0) d.big_list[0] = 0;
+  for (int u = i; u < d.cap_units; u += G) unit_reset(d, u);
 }
 
 template <typename R> __global__ void k_scene_finalize(Dev<R> d) {

@@ -36,17 +36,18 @@ __device__ __forceinline__ void uf_union(int* uf, int a, int b) {
   }
 }
 
-template <typename R> __global__ void k_uf_init(Dev<R> d) {
-  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < d.nb; f += gridDim.x * blockDim.x) {
-    d.uf[f] = f;
-    d.bc_cnt[f] = 0;
-    d.bc_fill[f] = 0;
-    d.body_claim[f] = -1;
-  }
-  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l <= d.nb + d.ns; l += gridDim.x * blockDim.x) {
-    d.isl_cnt[l] = 0;
-    d.isl_fill[l] = 0;
-  }
+// reset at the start of the step (k_step_begin): body f's union-find parent
+// and contact counters, island label l's counters, unit / sweep lane u
+template <typename R> __device__ __forceinline__ void uf_reset(const Dev<R>& d, int f) {
+  d.uf[f] = f;
+  d.bc_cnt[f] = 0;
+  d.bc_fill[f] = 0;
+  d.body_claim[f] = -1;
+}
+
+template <typename R> __device__ __forceinline__ void isl_reset(const Dev<R>& d, int l) {
+  d.isl_cnt[l] = 0;
+  d.isl_fill[l] = 0;
 }
 
 template <typename R> __global__ void k_uf_union(Dev<R> d) {
@@ -112,18 +113,15 @@ template <typename R> __global__ void k_unit_count(Dev<R> d) {
   }
 }
 
-template <typename R> __global__ void k_unit_reset(Dev<R> d) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.big_list[0] = 0;
-  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < d.cap_units; u += gridDim.x * blockDim.x) {
-    d.u_start[u] = 0x7fffffff;
-    d.u_end[u] = 0;
-    d.u_ncol[u] = 0;
-    d.u_bs[u] = -1;
-    d.u_nbody[u] = 0;
-    d.l_start[u] = 0x7fffffff;
-    d.l_end[u] = 0;
-    d.l_bs[u] = -1;
-  }
+template <typename R> __device__ __forceinline__ void unit_reset(const Dev<R>& d, int u) {
+  d.u_start[u] = 0x7fffffff;
+  d.u_end[u] = 0;
+  d.u_ncol[u] = 0;
+  d.u_bs[u] = -1;
+  d.u_nbody[u] = 0;
+  d.l_start[u] = 0x7fffffff;
+  d.l_end[u] = 0;
+  d.l_bs[u] = -1;
 }
 
 template <typename R> __global__ void k_unit_assign(Dev<R> d) {

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/rbh.h"
+#include "../../include/rbh_testing.h"
 #include "rbh_kernels.h"
 #include "rbh_sdf.h"
 
@@ -693,20 +694,31 @@ int grid_for(long n, int block, int maxg = 8192) {
   return (int)std::max<long>(1, std::min<long>(g, maxg));
 }
 
-// Exclusive prefix sum of in[0..n) with n = *n_dev clamped to cap (rbh_scan.h).
-// Each multi-tile scan of the step takes the next region of look-back state;
-// false if the step has more such scans than SCAN_SITES (nothing launched).
-template <typename T> bool scan(rbh_ctx* c, const T* in, int* out, const int* n_dev, int cap, int* total) {
+// Exclusive prefix sum of in[0..n) with n = *n_dev clamped to cap (rbh_scan.h):
+// one workgroup for a short capacity, else the look-back scan on `state`
+// (1 + scan_tiles<T>(cap) words, zero before the launch) with stalls flagged in err[0].
+template <typename T>
+void launch_scan(hipStream_t st, const T* in, int* out, const int* n_dev, int cap, int* total,
+                 unsigned long long* state, int* err) {
   if (cap < SCAN_SINGLE_TILES * SCAN_TILE) {
-    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, out, total);
-    return true;
+    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(SCAN_BLOCK), 0, st, in, n_dev, cap, out, total);
+    return;
   }
-  const int site = c->scan_site++;
-  const int tiles = scan_tiles<T>(cap);
-  if (site >= SCAN_SITES || 1 + tiles > c->scan_stride) return false;
-  unsigned long long* st = reinterpret_cast<unsigned long long*>(c->scan_state.p) + (size_t)site * c->scan_stride;
-  hipLaunchKernelGGL(k_scan_lookback<T>, dim3(tiles), dim3(SCAN_BLOCK), 0, c->stream, in, n_dev, cap, out, total, st,
-                     reinterpret_cast<int*>(c->errbuf.p));
+  hipLaunchKernelGGL(k_scan_lookback<T>, dim3(scan_tiles<T>(cap)), dim3(SCAN_BLOCK), 0, st, in, n_dev, cap, out, total,
+                     state, err);
+}
+
+// A scan of the step.  Each multi-tile scan takes the next region of the
+// step's look-back state; false if the step has more such scans than
+// SCAN_SITES (nothing launched).
+template <typename T> bool scan(rbh_ctx* c, const T* in, int* out, const int* n_dev, int cap, int* total) {
+  unsigned long long* st = nullptr;
+  if (cap >= SCAN_SINGLE_TILES * SCAN_TILE) {
+    const int site = c->scan_site++;
+    if (site >= SCAN_SITES || 1 + scan_tiles<T>(cap) > c->scan_stride) return false;
+    st = reinterpret_cast<unsigned long long*>(c->scan_state.p) + (size_t)site * c->scan_stride;
+  }
+  launch_scan<T>(c->stream, in, out, n_dev, cap, total, st, reinterpret_cast<int*>(c->errbuf.p));
   return true;
 }
 
@@ -1223,6 +1235,44 @@ int rbh_device_query(rbh_ctx* c, rbh_device_info* out) {
 int rbh_memory_usage(rbh_ctx* c, unsigned long long* bytes) {
   if (!c || !bytes) return RBH_E_ARG;
   *bytes = total_bytes(c);
+  return 0;
+}
+
+int rbh_prefix_sum(rbh_ctx* c, const void* in, int elem_bytes, int n, int cap, int* out, int* total) {
+  if (!c || !in || !out || !total) return fail(c, RBH_E_ARG, "prefix sum: null argument");
+  if (elem_bytes != 1 && elem_bytes != 4) return fail(c, RBH_E_ARG, "prefix sum: elements are 1 or 4 bytes");
+  if (n < 0 || cap < 1 || n > cap || cap > (1 << 28)) return fail(c, RBH_E_ARG, "prefix sum: need 0 <= n <= cap <= 2^28");
+  (void)hipSetDevice(c->device);
+  const int tiles = elem_bytes == 1 ? scan_tiles<unsigned char>(cap) : scan_tiles<int>(cap);
+  const size_t in_bytes = (size_t)cap * elem_bytes + 64, out_bytes = ((size_t)cap + 1) * 4;
+  const size_t st_bytes = ((size_t)tiles + 1) * 8;
+  // one allocation: input, output, state, then n, total and the error words
+  const size_t o_out = (in_bytes + 255) & ~(size_t)255, o_st = o_out + ((out_bytes + 255) & ~(size_t)255),
+               o_ints = o_st + ((st_bytes + 255) & ~(size_t)255), bytes = o_ints + 256;
+  unsigned char* buf = nullptr;
+  HIPCHK(c, hipMalloc(&buf, bytes));
+  int* ints = reinterpret_cast<int*>(buf + o_ints);   // [0] n, [1] total, [16..31] error words
+  hipError_t e = hipMemsetAsync(buf + o_st, 0, o_ints + 256 - o_st, c->stream);
+  if (e == hipSuccess && n > 0) e = hipMemcpyAsync(buf, in, (size_t)n * elem_bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ints, &n, 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    unsigned long long* st = reinterpret_cast<unsigned long long*>(buf + o_st);
+    int* dout = reinterpret_cast<int*>(buf + o_out);
+    if (elem_bytes == 1)
+      launch_scan<unsigned char>(c->stream, buf, dout, ints, cap, ints + 1, st, ints + 16);
+    else
+      launch_scan<int>(c->stream, reinterpret_cast<const int*>(buf), dout, ints, cap, ints + 1, st, ints + 16);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, c->stream);
+  }
+  int tail[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(tail, ints + 1, 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(tail + 1, ints + 16, 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(c, RBH_E_HIP, std::string("prefix sum: ") + hipGetErrorString(e));
+  if (tail[1] & ERR_SCAN) return fail(c, RBH_E_HIP, "prefix sum: a tile's wait for its predecessors timed out");
+  *total = tail[0];
   return 0;
 }
 

@@ -83,7 +83,7 @@ def close_all():
 
 atexit.register(close_all)
 
-# (name, argtypes) of every entry point declared in include/rbh.h
+# (name, argtypes) of every entry point declared in include/rbh.h and include/rbh_testing.h
 _PROTOS = {
     "rbh_version": [],
     "rbh_ctx_create": [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)],
@@ -116,6 +116,8 @@ _PROTOS = {
     "rbh_scene_totals": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int],
     "rbh_set_tuning": [C.c_void_p, C.c_void_p],
     "rbh_get_tuning": [C.c_void_p, C.c_void_p],
+    # include/rbh_testing.h
+    "rbh_prefix_sum": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)],
 }
 
 
@@ -214,6 +216,20 @@ class Context:
         self._check(self.lib.rbh_sdf_build(self.h, _ptr(V), len(V), _ptr(T), len(T), _ptr(_f64(min_coord, 3)),
                                            _ptr(_f64(max_coord, 3)), int(I), int(J), int(K), _ptr(out)))
         return out
+
+    def prefix_sum(self, values, cap=None):
+        """The step's exclusive prefix sum of a uint8 or int32 array (a test hook,
+        include/rbh_testing.h): returns (out[0..n], total), the launch sized for cap."""
+        a = np.ascontiguousarray(values)
+        if a.dtype not in (np.uint8, np.int32):
+            raise TypeError("prefix_sum takes uint8 or int32 values")
+        n = int(a.size)
+        cap = max(n, 1) if cap is None else int(cap)
+        out = np.empty(n + 1, dtype=np.int32)
+        tot = C.c_int()
+        self._check(self.lib.rbh_prefix_sum(self.h, a.ctypes.data_as(C.c_void_p), a.dtype.itemsize, n, cap,
+                                            out.ctypes.data_as(C.c_void_p), C.byref(tot)))
+        return out, tot.value
 
     def sdf_build_noalloc(self, V, T, min_coord, max_coord, I, J, K):
         """sdf_build staged in the library's fixed per-GPU array (no device allocation)."""

@@ -1,4 +1,4 @@
-"""The C ABI: librbh.so loads and exports every entry point include/rbh.h declares."""
+"""The C ABI: librbh.so loads and exports every entry point include/*.h declares."""
 import os
 import re
 
@@ -10,9 +10,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    txt = open(os.path.join(ROOT, "include", "rbh.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(rbh_\w+)\s*\(", txt)))
+    syms = set()
+    for name in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if name.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", name)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            syms.update(re.findall(r"\b(rbh_\w+)\s*\(", txt))
+    return sorted(syms)
 
 
 def test_header_declares_entry_points():

@@ -163,7 +163,7 @@ struct rbh_ctx {
   DBuf kidx, kpos, n_kept, kept_flag, kept_scan;
   DBuf c_ia, c_ib, c_tri, c_pair, c_p, c_n, n_contacts, c_rA, c_rB, c_b, c_r0, c_mu, c_imp, c_color;
   DBuf f_bs, uf, isl_cnt, isl_off, isl_fill, isl_max, c_lab, uperm, u_cnt, u_base, n_units, u_start, u_end, u_ncol,
-      u_bs, body_claim, body_loc, bc_cnt, bc_off, bc_fill, bc_list, pal, g_pc, g_done, col_off, merit, col_fill,
+      u_bs, body_claim, body_loc, bc_cnt, bc_off, bc_fill, bc_list, pal, g_pc, g_done, col_off, merit, l_pos,
       big_list, item_pair_of;
   DBuf u_T, isl_unit, u_nbody, u_boff, f_loc, f_unit, u_blist, s_loc, s_W;
   DBuf l_cnt, l_base, n_lanes, l_start, l_end, l_ncol, l_bs;
@@ -248,7 +248,7 @@ std::vector<DBuf*> all_bufs(rbh_ctx* c) {
           &c->c_imp, &c->c_color, &c->f_bs, &c->uf, &c->isl_cnt, &c->isl_off, &c->isl_fill, &c->isl_max,
           &c->c_lab, &c->uperm, &c->u_cnt, &c->u_base, &c->n_units, &c->u_start, &c->u_end, &c->u_ncol,
           &c->u_bs, &c->body_claim, &c->body_loc, &c->bc_cnt, &c->bc_off, &c->bc_fill, &c->bc_list, &c->pal,
-          &c->g_pc, &c->g_done, &c->col_off, &c->merit, &c->col_fill, &c->big_list, &c->item_pair_of, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
+          &c->g_pc, &c->g_done, &c->col_off, &c->merit, &c->l_pos, &c->big_list, &c->item_pair_of, &c->perm, &c->s_id, &c->s_n, &c->s_rA,
           &c->s_rB, &c->s_b, &c->s_r0, &c->x0, &c->x1, &c->errbuf, &c->scan_state, &c->u_T, &c->isl_unit,
           &c->u_nbody, &c->u_boff, &c->f_loc, &c->f_unit, &c->u_blist, &c->s_loc, &c->s_W, &c->l_cnt,
           &c->l_base, &c->n_lanes, &c->l_start, &c->l_end, &c->l_ncol, &c->l_bs, &c->s_JL, &c->s_JA, &c->s_JB,
@@ -552,7 +552,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
   d.bc_cnt = P(int*, bc_cnt); d.bc_off = P(int*, bc_off); d.bc_fill = P(int*, bc_fill);
   d.bc_list = P(int*, bc_list); d.pal = P(unsigned*, pal); d.g_pc = P(int*, g_pc);
   d.g_done = P(unsigned char*, g_done); d.col_off = P(int*, col_off); d.merit = P(unsigned long long*, merit);
-  d.col_fill = P(int*, col_fill); d.big_list = P(int*, big_list);
+  d.l_pos = P(int*, l_pos); d.big_list = P(int*, big_list);
   d.item_pair_of = P(int*, item_pair_of); d.cap_items = cap_items(c);
   d.perm = P(int*, perm); d.s_id = P(int2*, s_id); d.s_n = P(V4<R>*, s_n); d.s_rA = P(V4<R>*, s_rA);
   d.s_rB = P(V4<R>*, s_rB); d.s_b = P(V4<R>*, s_b); d.s_r0 = P(V4<R>*, s_r0); d.x0 = P(V4<R>*, x0);
@@ -602,7 +602,7 @@ int size_workspace(rbh_ctx* c) {
       {&c->body_loc, nb * 8}, {&c->bc_cnt, nb * 4}, {&c->bc_off, (nb + 1) * 4}, {&c->bc_fill, nb * 4},
       {&c->bc_list, 2 * ck * 4}, {&c->pal, ck * PALW * 4}, {&c->g_pc, ck * 4}, {&c->g_done, ck + 16},
       {&c->col_off, (2 * (ck + nu) + 16) * 4}, {&c->merit, (size_t)std::max(c->max_iter_batch, 1) * 8},
-      {&c->col_fill, (2 * (ck + nu) + 16) * 4}, {&c->big_list, (nu + 16) * 4},
+      {&c->l_pos, ck * 4}, {&c->big_list, (nu + 16) * 4},
       {&c->item_pair_of, (size_t)cap_items(c) * 4},
       {&c->u_T, ns * 4}, {&c->isl_unit, nl * 4}, {&c->u_nbody, nu * 4}, {&c->u_boff, (nu + 1) * 4},
       {&c->f_loc, nb * 4}, {&c->f_unit, nb * 4}, {&c->u_blist, nb * 4}, {&c->s_loc, ck * 8},
@@ -811,11 +811,11 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   RBH_SCAN(int, c, d.u_nbody, d.u_boff, d.n_units, c->cap_units, nullptr);
   hipLaunchKernelGGL(k_unit_blist<R>, dim3(grid_for(nb, 256)), dim3(256), 0, s, d);
 #undef RBH_SCAN
-  hipLaunchKernelGGL(k_color_small<R>, dim3(c->cap_units), dim3(64), 0, s, d);
+  hipLaunchKernelGGL(k_color_small<R>, dim3(std::min(c->cap_units, 8192)), dim3(64), 0, s, d);
   hipLaunchKernelGGL(k_color_units<R>, dim3(std::min(c->cap_units, c->prop.multiProcessorCount)),
                      dim3(COLOR_UNIT_BLOCK), 0, s, d);
-  hipLaunchKernelGGL(k_order_lanes<R>, dim3(c->cap_units), dim3(COLOR_BLOCK), 0, s, d);
-  hipLaunchKernelGGL(k_lane_gather<R>, dim3(c->cap_units, GATHER_SPLIT), dim3(COLOR_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_order_lanes<R>, dim3(std::min(c->cap_units, 4096)), dim3(COLOR_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_lane_gather<R>, dim3(std::min(c->cap_units, 2048), GATHER_SPLIT), dim3(COLOR_BLOCK), 0, s, d);
   if (multi) {
     mark(ST_COLOR);
     hinge_pass<R>(c, d, 0);

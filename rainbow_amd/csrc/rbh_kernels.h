@@ -126,7 +126,7 @@ template <typename R> struct Dev {
   unsigned* pal;                       // PALW words per unit-order position
   int* g_pc; unsigned char* g_done;    // colouring state of units too large for LDS
   int* col_off;                        // colour offsets of lane l at 2 * (l_start[l] + l)
-  int* col_fill;                       // the next free position of each colour of a lane (same layout)
+  int* l_pos;                          // per unit-order position p (contact uperm[p]): its position in colour order within its lane
   int items_nb;                        // a batch scene of more bodies than this solves from the per-contact
                                        // items (s_n .. s_W, x0, x1); -1: any group may
   int* big_list;                       // [0]: count, then the units left to k_color_units

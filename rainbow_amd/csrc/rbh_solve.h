@@ -959,15 +959,20 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
 // units of up to SMALL_UNIT contacts: one wavefront each, through per-body
 // bitmaps; larger units, and those outgrowing the bitmaps, are listed in
 // big_list for k_color_units
+// (grids of the per-unit and per-lane kernels are capped, and a workgroup
+// walks units or lanes in turn: a grid of cap_units workgroups, almost all of
+// them without work, costs ~1 ns per workgroup to dispatch)
 template <typename R> __global__ void __launch_bounds__(64) k_color_small(Dev<R> d) {
   __shared__ BitsColorLds s;
   if (aborted(d.err)) return;
-  const int u = blockIdx.x;
-  if (u >= *d.n_units) return;
-  const int K = d.u_end[u] - d.u_start[u];
-  if (K <= 0) return;
-  if ((K > SMALL_UNIT || !color_small_bits<R>(d, u, s)) && threadIdx.x == 0)
-    d.big_list[1 + atomicAdd(&d.big_list[0], 1)] = u;
+  const int nu = *d.n_units;
+  for (int u = blockIdx.x; u < nu; u += gridDim.x) {
+    const int K = d.u_end[u] - d.u_start[u];
+    if (K <= 0) continue;
+    if ((K > SMALL_UNIT || !color_small_bits<R>(d, u, s)) && threadIdx.x == 0)
+      d.big_list[1 + atomicAdd(&d.big_list[0], 1)] = u;
+    __syncthreads();   // s is reused by the next unit
+  }
 }
 
 // the listed units: a workgroup of COLOR_UNIT_BLOCK threads each (a grid of
@@ -1114,12 +1119,7 @@ template <typename R> __device__ __forceinline__ const int* lane_coff(const Dev<
 // result), or for the reference's serial Gauss-Seidel by contact index
 // (contacts of different islands commute, so the order inside each island
 // is all that matters, and that is the contact order).
-template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order_lanes(Dev<R> d) {
-  __shared__ int s_cnt[PALMAX + 1];
-  __shared__ int s_nc;
-  if (aborted(d.err)) return;
-  const int l = blockIdx.x;
-  if (l >= *d.n_lanes) return;
+template <typename R> __device__ __forceinline__ void order_lane(const Dev<R>& d, int l, int* s_cnt, int& s_nc) {
   const int p0 = d.l_start[l], p1 = d.l_end[l];
   if (p1 <= p0) return;
   const int K = p1 - p0;
@@ -1146,32 +1146,44 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order_lan
   for (int i = threadIdx.x; i < K; i += COLOR_BLOCK) atomicAdd(&s_cnt[d.c_color[d.uperm[p0 + i]]], 1);
   __syncthreads();
   if (threadIdx.x == 0) {
-    int* cfill = d.col_fill + 2 * ((size_t)p0 + l);
     int acc = 0;
     for (int c = 0; c < nc; ++c) {
       const int v = s_cnt[c];
       coff[c] = acc;
-      cfill[c] = acc;
+      s_cnt[c] = acc;     // from here on: the next free position of colour c
       acc += v;
     }
     coff[nc] = acc;
     d.l_ncol[l] = nc;
   }
+  __syncthreads();
+  // each contact's position in colour order, taken from LDS counters (the
+  // order inside a colour changes no result: its contacts share no free body)
+  for (int i = threadIdx.x; i < K; i += COLOR_BLOCK)
+    d.l_pos[p0 + i] = atomicAdd(&s_cnt[d.c_color[d.uperm[p0 + i]]], 1);
 }
 
-// Contacts into colour order inside their lane (k_order_lanes set the
-// colours' offsets): GATHER_SPLIT workgroups per lane, each taking every
-// GATHER_SPLIT-th run of COLOR_BLOCK positions; a contact takes the next free
-// position of its colour (the order inside a colour changes no result: its
-// contacts share no free body).
-constexpr int GATHER_SPLIT = 8;
-template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_lane_gather(Dev<R> d) {
+template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_order_lanes(Dev<R> d) {
+  __shared__ int s_cnt[PALMAX + 1];
+  __shared__ int s_nc;
   if (aborted(d.err)) return;
-  const int l = blockIdx.x;
-  if (l >= *d.n_lanes) return;
+  const int nl = *d.n_lanes;
+  for (int l = blockIdx.x; l < nl; l += gridDim.x) {
+    order_lane(d, l, s_cnt, s_nc);
+    __syncthreads();   // s_cnt and s_nc are reused by the next lane
+  }
+}
+
+// Contacts into colour order inside their lane, at the positions
+// k_order_lanes gave them: GATHER_SPLIT workgroups per lane, each taking every
+// GATHER_SPLIT-th run of COLOR_BLOCK positions.  (The positions used to be
+// taken here with a global atomic per contact on the lane's per-colour
+// counters, which sit in one or two cache lines: the atomics of a lane
+// serialised in one L2 channel.)
+constexpr int GATHER_SPLIT = 8;
+template <typename R> __device__ void gather_lane(const Dev<R>& d, int l) {
   const int p0 = d.l_start[l], p1 = d.l_end[l];
   if (p1 <= p0 || d.sp[d.bs_slot[d.l_bs[l]]].scheme == SCHEME_GS) return;   // (gathered by k_order_lanes)
-  int* cfill = d.col_fill + 2 * ((size_t)p0 + l);
   // the items only where the global layout may read them: a group too large
   // for the velocity-only layout, or the post-stabilisation rows
   const int bs = d.l_bs[l];
@@ -1181,9 +1193,14 @@ template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_lane_gath
     items = d.u_boff[u0 + d.u_cnt[bs]] - d.u_boff[u0] > d.items_nb;
   }
   for (int i = blockIdx.y * COLOR_BLOCK + threadIdx.x; i < p1 - p0; i += GATHER_SPLIT * COLOR_BLOCK) {
-    const int k = d.uperm[p0 + i];
-    gather_contact(d, p0 + atomicAdd(&cfill[d.c_color[k]], 1), k, items);
+    gather_contact(d, p0 + d.l_pos[p0 + i], d.uperm[p0 + i], items);
   }
+}
+
+template <typename R> __global__ void __launch_bounds__(COLOR_BLOCK) k_lane_gather(Dev<R> d) {
+  if (aborted(d.err)) return;
+  const int nl = *d.n_lanes;
+  for (int l = blockIdx.x; l < nl; l += gridDim.x) gather_lane(d, l);
 }
 
 // ------------------------------------------------------------ solve

@@ -71,22 +71,54 @@ template <typename R> __device__ __forceinline__ int contact_label(const Dev<R>&
   return d.bs_foff[bs + 1] + bs;
 }
 
+// ctr[key] += 1 for every active lane of the wavefront; returns the lane's
+// old value (its slot).  The lanes that share the first active lane's key
+// (on a batch scene of one island: the whole wavefront) take one atomic
+// together and ranks among themselves; the others add on their own.  An
+// island's contacts counted or placed one atomic each serialised in one L2
+// channel.  (Slots within one key come in no fixed order; the callers'
+// results do not depend on it.)
+__device__ __forceinline__ int wave_add_one(int* ctr, int key, bool active) {
+  const int lane = (int)(threadIdx.x & 63);
+  const unsigned long long act = __ballot(active);
+  if (!act) return 0;
+  const int first = __ffsll((unsigned long long)act) - 1;
+  const int k0 = __shfl(key, first, 64);
+  const unsigned long long m = __ballot(active && key == k0);
+  int base = 0;
+  if (lane == first) base = atomicAdd(&ctr[k0], __popcll(m));
+  base = __shfl(base, first, 64);
+  if ((m >> lane) & 1ull) return base + __popcll(m & ((1ull << lane) - 1ull));
+  return active ? atomicAdd(&ctr[key], 1) : 0;
+}
+
+// one contact per lane; the loop runs while any lane of the wavefront has one
 template <typename R> __global__ void k_isl_count(Dev<R> d) {
   if (aborted(d.err)) return;
   const int nk = *d.n_contacts;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
-    int lab = contact_label(d, k);
-    d.c_lab[k] = lab;
-    atomicAdd(&d.isl_cnt[lab], 1);
+  const int lane = (int)(threadIdx.x & 63), stride = (int)(gridDim.x * blockDim.x);
+  for (int k0 = (int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); k0 < nk; k0 += stride) {
+    const int k = k0 + lane;
+    const bool act = k < nk;
+    int lab = 0;
+    if (act) {
+      lab = contact_label(d, k);
+      d.c_lab[k] = lab;
+    }
+    (void)wave_add_one(d.isl_cnt, lab, act);
   }
 }
 
 template <typename R> __global__ void k_isl_fill(Dev<R> d) {
   if (aborted(d.err)) return;
   const int nk = *d.n_contacts;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nk; k += gridDim.x * blockDim.x) {
-    int lab = d.c_lab[k];
-    d.uperm[d.isl_off[lab] + atomicAdd(&d.isl_fill[lab], 1)] = k;
+  const int lane = (int)(threadIdx.x & 63), stride = (int)(gridDim.x * blockDim.x);
+  for (int k0 = (int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); k0 < nk; k0 += stride) {
+    const int k = k0 + lane;
+    const bool act = k < nk;
+    const int lab = act ? d.c_lab[k] : 0;
+    const int slot = wave_add_one(d.isl_fill, lab, act);
+    if (act) d.uperm[d.isl_off[lab] + slot] = k;
   }
 }
 

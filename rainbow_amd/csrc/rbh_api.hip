@@ -799,9 +799,13 @@ template <typename R> hipError_t launch_step(rbh_ctx* c, double dt, int flags, h
   hipLaunchKernelGGL(k_isl_count<R>, dim3(G), dim3(256), 0, s, d);
   RBH_SCAN(int, c, d.isl_cnt, d.isl_off, d.d_nb + 1, nb + ns + 1, nullptr);
   hipLaunchKernelGGL(k_isl_fill<R>, dim3(G), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(k_unit_count<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
-  RBH_SCAN(int, c, d.u_cnt, d.u_base, d.d_nb + 2, ns, d.n_units);
-  RBH_SCAN(int, c, d.l_cnt, d.l_base, d.d_nb + 2, ns, d.n_lanes);
+  if (ns <= UNIT_SCAN_MAX) {
+    hipLaunchKernelGGL(k_unit_count_scan<R>, dim3(1), dim3(SCAN_BLOCK), 0, s, d);
+  } else {
+    hipLaunchKernelGGL(k_unit_count<R>, dim3(grid_for(ns, 256)), dim3(256), 0, s, d);
+    RBH_SCAN(int, c, d.u_cnt, d.u_base, d.d_nb + 2, ns, d.n_units);
+    RBH_SCAN(int, c, d.l_cnt, d.l_base, d.d_nb + 2, ns, d.n_lanes);
+  }
   hipLaunchKernelGGL(k_unit_assign<R>, dim3(grid_for(nb + ns, 256)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_csr_count<R>, dim3(G), dim3(256), 0, s, d);
   RBH_SCAN(int, c, d.bc_cnt, d.bc_off, d.d_nb, nb, nullptr);

@@ -145,6 +145,48 @@ template <typename R> __global__ void k_unit_count(Dev<R> d) {
   }
 }
 
+// k_unit_count and the exclusive sums of the scenes' unit and lane counts
+// (u_base, l_base with the totals n_units, n_lanes) in one workgroup, one
+// launch instead of three, for batches of up to UNIT_SCAN_MAX scenes.
+constexpr int UNIT_SCAN_MAX = 8192;
+
+template <typename R> __global__ void __launch_bounds__(SCAN_BLOCK) k_unit_count_scan(Dev<R> d) {
+  __shared__ int sh[SCAN_BLOCK / 64];
+  const bool ab = aborted(d.err);
+  int cu = 0, cl = 0;
+  // base <= ns: the tile that holds u_base[ns] and l_base[ns] (the totals) runs
+  for (int base = 0; base <= d.ns; base += SCAN_BLOCK) {
+    const int bs = base + (int)threadIdx.x;
+    int uc = 0, lc = 0;
+    if (bs < d.ns) {
+      int p0, p1;
+      scene_pos_range(d, bs, p0, p1);
+      const int K = ab ? 0 : p1 - p0;
+      const int T = d.unit_T;
+      int TL = T;
+      if (d.unit_split > 0) TL = max(T, (K + d.unit_split - 1) / d.unit_split);
+      uc = (K + T - 1) / T;
+      lc = (K + TL - 1) / TL;
+      d.u_cnt[bs] = uc;
+      d.u_T[bs] = TL;
+      d.l_cnt[bs] = lc;
+    }
+    int tu, tl;
+    const int eu = block_exclusive_scan(uc, sh, tu);
+    const int el = block_exclusive_scan(lc, sh, tl);
+    if (bs <= d.ns) {
+      d.u_base[bs] = cu + eu;
+      d.l_base[bs] = cl + el;
+    }
+    cu += tu;
+    cl += tl;
+  }
+  if (threadIdx.x == 0) {
+    *d.n_units = cu;
+    *d.n_lanes = cl;
+  }
+}
+
 template <typename R> __device__ __forceinline__ void unit_reset(const Dev<R>& d, int u) {
   d.u_start[u] = 0x7fffffff;
   d.u_end[u] = 0;

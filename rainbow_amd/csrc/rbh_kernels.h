@@ -572,35 +572,50 @@ __device__ int cand_block(const Dev<R>& d, int X, int Y, int4 gX, int4 gY, int s
         ybase += 64;
       }
       wave_sync_lds();
+      // The lane's positions still to test against the staged run, all of
+      // them per staged box: one loop over the staged clusters and their
+      // triangles, uniform over the wavefront (every lane reads the same LDS
+      // entry, a broadcast).  A cluster is skipped when no position of the
+      // wavefront meets its box.  (A loop per position, each walking the
+      // clusters its own box meets, diverged across the lanes and repeated
+      // the staged reads for each of the four positions.)
+      for (int k = 0; k < ncl; ++k) {
+        const unsigned want = count_all ? live : (live & ~flg);
+        if (!__ballot(want != 0u)) break;                 // every position decided
+        const V4<R> cl = clo_s[k], ch = chi_s[k];
+        unsigned mk = 0;                                  // positions that meet the cluster box
 #pragma unroll
-      for (int q = 0; q < XQ; ++q) {
-        if (!((live >> q) & 1u) || (((flg >> q) & 1u) && !count_all)) continue;
-        const V4<R> lo{xl[q].x, xl[q].y, xl[q].z, R(0)}, hi{xh[q].x, xh[q].y, xh[q].z, R(0)};
-        bool hitq = false;
-        for (int k = 0; k < ncl && !(hitq && !count_all); ++k) {
-          if (!box_overlap(lo, hi, clo_s[k], chi_s[k])) continue;
-          const int2 rg = crange_s[k];
-          const int end = rg.x + rg.y;
-          // four staged boxes per pass, loaded together (indexes clamped into
-          // the cluster's run; a clamped repeat is not tested)
-          for (int e = rg.x; e < end; e += 4) {
-            V4<R> sl[4], sh[4];
+        for (int q = 0; q < XQ; ++q) {
+          const V4<R> lo{xl[q].x, xl[q].y, xl[q].z, R(0)}, hi{xh[q].x, xh[q].y, xh[q].z, R(0)};
+          if (((want >> q) & 1u) && box_overlap(lo, hi, cl, ch)) mk |= 1u << q;
+        }
+        if (!__ballot(mk != 0u)) continue;
+        const int2 rg = crange_s[k];
+        const int end = rg.x + rg.y;
+        // four staged boxes per pass, loaded together (indexes clamped into
+        // the cluster's run; a clamped repeat is not tested)
+        for (int e = rg.x; e < end; e += 4) {
+          V4<R> sl[4], sh[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const int ej = min(e + j, end - 1);
-              sl[j] = lo_s[ej]; sh[j] = hi_s[ej];
-            }
+          for (int j = 0; j < 4; ++j) {
+            const int ej = min(e + j, end - 1);
+            sl[j] = lo_s[ej]; sh[j] = hi_s[ej];
+          }
+#pragma unroll
+          for (int q = 0; q < XQ; ++q) {
+            if (!((mk >> q) & 1u)) continue;
+            const V4<R> lo{xl[q].x, xl[q].y, xl[q].z, R(0)}, hi{xh[q].x, xh[q].y, xh[q].z, R(0)};
             unsigned hit = 0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) hit |= (box_overlap(lo, hi, sl[j], sh[j]) & (e + j < end)) ? 1u << j : 0u;
             if (hit) {
-              hitq = true;
-              if (!count_all) break;
-              cnt += __popc(hit);
+              flg |= 1u << q;
+              if (count_all) cnt += __popc(hit);
+              else mk &= ~(1u << q);                      // decided: no further test
             }
           }
+          if (!count_all && !__ballot(mk != 0u)) break;
         }
-        if (hitq) flg |= 1u << q;
       }
       wave_sync_lds();
       if (ybase >= ntY) break;

@@ -989,22 +989,48 @@ __device__ bool color_small_bits(const Dev<R>& d, int u, BitsColorLds& s) {
     }
   }
   if (rnd >= MAX_ROUNDS) {
-    // leftovers get fresh colours max+1, ... in increasing contact order per island
+    // leftovers get fresh colours max+1, ... in increasing contact order per
+    // island.  The wavefront walks the (few) leftover contacts together; for
+    // each, every lane checks the unit's contacts it holds, and a wavefront max
+    // and sum give the island's largest colour (kept or tentative) and the
+    // leftover's rank.  (A walk over the whole unit by every leftover's lane
+    // made the wavefront run K iterations whenever one contact was left.)
     __syncthreads();
-    int fresh[SMALL_UNIT / 64];
+    constexpr int QS = SMALL_UNIT / 64;
+    int fresh[QS], il_q[QS], col_q[QS], key_q[QS];
+    bool left_q[QS];
 #pragma unroll
-    for (int q = 0; q < SMALL_UNIT / 64; ++q) {
+    for (int q = 0; q < QS; ++q) {
       const int i = lane + 64 * q;
+      const bool in = i < K;
       fresh[q] = -1;
-      if (i >= K || s.done[i]) continue;
-      const int il = s.isl[i], ki = s.key[i];
-      int mx = -1, rank = 0;
-      for (int j = 0; j < K; ++j) {
-        if (s.isl[j] != il) continue;
-        mx = max(mx, (int)s.col[j]);
-        if (j != i && !s.done[j] && s.key[j] < ki) ++rank;
+      il_q[q] = in ? s.isl[i] : -1;
+      col_q[q] = in ? (int)s.col[i] : -1;
+      key_q[q] = in ? s.key[i] : 0;
+      left_q[q] = in && !s.done[i];
+    }
+#pragma unroll
+    for (int q = 0; q < QS; ++q) {
+      unsigned long long m = __ballot(left_q[q]);
+      while (m) {                                   // (uniform over the wavefront)
+        const int b = __ffsll(m) - 1;
+        m &= m - 1;
+        const int i = 64 * q + b;
+        const int il = s.isl[i], ki = s.key[i];
+        int mx = -1, rank = 0;
+#pragma unroll
+        for (int r = 0; r < QS; ++r) {
+          if (il_q[r] != il) continue;              // (-1: no contact)
+          mx = max(mx, col_q[r]);
+          if (left_q[r] && key_q[r] < ki) ++rank;   // keys are distinct: i itself is not counted
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          mx = max(mx, __shfl_xor(mx, o, 64));
+          rank += __shfl_xor(rank, o, 64);
+        }
+        if (lane == b) fresh[q] = mx + 1 + rank;
       }
-      fresh[q] = mx + 1 + rank;
     }
     __syncthreads();
 #pragma unroll

@@ -570,7 +570,7 @@ template <typename R> Dev<R> make_dev(rbh_ctx* c, double dt, int flags, bool mul
 }
 
 // Look-back state of the step's multi-tile scans: SCAN_SITES regions (one per
-// scan of the launch sequence, which has 11), each a ticket word and one status
+// scan of the launch sequence, which has at most 11), each a ticket word and one status
 // word per look-back tile of the largest capacity.
 constexpr int SCAN_SITES = 16;
 template <typename T> int scan_tiles(int cap) { return (cap + LbTile<T>::TILE - 1) / LbTile<T>::TILE + 1; }
